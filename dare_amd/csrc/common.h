@@ -1,0 +1,102 @@
+// Shared internals of libdare_hip: context, error handling, device geometry, reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "dare_hip.h"
+
+namespace dare_hip {
+
+void set_error(const std::string& msg);
+
+#define DARE_HIP_TRY(expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            ::dare_hip::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
+            return DARE_ERR_HIP;                                                             \
+        }                                                                                    \
+    } while (0)
+
+#define DARE_CHECK_ARG(cond, msg)                                                            \
+    do {                                                                                     \
+        if (!(cond)) {                                                                       \
+            ::dare_hip::set_error(msg);                                                      \
+            return DARE_ERR_INVALID;                                                         \
+        }                                                                                    \
+    } while (0)
+
+#define DARE_LAUNCH_CHECK()                                                                  \
+    do {                                                                                     \
+        hipError_t e_ = hipGetLastError();                                                   \
+        if (e_ != hipSuccess) {                                                              \
+            ::dare_hip::set_error(std::string("kernel launch: ") + hipGetErrorString(e_));  \
+            return DARE_ERR_HIP;                                                             \
+        }                                                                                    \
+    } while (0)
+
+constexpr int kMaxComp = 8;
+constexpr int kReduceBlocks = 2048;     // partial sums per one-pass grid reduction (= max grid)
+constexpr int kMaxDots = 8;
+
+// Device-side reduction workspace: block partials + arrival counter (reset by the last block).
+struct ReduceWs {
+    double* partial;      // kReduceBlocks * kMaxDots
+    unsigned int* count;  // one counter
+};
+
+}  // namespace dare_hip
+
+struct dare_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dare_hip::ReduceWs ws{nullptr, nullptr};
+    double* dot_out = nullptr;     // device results of dare_dot
+    double* host_pinned = nullptr; // 64 doubles of pinned host memory for scalar readback
+    int num_cu = 256;
+    void* solve_buf = nullptr;     // Krylov work vectors, grown on demand
+    size_t solve_bytes = 0;
+    void* solve_state = nullptr;   // device-resident solver scalars
+};
+
+struct dare_csr {
+    dare_ctx* ctx = nullptr;
+    int64_t nrows = 0, nnz = 0, cap_rows = 0, cap_nnz = 0;
+    int32_t* row_ptr = nullptr;
+    int32_t* col = nullptr;
+    double* val = nullptr;
+    float* val32 = nullptr;      // FP32 copy of the values (mixed precision)
+    int64_t cap_val32 = 0;
+    int stencil_structured = 0;  // 1 if built by dare_csr_build_structure
+};
+
+namespace dare_hip {
+
+// Geometry of a grid, computed on the host with the reference's formulas
+// (CartesianMesh.inl:75-83) and passed to kernels by value.
+struct Geo {
+    int dim, N, g;
+    int n[3], np[3];
+    int per[3];
+    int any_periodic;
+    long long sl[3], si[3];
+    double dnr[3], A[3], V;
+    long long ncell_int, ncell_pad;
+};
+
+int make_geo(const dare_grid* gr, Geo* G);
+
+inline unsigned int ceil_div(long long a, long long b) { return static_cast<unsigned int>((a + b - 1) / b); }
+
+// Wave64 reduction of a double with DPP-free shuffles.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+}  // namespace dare_hip

@@ -1,0 +1,192 @@
+/*
+ * dare_hip.h — C-ABI of the MI355X (gfx950) finite-volume hot path of DaRe.
+ *
+ * Plain pointers and sizes only; every device pointer is HBM memory of the context's
+ * device, every call is asynchronous on the context's stream unless it returns host data.
+ * All functions return 0 (DARE_OK) or a negative DARE_ERR_* code; dare_last_error() gives
+ * a thread-local message. No C++ exception crosses this boundary.
+ *
+ * Each entry point names the reference interface it replaces (hpc-dave/dare v0.3.1,
+ * file:line). The reference is a C++ template library, not an FFI; the C++ host mirror
+ * in dare_amd/include/dare/ keeps its class names and call surface and calls these
+ * functions.
+ */
+#ifndef DARE_HIP_H_
+#define DARE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DARE_OK 0
+#define DARE_ERR_INVALID (-1)      /* bad argument or shape */
+#define DARE_ERR_OOM (-2)          /* device allocation failed */
+#define DARE_ERR_HIP (-3)          /* HIP runtime error */
+#define DARE_ERR_NOT_CONVERGED (-4)
+#define DARE_ERR_BREAKDOWN (-5)    /* Krylov breakdown (rho, gamma, mu or tau == 0) */
+#define DARE_ERR_UNSUPPORTED (-6)
+
+/* Limiter ids. Equations/FluxLimiter.h:41,73,105,145 (+ van Leer, closed form). */
+#define DARE_LIMITER_CDS 0
+#define DARE_LIMITER_UPWIND 1
+#define DARE_LIMITER_VANALBADA 2
+#define DARE_LIMITER_MINMOD 3
+#define DARE_LIMITER_VANLEER 4
+
+/* Ghost fill modes per boundary face (W,E,S,N,B,T). */
+#define DARE_GHOST_LEAVE 0
+#define DARE_GHOST_PERIODIC 1   /* HaloBuffer self-exchange of a periodic single-rank grid */
+#define DARE_GHOST_DIRICHLET 2  /* 2 v - phi(mirror) */
+#define DARE_GHOST_NEUMANN 3    /* phi(mirror) */
+#define DARE_GHOST_CONSTANT 4   /* v */
+
+/* Boundary edits applied during assembly (counter_diffusion2D.cpp:125-161 pattern). */
+#define DARE_BC_NONE 0
+#define DARE_BC_DIRICHLET 1
+#define DARE_BC_NEUMANN 2
+
+typedef struct dare_ctx dare_ctx;
+typedef struct dare_csr dare_csr;
+
+/* Grid of one single-rank subdomain. Replaces Grid::Cartesian<Dim> + CartesianRepresentation
+ * (Grid/Cartesian/CartesianMesh.inl:35-112, CartesianRepresentation.inl:32-112). Storage of a
+ * field: data[local_ordinal * ncomp + c], local ordinal over the ghost-padded box, last axis
+ * fastest (Data/GridVector.inl:37-41,69-76). */
+typedef struct {
+    int32_t dim;          /* 1, 2 or 3 */
+    int32_t ncomp;        /* components per cell (N), 1..8 */
+    int32_t ghost;        /* ghost layers per side */
+    int32_t n[3];         /* internal resolution; n[d] = 1 for d >= dim */
+    int32_t periodic[3];  /* non-zero: periodic along axis d */
+    double size[3];       /* domain lengths */
+} dare_grid;
+
+/* Terms of the per-cell assembly functor, evaluated on the device. Replaces the user lambda
+ * handed to Trilinos<SC>::Build (MatrixSystem/Trilinos.inl:50-60,308-328) for the operator
+ *   mb = ddt(phi); mb -= div(D_f * grad(mb)); mb += div(tvd * phi); rhs += V s; reaction; BCs.
+ * Pointers are device pointers (or NULL). */
+typedef struct {
+    double dt;               /* > 0: implicit transient term (Equations/DDT.inl:41-81) */
+    double diff_const;       /* constant diffusivity, used when diff_cell == NULL */
+    const double* diff_cell; /* per-cell diffusivity, padded layout, face value = harmonic mean */
+    double diff_comp[8];     /* per-component constant diffusivity when use_diff_comp != 0 */
+    int32_t use_diff_comp;
+    int32_t conv_limiter;    /* < 0: none, else implicit TVD convection (TVD_Cartesian.inl:185-261) */
+    const double* vel[3];    /* face velocities (face arrays, 1 component) */
+    const double* source;    /* per-cell source (padded layout), rhs += V * s */
+    const double* react;     /* ncomp == 2: per-cell 2x2 block [a b; c d], internal order */
+    int32_t bc_type[6];      /* W,E,S,N,B,T: DARE_BC_* */
+    double bc_value[6];
+} dare_terms;
+
+/* Solver parameters. Replaces the Teuchos::ParameterList keys "Convergence Tolerance",
+ * "Maximum Iterations" of BiCGStab2 (MatrixSystem/BiCGStab2.inl:95-111). */
+typedef struct {
+    double tol;              /* convergence tolerance */
+    int32_t tol_kind;        /* 0: ||r|| <= tol (reference RNORM), 1: ||r|| <= tol * ||b|| */
+    int32_t max_it;          /* maximum BiCGStab2 passes */
+    double inner_tol;        /* mixed precision: inner FP32 relative reduction per refinement */
+    int32_t max_refine;      /* mixed precision: maximum refinement steps */
+    int32_t inner_max_it;    /* mixed precision: passes per inner solve */
+    double* history;         /* optional host array, residual norm after every pass */
+    int32_t history_cap;
+} dare_solver_params;
+
+typedef struct {
+    int32_t converged;       /* 1 if the stopping test held */
+    int32_t iterations;      /* BiCGStab2 passes (BiCGStab2.inl:578) — total over refinements */
+    int32_t status;          /* 0 or DARE_ERR_BREAKDOWN */
+    int32_t refinements;     /* mixed precision: outer steps */
+    double residual;         /* last recursive residual norm (FP64 true residual for mixed) */
+    double bnorm;            /* ||b|| */
+    int32_t history_len;
+} dare_solver_result;
+
+/* ---- context, memory (replaces ScopeGuard/ExecutionManager set-up, ScopeGuard.cpp:30-151) */
+int dare_ctx_create(int32_t device, dare_ctx** out);
+int dare_ctx_destroy(dare_ctx* ctx);
+int dare_ctx_set_stream(dare_ctx* ctx, void* hip_stream);   /* NULL: the null stream */
+void* dare_ctx_get_stream(dare_ctx* ctx);
+int dare_ctx_synchronize(dare_ctx* ctx);
+const char* dare_last_error(void);
+int32_t dare_abi_version(void);
+
+int dare_malloc(dare_ctx* ctx, size_t bytes, void** out);
+int dare_free(dare_ctx* ctx, void* p);
+int dare_memcpy_h2d(dare_ctx* ctx, void* dst, const void* src, size_t bytes);
+int dare_memcpy_d2h(dare_ctx* ctx, void* dst, const void* src, size_t bytes);
+int dare_memcpy_d2d(dare_ctx* ctx, void* dst, const void* src, size_t bytes);
+int dare_fill_f64(dare_ctx* ctx, double* p, int64_t n, double v);
+
+/* ---- stencils (device pointers, FP64) ------------------------------------------------- */
+/* Ghost fill. Replaces Field::ExchangeHaloCells -> HaloBuffer::Exchange on one rank
+ * (Data/Field.inl:101-103, MPI/HaloBuffer.inl:196-208, SingleHaloBuffer.inl:95-127). */
+int dare_ghost_fill(dare_ctx* ctx, const dare_grid* g, const int32_t* mode6,
+                    const double* value6, double* phi);
+/* Face gradient of every face. Replaces Gradient::operator()(GridVector)
+ * (Grid/Cartesian/Gradient_Cartesian.inl:62-96) evaluated over the field. */
+int dare_grad_faces(dare_ctx* ctx, const dare_grid* g, const double* phi,
+                    double* fx, double* fy, double* fz);
+/* Value divergence of face arrays. Replaces Divergence::ApplyDivergence(FaceValueStencil)
+ * (Grid/Cartesian/Divergence_Cartesian.inl:211-235). Writes internal cells of out. */
+int dare_div_faces(dare_ctx* ctx, const dare_grid* g, const double* fx, const double* fy,
+                   const double* fz, double* out);
+/* TVD face flux phi_f u_f. Replaces TVD::Interpolate(GridVector) times the face velocity
+ * (TVD_Cartesian.inl:27-38,122-183; Stencils_Cartesian.inl:652-681). */
+int dare_tvd_face_flux(dare_ctx* ctx, const dare_grid* g, int32_t limiter, const double* phi,
+                       const double* ux, const double* uy, const double* uz,
+                       double* fx, double* fy, double* fz);
+/* Fused explicit TVD step phi_out = phi - (dt/V) div(phi_f u_f) on internal cells. */
+int dare_tvd_explicit_step(dare_ctx* ctx, const dare_grid* g, int32_t limiter,
+                           const double* phi, const double* ux, const double* uy,
+                           const double* uz, double dt, double* phi_out);
+
+/* ---- matrix assembly ------------------------------------------------------------------- */
+int dare_csr_create(dare_ctx* ctx, dare_csr** out);
+int dare_csr_destroy(dare_csr* csr);
+/* Row pointers and sorted columns of the per-cell operator. Replaces the structure part of
+ * Trilinos<SC>::BuildNew + MatrixBlock::Finalize + fillComplete (Trilinos.inl:287-385,
+ * MatrixBlock_Cartesian.inl:128-240). */
+int dare_csr_build_structure(dare_ctx* ctx, dare_csr* csr, const dare_grid* g,
+                             const dare_terms* t);
+/* Values, right-hand side and initial guess (x0 = phi_old). Replaces the functor loop of
+ * Trilinos.inl:308-328 and the value half of :372-383 (BuildReplace semantics, :389-457). */
+int dare_csr_fill_values(dare_ctx* ctx, dare_csr* csr, const dare_grid* g,
+                         const dare_terms* t, const double* phi_old, double* rhs, double* x0);
+/* Host-assembled CSR (generic functor path). Replaces insertGlobalValues/fillComplete. */
+int dare_csr_upload_host(dare_ctx* ctx, dare_csr* csr, int64_t nrows, int64_t nnz,
+                         const int32_t* row_ptr, const int32_t* col, const double* val);
+int dare_csr_info(const dare_csr* csr, int64_t* nrows, int64_t* nnz, const int32_t** row_ptr,
+                  const int32_t** col, const double** val);
+int dare_csr_download(dare_ctx* ctx, const dare_csr* csr, int32_t* row_ptr, int32_t* col,
+                      double* val);
+/* Solution back to the field. Replaces Trilinos::CopyTo (Trilinos.inl:168-177). */
+int dare_copy_to_field(dare_ctx* ctx, const dare_grid* g, const double* x, double* phi);
+
+/* ---- Krylov primitives (the Tpetra calls of BiCGStab2.inl: apply, dot, norm2, update) -- */
+int dare_spmv(dare_ctx* ctx, const dare_csr* csr, const double* x, double* y);
+int dare_dot(dare_ctx* ctx, int64_t n, const double* a, const double* b, double* out_host);
+/* Fused y = A x and <y, w> (the SpMV+dot launch of every BiCGStab2 step 6/9/13). out_host may
+ * be NULL (no readback, fully asynchronous). */
+int dare_spmv_dot(dare_ctx* ctx, const dare_csr* csr, const double* x, double* y, const double* w,
+                  double* out_host);
+int dare_axpby(dare_ctx* ctx, int64_t n, double a, const double* x, double b, double* y);
+
+/* ---- solvers ------------------------------------------------------------------------- */
+/* FP64 BiCGStab(2). Replaces TrilinosSolver::Solve(BumbleBee, "BICGSTAB2", ...) ->
+ * BiCGStab2::Solve (TrilinosSolver.inl:71-93,150-152; BiCGStab2.inl:167-582). x holds x0. */
+int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* csr, const double* b, double* x,
+                         const dare_solver_params* p, dare_solver_result* res);
+/* FP32 inner BiCGStab(2) inside FP64 iterative refinement; r = b - A x, the update of x and
+ * the stopping test in FP64. No reference counterpart (north-star precision form). */
+int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* csr, const double* b, double* x,
+                            const dare_solver_params* p, dare_solver_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  // DARE_HIP_H_

@@ -1,0 +1,271 @@
+"""GPU parity: every HIP kernel, called through the C-ABI, against the CPU oracle.
+
+Stencils, ghost fill, TVD and assembly are bit-exact (the kernels keep the reference's
+operation order); the CSR structure is bit-exact; SpMV is bit-exact (same per-row summation
+order); BiCGStab2 solutions agree within 1e-12 relative (dot products are reduced in a
+different order than the oracle's blocked sum).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import dare_amd as da
+import oracle as orc
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def host(t, ctx=None):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def grids(g_orc):
+    """dare_amd.Grid with the same fields as an oracle grid."""
+    g = da.Grid()
+    ctypes.memmove(ctypes.byref(g), ctypes.byref(g_orc), ctypes.sizeof(g))
+    return g
+
+
+CASES = [
+    dict(n=[12, 13, 14], ncomp=1, periodic=None),
+    dict(n=[9, 10, 11], ncomp=3, periodic=[1, 0, 1]),
+    dict(n=[17, 15], ncomp=2, periodic=[1, 1]),
+    dict(n=[23], ncomp=1, periodic=[1]),
+]
+
+
+def make_field(rng, g, flat_frac=0.3):
+    npad = orc.lib().orc_num_cells_padded(ctypes.byref(g)) * g.ncomp
+    phi = rng.normal(size=npad)
+    mask = rng.uniform(size=npad) < flat_frac     # flat patches: 0/0 ratios, inf ratios
+    phi[mask] = 1.0
+    return phi
+
+
+def face_vel(rng, g):
+    out = []
+    for d in range(g.dim):
+        u = rng.uniform(-1, 1, size=orc.lib().orc_num_faces(ctypes.byref(g), d))
+        u[rng.uniform(size=u.size) < 0.05] = 0.0
+        out.append(u)
+    return out
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_ghost_fill_matches_oracle(ctx, rng, case):
+    c = CASES[case]
+    go = orc.grid(c["n"], ghost=2, ncomp=c["ncomp"], periodic=c["periodic"])
+    dim = go.dim
+    modes, values = [], []
+    for d in range(dim):
+        for side in range(2):
+            if c["periodic"] and c["periodic"][d]:
+                modes.append(1)
+            else:
+                modes.append([2, 3, 4, 0][(d + side) % 4])
+            values.append(float(rng.normal()))
+    phi = make_field(rng, go, 0.0)
+    exp = orc.ghost_fill(go, phi, modes, values + [0.0] * (6 - len(values)))
+    t = dev(phi)
+    da.ghost_fill(ctx, grids(go), t, modes, values + [0.0] * (6 - len(values)))
+    assert bits_equal(host(t), exp)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_grad_and_div_faces_match_oracle(ctx, rng, case):
+    import torch
+    c = CASES[case]
+    go = orc.grid(c["n"], size=[0.7, 1.3, 2.1][:len(c["n"])], ghost=2, ncomp=c["ncomp"])
+    phi = make_field(rng, go)
+    exp = orc.grad_faces(go, phi)
+    g = grids(go)
+    fd = [torch.zeros(e.size, dtype=torch.float64, device="cuda") for e in exp]
+    da.grad_faces(ctx, g, dev(phi), fd)
+    for d in range(go.dim):
+        assert bits_equal(host(fd[d]), exp[d]), d
+    faces = [rng.normal(size=e.size) for e in exp]
+    exp_div = orc.div_faces(go, faces)
+    out = torch.zeros(exp_div.size, dtype=torch.float64, device="cuda")
+    da.div_faces(ctx, g, [dev(f) for f in faces], out)
+    assert bits_equal(host(out), exp_div)
+
+
+@pytest.mark.parametrize("lim", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_tvd_flux_and_step_match_oracle(ctx, rng, lim, case):
+    import torch
+    c = CASES[case]
+    go = orc.grid(c["n"], size=[1.0, 0.8, 1.1][:len(c["n"])], ghost=2, ncomp=c["ncomp"])
+    phi = make_field(rng, go)
+    vel = face_vel(rng, go)
+    exp = orc.tvd_face_flux(go, lim, phi, vel)
+    g = grids(go)
+    fd = [torch.zeros(e.size, dtype=torch.float64, device="cuda") for e in exp]
+    vd = [dev(v) for v in vel]
+    pd = dev(phi)
+    da.tvd_face_flux(ctx, g, lim, pd, vd, fd)
+    for d in range(go.dim):
+        assert bits_equal(host(fd[d]), exp[d]), (lim, d)
+    dt = 1e-3
+    exp_step = orc.tvd_explicit_step(go, lim, phi, vel, dt)
+    out = pd.clone()
+    da.tvd_explicit_step(ctx, g, lim, pd, vd, dt, out)
+    assert bits_equal(host(out), exp_step)
+
+
+ASSEMBLY = ["diff_dirichlet", "var_conv_src", "periodic_conv", "react2", "neumann_2d", "steady_1d"]
+
+
+def assembly_case(rng, name):
+    if name == "diff_dirichlet":
+        go = orc.grid([10, 11, 12], size=[1, 1, 1], ghost=2)
+        kw = dict(dt=1e-2, diff=1e-3, bc_type=[1] * 6, bc_value=[0.0] * 6)
+    elif name == "var_conv_src":
+        go = orc.grid([9, 8, 10], size=[1, 0.9, 1.2], ghost=2)
+        kw = dict(dt=5e-3, diff_cell=True, limiter=4, source=True, bc_type=[1, 2, 1, 2, 1, 1],
+                  bc_value=[0.3, 0, -1.0, 0, 0.5, 0.25])
+    elif name == "periodic_conv":
+        go = orc.grid([8, 9, 7], ghost=2, periodic=[1, 1, 1])
+        kw = dict(dt=5e-3, diff=2e-3, limiter=3)
+    elif name == "react2":
+        go = orc.grid([7, 8, 9], ghost=2, ncomp=2)
+        kw = dict(dt=0.1, diff_comp=[1e-3, 5e-3], react=True, bc_type=[1] * 6,
+                  bc_value=[0.0] * 6)
+    elif name == "neumann_2d":
+        go = orc.grid([13, 11], ghost=2, ncomp=3)
+        kw = dict(dt=0.2, diff=0.7, limiter=2, bc_type=[2, 1, 1, 2], bc_value=[0, 1.0, 0.5, 0])
+    else:
+        go = orc.grid([31], ghost=2)
+        kw = dict(dt=0.0, diff=1.0, bc_type=[1, 1], bc_value=[0.0, 1.0])
+    npad = orc.lib().orc_num_cells_padded(ctypes.byref(go))
+    arrays = {}
+    if kw.pop("diff_cell", False):
+        arrays["diff_cell"] = np.exp(rng.normal(np.log(1e-3), 1.0, size=npad * go.ncomp))
+    if kw.pop("source", False):
+        arrays["source"] = rng.uniform(size=npad * go.ncomp)
+    if kw.pop("react", False):
+        ncell = orc.lib().orc_num_cells_internal(ctypes.byref(go))
+        k1, k2 = rng.uniform(0.1, 10, ncell), rng.uniform(0.1, 10, ncell)
+        arrays["react"] = np.stack([-k1, k2, k1, -k2], 1).ravel().copy()
+    if kw.get("limiter", -1) >= 0:
+        arrays["vel"] = face_vel(rng, go)
+    phi = make_field(rng, go, 0.2)
+    return go, kw, arrays, phi
+
+
+@pytest.mark.parametrize("name", ASSEMBLY)
+def test_assembly_matches_oracle(ctx, rng, name):
+    import torch
+    go, kw, arrays, phi = assembly_case(rng, name)
+    to = orc.terms(**kw, **arrays)
+    rp, col, val, rhs, x0 = orc.assemble(go, to, phi)
+    g = grids(go)
+    darr = {k: ([dev(v) for v in a] if isinstance(a, list) else dev(a)) for k, a in arrays.items()}
+    td = da.Terms.make(**kw, **darr)
+    A = da.Csr(ctx)
+    da.build_structure(ctx, A, g, td)
+    drhs = torch.zeros(len(rhs), dtype=torch.float64, device="cuda")
+    dx0 = torch.zeros(len(rhs), dtype=torch.float64, device="cuda")
+    da.fill_values(ctx, A, g, td, dev(phi), drhs, dx0)
+    ctx.synchronize()
+    grp, gcol, gval = A.download()
+    assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
+    assert bits_equal(gval, val)
+    assert bits_equal(host(drhs), rhs) and bits_equal(host(dx0), x0)
+    # values are refilled in place on the same structure (BuildReplace semantics)
+    phi2 = make_field(rng, go, 0.2)
+    rp2, col2, val2, rhs2, _ = orc.assemble(go, to, phi2)
+    da.fill_values(ctx, A, g, td, dev(phi2), drhs, dx0)
+    ctx.synchronize()
+    assert bits_equal(A.download()[2], val2) and bits_equal(host(drhs), rhs2)
+    # SpMV: identical per-row summation order -> bit-exact
+    x = rng.normal(size=len(rhs))
+    y = torch.zeros(len(rhs), dtype=torch.float64, device="cuda")
+    da.spmv(ctx, A, dev(x), y)
+    assert bits_equal(host(y), orc.spmv(rp2, col2, val2, x))
+    A.close()
+
+
+@pytest.mark.parametrize("name", ["diff_dirichlet", "var_conv_src", "periodic_conv", "react2"])
+def test_bicgstab2_matches_oracle(ctx, rng, name):
+    import torch
+    go, kw, arrays, phi = assembly_case(rng, name)
+    to = orc.terms(**kw, **arrays)
+    rp, col, val, rhs, x0 = orc.assemble(go, to, phi)
+    tol = 1e-12 * np.linalg.norm(rhs)
+    xo, ok, it, res, hist = orc.bicgstab2(rp, col, val, rhs, x0, tol, 200, 200)
+    assert ok
+    A = da.Csr(ctx)
+    da.upload_host_csr(ctx, A, rp, col, val)
+    xd = dev(x0)
+    r = da.bicgstab2(ctx, A, dev(rhs), xd, tol=tol, tol_kind=0, max_it=200, history=200)
+    assert r["converged"] == 1 and r["status"] == 0
+    assert abs(r["iterations"] - it) <= 1
+    xg = host(xd)
+    assert np.max(np.abs(xg - xo)) <= 1e-12 * np.max(np.abs(xo)) + tol
+    k = min(len(hist), len(r["history"]), 3)
+    for a, b in zip(r["history"][:k], hist[:k]):
+        assert abs(a - b) <= 1e-9 * abs(b)
+    A.close()
+
+
+def test_bicgstab2_pseudo2d_laplace_closed_form(ctx):
+    from test_oracle import csr_from_dense, laplace_pseudo2d
+    Ad, b, x0 = laplace_pseudo2d()
+    rp, col, val = csr_from_dense(Ad)
+    A = da.Csr(ctx)
+    da.upload_host_csr(ctx, A, rp, col, val)
+    xd = dev(x0)
+    r = da.bicgstab2(ctx, A, dev(b), xd, tol=1e-12, max_it=5000)
+    assert r["converged"] == 1
+    nodes = np.repeat(np.arange(23), 4)
+    assert np.max(np.abs(host(xd) - (nodes + 0.5) / 23)) < 1e-8
+
+
+def test_bicgstab2_breakdown_and_limits(ctx):
+    import torch
+    from test_oracle import csr_from_dense, laplace_pseudo2d
+    Ad, b, x0 = laplace_pseudo2d()
+    rp, col, val = csr_from_dense(Ad)
+    A = da.Csr(ctx)
+    da.upload_host_csr(ctx, A, rp, col, val)
+    xd = dev(x0)
+    r = da.bicgstab2(ctx, A, dev(b), xd, tol=1e3)
+    assert r["converged"] == 1 and r["iterations"] == 0
+    z = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+    r = da.bicgstab2(ctx, A, z, z.clone(), tol=1e-12)
+    assert r["converged"] == 1 and r["iterations"] == 0
+    xd = dev(x0)
+    r = da.bicgstab2(ctx, A, dev(b), xd, tol=1e-30, max_it=3)
+    assert r["converged"] == 0 and r["iterations"] == 4
+    # zero operator: gamma == 0 breakdown is reported, not raised
+    Z = da.Csr(ctx)
+    da.upload_host_csr(ctx, Z, rp, col, np.zeros_like(val))
+    r = da.bicgstab2(ctx, Z, dev(b), dev(x0), tol=1e-12)
+    assert r["converged"] == 0 and r["status"] == da.DARE_ERR_BREAKDOWN
+
+
+@pytest.mark.parametrize("name", ["diff_dirichlet", "var_conv_src"])
+def test_mixed_refinement_reaches_fp64_residual(ctx, rng, name):
+    go, kw, arrays, phi = assembly_case(rng, name)
+    to = orc.terms(**kw, **arrays)
+    rp, col, val, rhs, x0 = orc.assemble(go, to, phi)
+    A = da.Csr(ctx)
+    da.upload_host_csr(ctx, A, rp, col, val)
+    xd = dev(x0)
+    r = da.refine_mixed(ctx, A, dev(rhs), xd, tol=1e-10, tol_kind=1, inner_tol=1e-4)
+    assert r["converged"] == 1
+    x = host(xd)
+    true_rel = np.linalg.norm(rhs - orc.spmv(rp, col, val, x)) / np.linalg.norm(rhs)
+    assert true_rel <= 1e-10
+    xo, ok, *_ = orc.bicgstab2(rp, col, val, rhs, x0, 1e-13 * np.linalg.norm(rhs), 300)
+    assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
