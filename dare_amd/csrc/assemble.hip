@@ -466,15 +466,15 @@ extern "C" {
 int dare_csr_create(dare_ctx* ctx, dare_csr** out) {
     DARE_CHECK_ARG(ctx && out, "NULL argument");
     dare_csr* A = new dare_csr();
-    A->ctx = ctx;
+    A->device = ctx->device;
     *out = A;
     return DARE_OK;
 }
 
 int dare_csr_destroy(dare_csr* A) {
     if (!A) return DARE_OK;
-    hipSetDevice(A->ctx->device);
-    hipStreamSynchronize(A->ctx->stream);
+    // hipFree synchronizes with outstanding work on the device before releasing memory
+    hipSetDevice(A->device);
     if (A->row_ptr) hipFree(A->row_ptr);
     if (A->col) hipFree(A->col);
     if (A->val) hipFree(A->val);

@@ -64,7 +64,7 @@ struct dare_ctx {
 };
 
 struct dare_csr {
-    dare_ctx* ctx = nullptr;
+    int device = 0;              // owner device; the creating context may be destroyed first
     int64_t nrows = 0, nnz = 0, cap_rows = 0, cap_nnz = 0;
     int32_t* row_ptr = nullptr;
     int32_t* col = nullptr;

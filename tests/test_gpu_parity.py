@@ -212,9 +212,10 @@ def test_bicgstab2_matches_oracle(ctx, rng, name):
     assert abs(r["iterations"] - it) <= 1
     xg = host(xd)
     assert np.max(np.abs(xg - xo)) <= 1e-12 * np.max(np.abs(xo)) + tol
+    # residual history of the first passes: same method, dots reduced in another order
     k = min(len(hist), len(r["history"]), 3)
     for a, b in zip(r["history"][:k], hist[:k]):
-        assert abs(a - b) <= 1e-9 * abs(b)
+        assert abs(a - b) <= 1e-8 * abs(b) + 1e-14 * float(np.max(hist))
     A.close()
 
 
