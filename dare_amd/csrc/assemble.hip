@@ -31,6 +31,7 @@ struct Terms {
     const double* vel[3];
     const double* source;
     const double* react;
+    int cross;                 // ncomp == 2 with a reaction block: one cross-component entry
     int bc_type[6];
     double bc_value[6];
     long long fs[3][3];        // face array strides
@@ -83,14 +84,22 @@ __device__ inline long long nb_col(const Geo& G, const int q[3], int pos) {
 
 // Builds the (unsorted) entries of one row in Finalize order: explicit cross entry first, then
 // W,S,B,C,T,N,E for set positions. Then stable-sorts by column and merges duplicates.
-template <bool VALUES>
+// MODE 0: structure only; 1: evaluate the functor; 2: values taken from the diagonal-view
+// slots (slot_in) written earlier by mode 1. SORT = false stops after the values (the
+// diagonal-view fill needs no column order).
+enum { kRowStructure = 0, kRowValues = 1, kRowFromSlots = 2 };
+constexpr int kPosOfSlot[7] = {1, 3, 5, 0, 6, 4, 2};
+
+template <int MODE, bool SORT>
 __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, long long node,
-                           int* cols, double* vals, double* rhs_out, double* x0_out) {
+                           int* cols, double* vals, double* rhs_out, double* x0_out,
+                           double* slotv = nullptr, const double* slot_in = nullptr) {
+    constexpr bool VALUES = MODE == kRowValues;
     double nbv[7] = {0., 0., 0., 0., 0., 0., 0.};
     double rhs = 0.;
     bool removed[7] = {false, false, false, false, false, false, false};
     double cross = 0.;
-    const bool has_cross = T.react != nullptr && G.N == 2;
+    const bool has_cross = T.cross != 0;
     long long lp = 0;
     for (int d = 0; d < G.dim; d++) lp += G.sl[d] * (q[d] + G.g);
     const int N = G.N;
@@ -201,7 +210,19 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
         }
         *rhs_out = rhs;
         *x0_out = phi[lp * N + c];
+        if (slotv) {      // diagonal view: W,S,B,C,T,N,E,cross; removed entries hold 0
+            for (int sl = 0; sl < 7; sl++) {
+                const int pos = kPosOfSlot[sl];
+                slotv[sl] = (pos <= 2 * G.dim && !removed[pos]) ? nbv[pos] : 0.;
+            }
+            slotv[7] = cross;
+        }
     }
+    if (MODE == kRowFromSlots) {
+        for (int sl = 0; sl < 7; sl++) nbv[kPosOfSlot[sl]] = slot_in[sl];
+        cross = has_cross ? slot_in[7] : 0.;
+    }
+    if (!SORT) return 0;
 
     // Finalize order, then stable sort by column + merge of equal columns
     long long kc[kMaxEntries];
@@ -243,8 +264,8 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
         }
     }
     for (int i = 0; i < w; i++) {
-        cols[i] = static_cast<int>(kc[i]);
-        if (VALUES) vals[i] = kv[i];
+        if (cols) cols[i] = static_cast<int>(kc[i]);
+        if (MODE != kRowStructure) vals[i] = kv[i];
     }
     return w;
 }
@@ -269,7 +290,7 @@ __device__ inline int row_count(const Geo& G, const Terms& T, long long row) {
     long long node;
     row_to_cell(G, row, q, &c, &node);
     int cols[kMaxEntries];
-    return row_entries<false>(G, T, q, c, node, cols, nullptr, nullptr, nullptr);
+    return row_entries<kRowStructure, true>(G, T, q, c, node, cols, nullptr, nullptr, nullptr);
 }
 
 // block-wide inclusive scan of one int per thread (wave64 shuffles + LDS)
@@ -349,7 +370,7 @@ __global__ __launch_bounds__(kScanThreads) void k_write_structure(Geo G, Terms T
         long long node;
         row_to_cell(G, row, q, &c, &node);
         int cols[kMaxEntries];
-        const int k = row_entries<false>(G, T, q, c, node, cols, nullptr, nullptr, nullptr);
+        const int k = row_entries<kRowStructure, true>(G, T, q, c, node, cols, nullptr, nullptr, nullptr);
         for (int j = 0; j < k; j++) col[off + j] = cols[j];
         off += k;
     }
@@ -368,11 +389,52 @@ __global__ __launch_bounds__(256) void k_fill_values(Geo G, Terms T, long long n
     int cols[kMaxEntries];
     double vals[kMaxEntries];
     double r = 0., x = 0.;
-    const int k = row_entries<true>(G, T, q, c, node, cols, vals, &r, &x);
+    const int k = row_entries<kRowValues, true>(G, T, q, c, node, cols, vals, &r, &x);
     const int o = row_ptr[row];
     for (int j = 0; j < k; j++) val[o + j] = vals[j];
     rhs[row] = r;
     if (x0) x0[row] = x;
+}
+
+// Diagonal-view fill (3-D): one lane per cell, lanes along k, no column sort. Writes the slot
+// values (coalesced, slot-major), rhs and x0. The CSR values are materialized from the slots
+// only when a CSR consumer asks for them (k_materialize_csr, same sort and merge).
+__global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __restrict__ dia, int slots,
+                                                  double* __restrict__ rhs, double* __restrict__ x0) {
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (k >= G.n[2] || j >= G.n[1]) return;
+    const int q[3] = {i, j, k};
+    const long long node = (static_cast<long long>(i) * G.n[1] + j) * G.n[2] + k;
+    const long long nrows = G.ncell_int * G.N;
+    for (int c = 0; c < G.N; c++) {
+        const long long row = node * G.N + c;
+        double slotv[kDiaSlots];
+        double r = 0., x = 0.;
+        row_entries<kRowValues, false>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
+        for (int sl = 0; sl < slots; sl++) dia[sl * nrows + row] = slotv[sl];
+        rhs[row] = r;
+        if (x0) x0[row] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_materialize_csr(Geo G, Terms T, long long nrows,
+                                                         const int* __restrict__ row_ptr,
+                                                         const double* __restrict__ dia, int slots,
+                                                         double* __restrict__ val) {
+    const long long row = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    int q[3], c;
+    long long node;
+    row_to_cell(G, row, q, &c, &node);
+    double sl_in[kDiaSlots];
+    for (int sl = 0; sl < kDiaSlots; sl++) sl_in[sl] = sl < slots ? dia[sl * nrows + row] : 0.;
+    double vals[kMaxEntries];
+    const int k = row_entries<kRowFromSlots, true>(G, T, q, c, node, nullptr, vals, nullptr, nullptr, nullptr,
+                                                   sl_in);
+    const int o = row_ptr[row];
+    for (int jj = 0; jj < k; jj++) val[o + jj] = vals[jj];
 }
 
 int make_terms(const Geo& G, const dare_terms* t, const double* phi, Terms* T) {
@@ -387,6 +449,7 @@ int make_terms(const Geo& G, const dare_terms* t, const double* phi, Terms* T) {
     for (int d = 0; d < 3; d++) T->vel[d] = t->vel[d];
     T->source = t->source;
     T->react = t->react;
+    T->cross = (t->react != nullptr && G.N == 2) ? 1 : 0;
     for (int f = 0; f < 6; f++) {
         T->bc_type[f] = f < 2 * G.dim ? t->bc_type[f] : 0;
         T->bc_value[f] = t->bc_value[f];
@@ -428,6 +491,23 @@ int make_terms(const Geo& G, const dare_terms* t, const double* phi, Terms* T) {
 }
 
 }  // namespace
+
+int csr_ensure_values(dare_csr* A, hipStream_t stream) {
+    if (!A->vals_stale) return DARE_OK;
+    Geo G;
+    int rc = make_geo(&A->grid_desc, &G);
+    if (rc) return rc;
+    Terms T;
+    std::memset(&T, 0, sizeof(T));
+    T.cross = A->cross;
+    for (int f = 0; f < 6; f++) T.bc_type[f] = A->bc_type[f];
+    hipLaunchKernelGGL(k_materialize_csr, dim3(ceil_div(A->nrows, 256)), dim3(256), 0, stream, G, T, A->nrows,
+                       A->row_ptr, A->dia, A->cross ? 8 : 7, A->val);
+    DARE_LAUNCH_CHECK();
+    A->vals_stale = 0;
+    return DARE_OK;
+}
+
 }  // namespace dare_hip
 
 using namespace dare_hip;
@@ -479,6 +559,8 @@ int dare_csr_destroy(dare_csr* A) {
     if (A->col) hipFree(A->col);
     if (A->val) hipFree(A->val);
     if (A->val32) hipFree(A->val32);
+    if (A->dia) hipFree(A->dia);
+    if (A->dia32) hipFree(A->dia32);
     delete A;
     return DARE_OK;
 }
@@ -530,6 +612,38 @@ int dare_csr_build_structure(dare_ctx* ctx, dare_csr* A, const dare_grid* g, con
     A->nrows = nrows;
     A->nnz = total;
     A->stencil_structured = 1;
+    A->grid_desc = *g;
+    for (int f = 0; f < 6; f++) A->bc_type[f] = T.bc_type[f];
+    A->cross = T.cross;
+    A->vals_stale = 0;
+    // diagonal view for the solver (3-D, 1 or 2 components)
+    A->has_dia = 0;
+    if (G.dim == 3 && (G.N == 1 || G.N == 2)) {
+        DiaView& V = A->dia_view;
+        for (int d = 0; d < 3; d++) {
+            V.n[d] = G.n[d];
+            V.per[d] = G.per[d];
+            V.rm_lo[d] = T.bc_type[2 * d] != 0;
+            V.rm_hi[d] = T.bc_type[2 * d + 1] != 0;
+        }
+        V.N = G.N;
+        V.cross = (t->react != nullptr && G.N == 2) ? 1 : 0;
+        V.nrows = nrows;
+        const int slots = V.cross ? 8 : 7;
+        const int64_t need = static_cast<int64_t>(slots) * nrows;
+        if (A->cap_dia < need) {
+            if (A->dia) hipFree(A->dia);
+            A->dia = nullptr;
+            A->cap_dia = 0;
+            if (hipMalloc(&A->dia, sizeof(double) * need) != hipSuccess) {
+                hipGetLastError();
+                set_error("diagonal view allocation failed");
+                return DARE_ERR_OOM;
+            }
+            A->cap_dia = need;
+        }
+        A->has_dia = 1;
+    }
     return DARE_OK;
 }
 
@@ -544,9 +658,20 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
     if (rc) return rc;
     const long long nrows = G.ncell_int * G.N;
     DARE_CHECK_ARG(A->stencil_structured && A->nrows == nrows, "CSR structure does not match the grid");
+    DARE_CHECK_ARG(T.cross == A->cross, "reaction terms differ from the ones the structure was built with");
+    for (int f = 0; f < 6; f++)
+        DARE_CHECK_ARG(T.bc_type[f] == A->bc_type[f], "boundary edits differ from the structure's");
+    if (A->has_dia) {
+        const dim3 grid(ceil_div(G.n[2], 64), ceil_div(G.n[1], 4), G.n[0]);
+        hipLaunchKernelGGL(k_fill_dia, grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->cross ? 8 : 7, rhs, x0);
+        DARE_LAUNCH_CHECK();
+        A->vals_stale = 1;
+        return DARE_OK;
+    }
     hipLaunchKernelGGL(k_fill_values, dim3(ceil_div(nrows, 256)), dim3(256), 0, ctx->stream, G, T, nrows,
                        A->row_ptr, A->val, rhs, x0);
     DARE_LAUNCH_CHECK();
+    A->vals_stale = 0;
     return DARE_OK;
 }
 
@@ -570,12 +695,21 @@ int dare_csr_upload_host(dare_ctx* ctx, dare_csr* A, int64_t nrows, int64_t nnz,
     A->nrows = nrows;
     A->nnz = nnz;
     A->stencil_structured = 0;
+    A->has_dia = 0;
+    A->vals_stale = 0;
     return DARE_OK;
 }
 
 int dare_csr_info(const dare_csr* A, int64_t* nrows, int64_t* nnz, const int32_t** row_ptr,
                   const int32_t** col, const double** val) {
     DARE_CHECK_ARG(A != nullptr, "NULL csr");
+    if (A->vals_stale) {    // hand out valid CSR values: materialize on the null stream
+        // the fill ran on a context stream that may not synchronize with the null stream
+        DARE_HIP_TRY(hipDeviceSynchronize());
+        int rc = csr_ensure_values(const_cast<dare_csr*>(A), nullptr);
+        if (rc) return rc;
+        DARE_HIP_TRY(hipDeviceSynchronize());
+    }
     if (nrows) *nrows = A->nrows;
     if (nnz) *nnz = A->nnz;
     if (row_ptr) *row_ptr = A->row_ptr;
@@ -586,6 +720,8 @@ int dare_csr_info(const dare_csr* A, int64_t* nrows, int64_t* nnz, const int32_t
 
 int dare_csr_download(dare_ctx* ctx, const dare_csr* A, int32_t* row_ptr, int32_t* col, double* val) {
     DARE_CHECK_ARG(ctx && A, "NULL argument");
+    int rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
+    if (rc) return rc;
     if (row_ptr)
         DARE_HIP_TRY(hipMemcpyAsync(row_ptr, A->row_ptr, sizeof(int32_t) * (A->nrows + 1), hipMemcpyDeviceToHost, ctx->stream));
     if (col && A->nnz)

@@ -40,7 +40,7 @@ void set_error(const std::string& msg);
     } while (0)
 
 constexpr int kMaxComp = 8;
-constexpr int kReduceBlocks = 2048;     // partial sums per one-pass grid reduction (= max grid)
+constexpr int kReduceBlocks = 8192;     // partial sums per one-pass grid reduction (= max grid)
 constexpr int kMaxDots = 8;
 
 // Device-side reduction workspace: block partials + arrival counter (reset by the last block).
@@ -63,17 +63,6 @@ struct dare_ctx {
     void* solve_state = nullptr;   // device-resident solver scalars
 };
 
-struct dare_csr {
-    int device = 0;              // owner device; the creating context may be destroyed first
-    int64_t nrows = 0, nnz = 0, cap_rows = 0, cap_nnz = 0;
-    int32_t* row_ptr = nullptr;
-    int32_t* col = nullptr;
-    double* val = nullptr;
-    float* val32 = nullptr;      // FP32 copy of the values (mixed precision)
-    int64_t cap_val32 = 0;
-    int stencil_structured = 0;  // 1 if built by dare_csr_build_structure
-};
-
 namespace dare_hip {
 
 // Geometry of a grid, computed on the host with the reference's formulas
@@ -88,7 +77,49 @@ struct Geo {
     long long ncell_int, ncell_pad;
 };
 
+// Slot-major diagonal view of a stencil-structured CSR (3-D, 1 or 2 components): the value
+// of slot s in row r is dia[s * nrows + r]. Slots: 0 W, 1 S, 2 B, 3 C, 4 T, 5 N, 6 E,
+// 7 cross-component entry. Entries removed by a boundary edit hold 0 and are never read.
+constexpr int kDiaSlots = 8;
+struct DiaView {
+    int n[3];           // cells along i (slowest), j, k (fastest)
+    int N;              // components (1 or 2)
+    int per[3];
+    int rm_lo[3], rm_hi[3];   // boundary entry removed at the low / high face of each axis
+    int cross;          // 1: slot 7 holds the cross-component entry
+    long long nrows;
+};
+
+}  // namespace dare_hip
+
+struct dare_csr {
+    int device = 0;              // owner device; the creating context may be destroyed first
+    int64_t nrows = 0, nnz = 0, cap_rows = 0, cap_nnz = 0;
+    int32_t* row_ptr = nullptr;
+    int32_t* col = nullptr;
+    double* val = nullptr;
+    float* val32 = nullptr;      // FP32 copy of the values (mixed precision, generic path)
+    int64_t cap_val32 = 0;
+    int stencil_structured = 0;  // 1 if built by dare_csr_build_structure
+    // diagonal view written next to the CSR values by dare_csr_fill_values (structured, 3-D)
+    int has_dia = 0;
+    dare_hip::DiaView dia_view{};
+    double* dia = nullptr;
+    float* dia32 = nullptr;
+    int64_t cap_dia = 0, cap_dia32 = 0;
+    // structural description kept for lazy CSR value materialization from the diagonal view
+    dare_grid grid_desc{};
+    int bc_type[6] = {0, 0, 0, 0, 0, 0};
+    int cross = 0;
+    int vals_stale = 0;          // 1: CSR values not yet written for the last fill
+};
+
+namespace dare_hip {
+
 int make_geo(const dare_grid* gr, Geo* G);
+
+// Writes the CSR values of a matrix whose last fill only wrote the diagonal view.
+int csr_ensure_values(dare_csr* A, hipStream_t stream);
 
 inline unsigned int ceil_div(long long a, long long b) { return static_cast<unsigned int>((a + b - 1) / b); }
 

@@ -283,6 +283,89 @@ __global__ __launch_bounds__(256) void k_tvd_step(Geo G, FaceGeo F, int lim, con
     }
 }
 
+// Face value between L and L+1 given the four cells a = L-1, b = L, c = L+1, d = L+2
+// (same selection and arithmetic as tvd_face), times the face velocity.
+__device__ inline double tvd_flux4(double a, double b, double c, double d, int lim, double u) {
+    double pUU, pU, pD;
+    if (u >= 0.) {
+        pUU = a;
+        pU = b;
+        pD = c;
+    } else {
+        pUU = d;
+        pU = c;
+        pD = b;
+    }
+    const double r = (pU - pUU) / (pD - pU);
+    const double psi = limiter(lim, r);
+    return (pU + (psi * 0.5) * (pD - pU)) * u;
+}
+
+constexpr int kStepTileK = 64, kStepTileJ = 4, kStepTileI = 16, kStepXcd = 8;
+
+// 3-D, one component: a 64 (k) x 4 (j) tile marches along i holding the planes i-2..i+2 of phi
+// in registers; the x-face flux at (i, i+1) is carried to the next plane as its low face.
+__global__ __launch_bounds__(256) void k_tvd_step3(Geo G, int lim, const double* __restrict__ phi,
+                                                   const double* __restrict__ ux, const double* __restrict__ uy,
+                                                   const double* __restrict__ uz, double fac,
+                                                   double* __restrict__ out, int tiles_k, int tiles_j,
+                                                   int tiles_i, int per_xcd) {
+    const int tile = static_cast<int>(blockIdx.x % kStepXcd) * per_xcd + static_cast<int>(blockIdx.x / kStepXcd);
+    if (tile >= tiles_k * tiles_j * tiles_i) return;
+    const int tk = tile % tiles_k, t2 = tile / tiles_k, tj = t2 % tiles_j, ti = t2 / tiles_j;
+    const int k = tk * kStepTileK + (threadIdx.x & 63);
+    const int j = tj * kStepTileJ + (threadIdx.x >> 6);
+    const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+    if (k >= n2 || j >= n1) return;
+    const int i0 = ti * kStepTileI;
+    const int i1 = (i0 + kStepTileI < n0) ? i0 + kStepTileI : n0;
+    const long long sI = G.sl[0], sJ = G.sl[1];
+    long long lp = (static_cast<long long>(i0 + G.g) * G.np[1] + (j + G.g)) * G.np[2] + (k + G.g);
+    // face arrays: x (n0+1, n1, n2), y (n0, n1+1, n2), z (n0, n1, n2+1)
+    const long long fxs = static_cast<long long>(n1) * n2;
+    long long fx = static_cast<long long>(i0) * fxs + static_cast<long long>(j) * n2 + k;
+    long long fy = (static_cast<long long>(i0) * (n1 + 1) + j) * n2 + k;
+    const long long fys = static_cast<long long>(n1 + 1) * n2;
+    long long fz = (static_cast<long long>(i0) * n1 + j) * (n2 + 1) + k;
+    const long long fzs = static_cast<long long>(n1) * (n2 + 1);
+    double pm2 = phi[lp - 2 * sI], pm1 = phi[lp - sI], p0 = phi[lp], pp1 = phi[lp + sI];
+    double u_lo = ux[fx];
+    double f_lo = tvd_flux4(pm2, pm1, p0, pp1, lim, u_lo);
+    const double A0 = G.A[0], A1 = G.A[1], A2 = G.A[2];
+    for (int i = i0; i < i1; i++) {
+        const double pp2 = phi[lp + 2 * sI];
+        const double u_hi = ux[fx + fxs];
+        const double f_hi = tvd_flux4(pm1, p0, pp1, pp2, lim, u_hi);
+        // y faces: cells j-2..j+2 of this plane
+        const double ym2 = phi[lp - 2 * sJ], ym1 = phi[lp - sJ], yp1 = phi[lp + sJ], yp2 = phi[lp + 2 * sJ];
+        const double vy_lo = uy[fy], vy_hi = uy[fy + n2];
+        const double gy_lo = tvd_flux4(ym2, ym1, p0, yp1, lim, vy_lo);
+        const double gy_hi = tvd_flux4(ym1, p0, yp1, yp2, lim, vy_hi);
+        // z faces: cells k-2..k+2
+        const double zm2 = phi[lp - 2], zm1 = phi[lp - 1], zp1 = phi[lp + 1], zp2 = phi[lp + 2];
+        const double wz_lo = uz[fz], wz_hi = uz[fz + 1];
+        const double hz_lo = tvd_flux4(zm2, zm1, p0, zp1, lim, wz_lo);
+        const double hz_hi = tvd_flux4(zm1, p0, zp1, zp2, lim, wz_hi);
+        double v = A0 * f_hi;
+        v -= A0 * f_lo;
+        v += A1 * gy_hi;
+        v -= A1 * gy_lo;
+        v += A2 * hz_hi;
+        v -= A2 * hz_lo;
+        out[lp] = p0 - fac * v;
+        // roll
+        pm2 = pm1;
+        pm1 = p0;
+        p0 = pp1;
+        pp1 = pp2;
+        f_lo = f_hi;
+        lp += sI;
+        fx += fxs;
+        fy += fys;
+        fz += fzs;
+    }
+}
+
 __global__ void k_fill(double* __restrict__ p, long long n, double v) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -424,6 +507,17 @@ int dare_tvd_explicit_step(dare_ctx* ctx, const dare_grid* g, int32_t limiter, c
     const FaceGeo F = make_face_geo(G);
     const Launch L = cell_launch(G);
     const double fac = dt / G.V;
+    if (G.dim == 3 && G.N == 1) {
+        const int tk = static_cast<int>(ceil_div(G.n[2], kStepTileK));
+        const int tj = static_cast<int>(ceil_div(G.n[1], kStepTileJ));
+        const int ti = static_cast<int>(ceil_div(G.n[0], kStepTileI));
+        const long long T = static_cast<long long>(tk) * tj * ti;
+        const int per = static_cast<int>((T + kStepXcd - 1) / kStepXcd);
+        hipLaunchKernelGGL(k_tvd_step3, dim3(kStepXcd * per), dim3(256), 0, ctx->stream, G, limiter, phi, ux, uy, uz,
+                           fac, phi_out, tk, tj, ti, per);
+        DARE_LAUNCH_CHECK();
+        return DARE_OK;
+    }
     DARE_DISPATCH_DIM(G, k_tvd_step, L, G, F, limiter, phi, ux, uy, uz, fac, phi_out);
     DARE_LAUNCH_CHECK();
     return DARE_OK;

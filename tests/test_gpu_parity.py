@@ -270,3 +270,103 @@ def test_mixed_refinement_reaches_fp64_residual(ctx, rng, name):
     assert true_rel <= 1e-10
     xo, ok, *_ = orc.bicgstab2(rp, col, val, rhs, x0, 1e-13 * np.linalg.norm(rhs), 300)
     assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
+
+
+DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "react2"]
+
+
+def device_assembly(ctx, rng, name):
+    """Assemble on the device (structure + values): the matrix then carries the diagonal view
+    the solver traverses."""
+    import torch
+    go, kw, arrays, phi = assembly_case(rng, name)
+    to = orc.terms(**kw, **arrays)
+    ref = orc.assemble(go, to, phi)
+    g = grids(go)
+    darr = {k: ([dev(v) for v in a] if isinstance(a, list) else dev(a)) for k, a in arrays.items()}
+    td = da.Terms.make(**kw, **darr)
+    A = da.Csr(ctx)
+    da.build_structure(ctx, A, g, td)
+    rhs = torch.zeros(len(ref[3]), dtype=torch.float64, device="cuda")
+    x0 = torch.zeros_like(rhs)
+    da.fill_values(ctx, A, g, td, dev(phi), rhs, x0)
+    ctx.synchronize()
+    return go, A, rhs, x0, ref, (td, darr)
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+def test_diagonal_view_spmv_matches_oracle(ctx, rng, name):
+    """The solver's operator (tiled traversal of the diagonal view) against the oracle's CSR
+    SpMV: same per-row order, so bit-exact except on periodic-wrapped rows."""
+    import torch
+    go, A, rhs, x0, (rp, col, val, orhs, ox0), keep = device_assembly(ctx, rng, name)
+    x = rng.normal(size=len(orhs))
+    w = rng.normal(size=len(orhs))
+    y = torch.zeros(len(orhs), dtype=torch.float64, device="cuda")
+    d = da.spmv_dot(ctx, A, dev(x), y, dev(w))
+    exp = orc.spmv(rp, col, val, x)
+    got = host(y)
+    if name == "periodic_conv":
+        assert np.max(np.abs(got - exp)) <= 1e-14 * np.max(np.abs(exp))
+    else:
+        assert bits_equal(got, exp)
+    assert abs(d - float(exp @ w)) <= 1e-12 * float(np.abs(exp) @ np.abs(w))
+    A.close()
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+def test_solver_on_device_assembly_matches_oracle(ctx, rng, name):
+    go, A, rhs, x0, (rp, col, val, orhs, ox0), keep = device_assembly(ctx, rng, name)
+    tol = 1e-12 * np.linalg.norm(orhs)
+    xo, ok, it, res, hist = orc.bicgstab2(rp, col, val, orhs, ox0, tol, 300)
+    assert ok
+    r = da.bicgstab2(ctx, A, rhs, x0, tol=tol, tol_kind=0, max_it=300)
+    assert r["converged"] == 1 and abs(r["iterations"] - it) <= 1
+    x = host(x0)
+    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo)) + tol
+    A.close()
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+def test_mixed_on_device_assembly(ctx, rng, name):
+    go, A, rhs, x0, (rp, col, val, orhs, ox0), keep = device_assembly(ctx, rng, name)
+    r = da.refine_mixed(ctx, A, rhs, x0, tol=1e-10, tol_kind=1, inner_tol=1e-4)
+    assert r["converged"] == 1
+    x = host(x0)
+    assert np.linalg.norm(orhs - orc.spmv(rp, col, val, x)) <= 1e-10 * np.linalg.norm(orhs)
+    A.close()
+
+
+def test_full_size_structure_and_solve_properties(ctx):
+    """256^3 (the benchmark size), properties that do not need the oracle: nnz = 7n^3 - 6n^2,
+    row lengths 4..7, sorted columns, SpMV of ones = row sums, and the FP64 solve's true
+    relative residual."""
+    import torch
+    from dare_amd.workloads import C2Rotation
+    n = 256
+    W = C2Rotation(n)
+    A = da.Csr(ctx)
+    da.build_structure(ctx, A, W.grid, W.terms)
+    da.fill_values(ctx, A, W.grid, W.terms, W.phi, W.rhs, W.x)
+    ctx.synchronize()
+    nr, nnz, rp_ptr, col_ptr, val_ptr = A.info()
+    assert nr == n ** 3 and nnz == 7 * n ** 3 - 6 * n ** 2
+    rp, col, val = A.download()
+    lens = np.diff(rp)
+    assert lens.min() == 4 and lens.max() == 7 and rp[-1] == nnz
+    first = rp[:-1]
+    d = np.diff(col)
+    start_mask = np.zeros(nnz, bool)
+    start_mask[first] = True
+    assert np.all(d[~start_mask[1:]] > 0)        # ascending within every row
+    ones = torch.ones(nr, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(ones)
+    da.spmv(ctx, A, ones, y)
+    rowsum = np.add.reduceat(val, first)
+    assert np.array_equal(host(y), rowsum) or np.allclose(host(y), rowsum, rtol=1e-15, atol=0)
+    r = da.bicgstab2(ctx, A, W.rhs, W.x, tol=1e-8, tol_kind=1, max_it=100)
+    assert r["converged"] == 1
+    da.spmv(ctx, A, W.x, y)
+    rel = float(torch.linalg.norm(W.rhs - y) / torch.linalg.norm(W.rhs))
+    assert rel <= 1e-8
+    A.close()
