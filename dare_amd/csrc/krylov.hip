@@ -121,11 +121,14 @@ __device__ void apply_tail(int op, KState* st, const double* d) {
     }
 }
 
-// Reduce NDOT per-thread sums over the grid; the last work-group applies the tail.
+// Block-level part of a grid reduction: each work-group writes its NDOT partial sums with
+// plain stores. No fence or arrival counter here: on MI355X the eight XCD L2 caches are not
+// coherent, and an agent-scope release fence in every work-group writes the L2 back each time.
+// The kernel boundary publishes the partials; k_finish (one work-group) sums them in block
+// order — deterministic — and applies the BiCGStab2 scalar update on the device.
 template <int NDOT>
-__device__ inline void grid_reduce(double (&acc)[NDOT], ReduceWs ws, KState* st, int tail) {
+__device__ inline void block_partials(double (&acc)[NDOT], ReduceWs ws) {
     __shared__ double red[NDOT][kThreads / 64];
-    __shared__ int is_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < NDOT; j++) {
@@ -133,40 +136,61 @@ __device__ inline void grid_reduce(double (&acc)[NDOT], ReduceWs ws, KState* st,
         if (lane == 0) red[j][wid] = v;
     }
     __syncthreads();
-    if (threadIdx.x < NDOT) {
+    if (threadIdx.x < NDOT) {     // dot-major: k_finish reads each dot's partials contiguously
         double s = 0.;
         for (int w = 0; w < kThreads / 64; w++) s += red[threadIdx.x][w];
-        ws.partial[blockIdx.x * kMaxDots + threadIdx.x] = s;
+        ws.partial[threadIdx.x * kReduceBlocks + blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(ws.count, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+constexpr int kFinishThreads = 1024;
+
+template <int NDOT>
+__global__ __launch_bounds__(kFinishThreads) void k_finish(ReduceWs ws, unsigned nblocks, KState* st, int tail,
+                                                           int check_status) {
+    if (check_status && st->status) return;       // the pass has broken down: keep its state
+    __shared__ double red[kMaxDots][kFinishThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double tot[NDOT];
 #pragma unroll
     for (int j = 0; j < NDOT; j++) {
+        // fixed order: thread t owns blocks t, t + 1024, ...; loads issued before the adds
+        const double* p = ws.partial + j * kReduceBlocks;
+        double v[kReduceBlocks / kFinishThreads];
+#pragma unroll
+        for (int q = 0; q < kReduceBlocks / kFinishThreads; q++) {
+            const unsigned b = threadIdx.x + q * kFinishThreads;
+            v[q] = b < nblocks ? p[b] : 0.;
+        }
         double s = 0.;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += kThreads)
-            s += __hip_atomic_load(&ws.partial[b * kMaxDots + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int q = 0; q < kReduceBlocks / kFinishThreads; q++) s += v[q];
         s = wave_sum(s);
-        __syncthreads();
         if (lane == 0) red[j][wid] = s;
-        __syncthreads();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NDOT; j++) {
         double t = 0.;
-        for (int w = 0; w < kThreads / 64; w++) t += red[j][w];
+        for (int w = 0; w < kFinishThreads / 64; w++) t += red[j][w];
         tot[j] = t;
     }
     if (threadIdx.x == 0) {
         double d[kMaxDots];
         for (int j = 0; j < kMaxDots; j++) d[j] = j < NDOT ? tot[j] : 0.;
         apply_tail(tail, st, d);
-        *ws.count = 0u;
     }
+}
+
+// grid_reduce keeps the call sites of the kernels: the block part only.
+template <int NDOT>
+__device__ inline void grid_reduce(double (&acc)[NDOT], ReduceWs ws, KState*, int) {
+    block_partials<NDOT>(acc, ws);
+}
+
+template <int NDOT>
+inline void launch_finish(hipStream_t sm, ReduceWs ws, unsigned nblocks, KState* st, int tail, int check) {
+    hipLaunchKernelGGL((k_finish<NDOT>), dim3(1), dim3(kFinishThreads), 0, sm, ws, nblocks, st, tail, check);
 }
 
 // ------------------------------------------------------------------------------ operators
@@ -629,8 +653,9 @@ struct SolveOp {
 
     template <typename XT, class Epi>
     void launch(hipStream_t sm, const XT* x, const Epi& epi, ReduceWs ws, KState* st, int tail, int check) const {
+        unsigned grid;
         if (dia) {
-            const unsigned grid = static_cast<unsigned>(kXcd * d.per_xcd);   // <= kReduceBlocks
+            grid = static_cast<unsigned>(kXcd * d.per_xcd);   // <= kReduceBlocks
             if (d.V.N == 2)
                 hipLaunchKernelGGL((k_apply_dia<2, VT, XT, Epi>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws, st,
                                    tail, check);
@@ -638,9 +663,11 @@ struct SolveOp {
                 hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws, st,
                                    tail, check);
         } else {
-            hipLaunchKernelGGL((k_apply_csr<VT, XT, Epi>), dim3(grid_rows(n)), dim3(kThreads), 0, sm, csr, x, epi, ws,
+            grid = grid_rows(n);
+            hipLaunchKernelGGL((k_apply_csr<VT, XT, Epi>), dim3(grid), dim3(kThreads), 0, sm, csr, x, epi, ws,
                                st, tail, check);
         }
+        if (tail >= 0) launch_finish<Epi::NDOT>(sm, ws, grid, st, tail, check);
     }
 };
 
@@ -688,7 +715,7 @@ static_assert(sizeof(KState) <= 64 * sizeof(double), "pinned readback block too 
 template <typename VT, typename XT>
 int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
                   double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                  bool* converged, XT* work) {
+                  bool* converged, XT* work, int min_passes) {
     const long long n = A.n;
     XT* r = work;
     XT* rh = r + n;
@@ -732,6 +759,7 @@ int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, doubl
         hipLaunchKernelGGL((k_upd_urs<XT>), dim3(gr), dim3(kThreads), 0, sm, n, u, r, v, s, w, st);
         A.launch(sm, s, EpiGcr<XT>{t, r}, ws, st, TAIL_GCR, 1);
         hipLaunchKernelGGL((k_end<XT>), dim3(gr), dim3(kThreads), 0, sm, n, x, r, s, t, u, v, w, rh, ws, st);
+        launch_finish<2>(sm, ws, gr, st, TAIL_END, 1);
         DARE_LAUNCH_CHECK();
         rc = read_state(ctx, st, &h);
         if (rc) return rc;
@@ -752,7 +780,7 @@ int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, doubl
         }
         conv = h.rnorm;
         if (hist && *hist_len < hist_cap) hist[(*hist_len)++] = conv;
-        if (conv <= tol_abs && k > 1) break;
+        if (conv <= tol_abs && k >= min_passes) break;   // reference: k > 1 (min_passes 2)
         ++k;
     }
     *iters = k;
@@ -766,6 +794,7 @@ static int dot_dev(dare_ctx* ctx, long long n, const double* a, const double* b,
     int rc = ws_reserve(ctx, 0, &W);
     if (rc) return rc;
     hipLaunchKernelGGL((k_dot<double>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, a, b, ctx->ws, W.st);
+    launch_finish<1>(ctx->stream, ctx->ws, grid_for(n), W.st, TAIL_STORE, 0);
     DARE_LAUNCH_CHECK();
     KState h;
     rc = read_state(ctx, W.st, &h);
@@ -858,7 +887,7 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     double rn = 0.;
     bool conv = false;
     rc = bicgstab2_run<double, double>(ctx, op, b, x, tol_abs, p->max_it, p->history, p->history_cap, &hl, &iters,
-                                       &rn, &status, &conv, static_cast<double*>(W.buf));
+                                       &rn, &status, &conv, static_cast<double*>(W.buf), 2);
     if (rc) return rc;
     res->converged = conv ? 1 : 0;
     res->iterations = iters;
@@ -959,7 +988,7 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         double irn = 0.;
         bool iconv = false;
         rc = bicgstab2_run<float, float>(ctx, op32, r32, d32, inner_tol * rn, inner_max, nullptr, 0, &dummy_len, &it,
-                                         &irn, &ist, &iconv, work);
+                                         &irn, &ist, &iconv, work, 2);
         if (rc) return rc;
         total_it += it;
         hipLaunchKernelGGL(k_add_correction, dim3(gr), dim3(kThreads), 0, ctx->stream, n, x, d32);
