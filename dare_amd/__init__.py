@@ -128,6 +128,14 @@ class SolverResult(ctypes.Structure):
                 ("history_len", ctypes.c_int32)]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("tile_k", ctypes.c_int32), ("tile_i", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("order", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _ptr(t):
     """Device pointer of a torch tensor (or an int address), None -> NULL."""
     if t is None:
@@ -187,6 +195,13 @@ def lib():
         "dare_axpby": (i32, [vp, i64, dbl, vp, dbl, vp]),
         "dare_bicgstab2_solve": (i32, [vp, vp, vp, vp, P, R]),
         "dare_refine_solve_mixed": (i32, [vp, vp, vp, vp, P, R]),
+        "dare_batch_create": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "dare_batch_destroy": (i32, [vp]),
+        "dare_batch_vectors": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64)]),
+        "dare_batch_set_config": (i32, [vp, vp, ctypes.POINTER(BatchConfig)]),
+        "dare_batch_get_config": (i32, [vp, ctypes.POINTER(BatchConfig)]),
+        "dare_batch_autotune": (i32, [vp, vp, i32, ctypes.POINTER(BatchConfig), ctypes.POINTER(dbl)]),
+        "dare_batch_solve": (i32, [vp, vp, P, i32, i32, R]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -381,3 +396,65 @@ def refine_mixed(ctx, csr, b, x, tol=1e-8, tol_kind=1, max_it=1000, inner_tol=1e
     out = {k: getattr(r, k) for k, _ in SolverResult._fields_}
     out["history"] = list(hist[:r.history_len])
     return out
+
+
+class Batch:
+    """Batched solves over many structured systems (dare_batch). `systems` are Csr handles
+    built by build_structure on 3-D grids; the batch owns the concatenated b and x, and
+    `rhs_ptr(s)` / `x_ptr(s)` are the device addresses of system s's rows in them."""
+
+    def __init__(self, ctx, systems):
+        self.ctx = ctx
+        self.systems = list(systems)
+        arr = (ctypes.c_void_p * len(self.systems))(*[c.h.value for c in self.systems])
+        self.h = ctypes.c_void_p()
+        check(lib().dare_batch_create(ctx.h, len(self.systems), arr, ctypes.byref(self.h)), "dare_batch_create")
+        b, x = ctypes.c_void_p(), ctypes.c_void_p()
+        offs = (ctypes.c_int64 * (len(self.systems) + 1))()
+        check(lib().dare_batch_vectors(self.h, ctypes.byref(b), ctypes.byref(x), offs), "dare_batch_vectors")
+        self.b_addr, self.x_addr = b.value, x.value
+        self.offsets = np.array(offs[:], dtype=np.int64)
+
+    @property
+    def rows(self):
+        return int(self.offsets[-1])
+
+    def rhs_ptr(self, s):
+        return self.b_addr + 8 * int(self.offsets[s])
+
+    def x_ptr(self, s):
+        return self.x_addr + 8 * int(self.offsets[s])
+
+    def config(self):
+        c = BatchConfig()
+        check(lib().dare_batch_get_config(self.h, ctypes.byref(c)), "dare_batch_get_config")
+        return c.as_dict()
+
+    def set_config(self, tile_k=0, tile_i=16, grid=2048, order=1):
+        c = BatchConfig(tile_k, tile_i, grid, order)
+        check(lib().dare_batch_set_config(self.ctx.h, self.h, ctypes.byref(c)), "dare_batch_set_config")
+
+    def autotune(self, reps=5):
+        c, us = BatchConfig(), ctypes.c_double()
+        check(lib().dare_batch_autotune(self.ctx.h, self.h, reps, ctypes.byref(c), ctypes.byref(us)),
+              "dare_batch_autotune")
+        return c.as_dict(), us.value
+
+    def solve(self, tol=1e-8, tol_kind=1, max_it=1000, mixed=False, inner_tol=1e-4, max_refine=30,
+              inner_max_it=1000, graph_passes=4):
+        p, hist = _params(tol, tol_kind, max_it, inner_tol, max_refine, inner_max_it, 0)
+        res = (SolverResult * len(self.systems))()
+        check(lib().dare_batch_solve(self.ctx.h, self.h, ctypes.byref(p), 1 if mixed else 0, graph_passes, res),
+              "dare_batch_solve")
+        return [{k: getattr(r, k) for k, _ in SolverResult._fields_} for r in res]
+
+    def close(self):
+        if self.h:
+            lib().dare_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

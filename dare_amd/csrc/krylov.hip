@@ -2,7 +2,7 @@
 //
 // BiCGStab2::Solve (MatrixSystem/BiCGStab2.inl:167-582) is restated as nine fused launches per
 // pass. All method scalars (alpha, beta, gamma, rho, omega, mu, nu, tau) live in device memory
-// and are updated by the last work-group of each one-pass grid reduction, so the host reads
+// and are updated by the one-work-group finish kernel of each reduction, so the host reads
 // back only one small status block per pass (the convergence test). Reductions are
 // deterministic: fixed grid, fixed per-thread order, block partials summed in block order.
 //
@@ -15,633 +15,10 @@
 #include <cstring>
 #include <vector>
 
-#include "common.h"
+#include "krylov_core.h"
 
 namespace dare_hip {
 namespace {
-
-constexpr int kThreads = 256;
-constexpr int kGridMax = 2048;      // work-groups of the grid-stride launches
-constexpr int kTileK = 64, kTileJ = 4, kTileI = 16;
-constexpr int kXcd = 8;             // MI355X: 8 accelerator complex dies, blocks dealt round robin
-
-// Device-resident BiCGStab2 state (BiCGStab2.inl:180-196 scalars).
-struct KState {
-    double alpha, beta, gamma, rho0, rho1;
-    double om1, om2, mu, nu, tau;
-    double rnorm;      // ||r|| after the last update
-    double dots[kMaxDots];
-    int status;        // breakdown inside the pass (step number), 0 = none
-    int next_status;   // breakdown in the prelude of the next pass (steps 3-4)
-};
-
-enum TailOp : int {
-    TAIL_NONE = 0,
-    TAIL_STORE = 1,     // store the dots only
-    TAIL_INIT = 2,      // ||r0||, prelude of pass 1 (steps 1, 3, 4)
-    TAIL_GAMMA = 3,     // gamma = <v, r0>, alpha = rho0 / gamma (steps 7 / 14)
-    TAIL_RHO = 4,       // rho1 = <r0, s>, beta, rho0 = rho1 (step 11)
-    TAIL_GCR = 5,       // omega_1,2 (steps 19-23)
-    TAIL_END = 6,       // ||r||, prelude of the next pass (steps 3-4)
-};
-
-__device__ void apply_tail(int op, KState* st, const double* d) {
-    switch (op) {
-    case TAIL_STORE:
-        for (int j = 0; j < kMaxDots; j++) st->dots[j] = d[j];
-        break;
-    case TAIL_INIT: {
-        st->rnorm = sqrt(d[0]);
-        st->alpha = 0.;
-        st->rho0 = 1.;
-        st->om1 = st->om2 = 1.;
-        st->status = 0;
-        st->next_status = 0;
-        st->rho0 = -st->om2 * st->rho0;                     // (3)
-        if (st->rho0 == 0.0) {
-            st->next_status = 3;
-            break;
-        }
-        st->rho1 = d[1];                                     // (4)
-        st->beta = st->alpha * st->rho1 / st->rho0;
-        st->rho0 = st->rho1;
-        if (st->rho0 == 0.0) st->next_status = 4;
-        break;
-    }
-    case TAIL_GAMMA:
-        st->gamma = d[0];
-        if (st->gamma == 0.0) {
-            st->status = 7;
-            break;
-        }
-        st->alpha = st->rho0 / st->gamma;
-        break;
-    case TAIL_RHO:
-        st->rho1 = d[0];
-        st->beta = st->alpha * st->rho1 / st->rho0;
-        st->rho0 = st->rho1;
-        break;
-    case TAIL_GCR: {
-        // omega/mu/nu/tau are long double in the reference; FP64 here (they are cast to
-        // double before use there as well, :545-568)
-        double om1 = d[0], mu = d[1], nu = d[2], tau = d[3], om2 = d[4];
-        st->mu = mu;
-        st->nu = nu;
-        if (mu == 0.) {
-            st->status = 20;
-            break;
-        }
-        tau -= nu * nu / mu;
-        st->tau = tau;
-        if (tau == 0.0) {
-            st->status = 21;
-            break;
-        }
-        om2 = (om2 - (nu * om1) / mu) / tau;
-        om1 = (om1 - nu * om2) / mu;
-        st->om1 = om1;
-        st->om2 = om2;
-        break;
-    }
-    case TAIL_END: {
-        st->rnorm = sqrt(d[0]);
-        st->next_status = 0;
-        st->rho0 = -st->om2 * st->rho0;
-        if (st->rho0 == 0.0) {
-            st->next_status = 3;
-            break;
-        }
-        st->rho1 = d[1];
-        st->beta = st->alpha * st->rho1 / st->rho0;
-        st->rho0 = st->rho1;
-        if (st->rho0 == 0.0) st->next_status = 4;
-        break;
-    }
-    default: break;
-    }
-}
-
-// Block-level part of a grid reduction: each work-group writes its NDOT partial sums with
-// plain stores. No fence or arrival counter here: on MI355X the eight XCD L2 caches are not
-// coherent, and an agent-scope release fence in every work-group writes the L2 back each time.
-// The kernel boundary publishes the partials; k_finish (one work-group) sums them in block
-// order — deterministic — and applies the BiCGStab2 scalar update on the device.
-template <int NDOT>
-__device__ inline void block_partials(double (&acc)[NDOT], ReduceWs ws) {
-    __shared__ double red[NDOT][kThreads / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < NDOT; j++) {
-        const double v = wave_sum(acc[j]);
-        if (lane == 0) red[j][wid] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < NDOT) {     // dot-major: k_finish reads each dot's partials contiguously
-        double s = 0.;
-        for (int w = 0; w < kThreads / 64; w++) s += red[threadIdx.x][w];
-        ws.partial[threadIdx.x * kReduceBlocks + blockIdx.x] = s;
-    }
-}
-
-constexpr int kFinishThreads = 1024;
-
-template <int NDOT>
-__global__ __launch_bounds__(kFinishThreads) void k_finish(ReduceWs ws, unsigned nblocks, KState* st, int tail,
-                                                           int check_status) {
-    if (check_status && st->status) return;       // the pass has broken down: keep its state
-    __shared__ double red[kMaxDots][kFinishThreads / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double tot[NDOT];
-#pragma unroll
-    for (int j = 0; j < NDOT; j++) {
-        // fixed order: thread t owns blocks t, t + 1024, ...; loads issued before the adds
-        const double* p = ws.partial + j * kReduceBlocks;
-        double v[kReduceBlocks / kFinishThreads];
-#pragma unroll
-        for (int q = 0; q < kReduceBlocks / kFinishThreads; q++) {
-            const unsigned b = threadIdx.x + q * kFinishThreads;
-            v[q] = b < nblocks ? p[b] : 0.;
-        }
-        double s = 0.;
-#pragma unroll
-        for (int q = 0; q < kReduceBlocks / kFinishThreads; q++) s += v[q];
-        s = wave_sum(s);
-        if (lane == 0) red[j][wid] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < NDOT; j++) {
-        double t = 0.;
-        for (int w = 0; w < kFinishThreads / 64; w++) t += red[j][w];
-        tot[j] = t;
-    }
-    if (threadIdx.x == 0) {
-        double d[kMaxDots];
-        for (int j = 0; j < kMaxDots; j++) d[j] = j < NDOT ? tot[j] : 0.;
-        apply_tail(tail, st, d);
-    }
-}
-
-// grid_reduce keeps the call sites of the kernels: the block part only.
-template <int NDOT>
-__device__ inline void grid_reduce(double (&acc)[NDOT], ReduceWs ws, KState*, int) {
-    block_partials<NDOT>(acc, ws);
-}
-
-template <int NDOT>
-inline void launch_finish(hipStream_t sm, ReduceWs ws, unsigned nblocks, KState* st, int tail, int check) {
-    hipLaunchKernelGGL((k_finish<NDOT>), dim3(1), dim3(kFinishThreads), 0, sm, ws, nblocks, st, tail, check);
-}
-
-// ------------------------------------------------------------------------------ operators
-template <typename VT>
-struct CsrOp {
-    const int* rp;
-    const int* col;
-    const VT* val;
-    long long n;
-};
-
-template <typename VT>
-struct DiaOp {
-    DiaView V;
-    const VT* D;
-    int tiles_k, tiles_j, tiles_i;
-    int tile_i;         // planes per tile (grown until the grid fits the reduction slots)
-    int per_xcd;        // tiles owned by each XCD (grid = kXcd * per_xcd)
-};
-
-template <typename VT>
-DiaOp<VT> make_dia(const DiaView& V, const VT* D) {
-    DiaOp<VT> A;
-    A.V = V;
-    A.D = D;
-    A.tiles_k = (V.n[2] + kTileK - 1) / kTileK;
-    A.tiles_j = (V.n[1] + kTileJ - 1) / kTileJ;
-    A.tile_i = kTileI;
-    long long T = 0;
-    for (;;) {
-        A.tiles_i = (V.n[0] + A.tile_i - 1) / A.tile_i;
-        T = static_cast<long long>(A.tiles_k) * A.tiles_j * A.tiles_i;
-        if (kXcd * ((T + kXcd - 1) / kXcd) <= kReduceBlocks || A.tile_i >= V.n[0]) break;
-        A.tile_i *= 2;
-    }
-    A.per_xcd = static_cast<int>((T + kXcd - 1) / kXcd);
-    return A;
-}
-
-// ------------------------------------------------------------------------------ epilogues
-// load(i) fetches the row's vector operands (issued together with the matrix loads);
-// operator() gets y = (A x)_i, xc = x_i and the loaded operands, stores results and adds to
-// the per-thread dot accumulators.
-template <typename XT>
-struct EpiInit {        // r = b - A x; rh = r; u = v = w = 0; <r,r>, <rh,r>   (BiCGStab2.inl:265-273)
-    static constexpr int NDOT = 2;
-    const XT* b;
-    XT *r, *rh, *u, *v, *w;
-    __device__ double load(long long i) const { return static_cast<double>(b[i]); }
-    __device__ void operator()(long long i, double y, double, double bi, double* acc) const {
-        const XT ri = static_cast<XT>(bi - y);
-        r[i] = ri;
-        rh[i] = ri;
-        u[i] = XT(0);
-        v[i] = XT(0);
-        w[i] = XT(0);
-        const double rd = static_cast<double>(ri);
-        acc[0] += rd * rd;
-        acc[1] += rd * rd;
-    }
-};
-
-template <typename XT>
-struct EpiDot {         // out = A p; <out, w>                   (6)-(7), (9)+(11), (13)-(14)
-    static constexpr int NDOT = 1;
-    XT* out;
-    const XT* w;
-    __device__ double load(long long i) const { return static_cast<double>(w[i]); }
-    __device__ void operator()(long long i, double y, double, double wi, double* acc) const {
-        const XT yi = static_cast<XT>(y);
-        out[i] = yi;
-        acc[0] += static_cast<double>(yi) * wi;
-    }
-};
-
-template <typename XT>
-struct EpiGcr {         // t = A s; <r,s>, <s,s>, <s,t>, <t,t>, <r,t>   (18)-(20)
-    static constexpr int NDOT = 5;
-    XT* t;
-    const XT* r;
-    __device__ double load(long long i) const { return static_cast<double>(r[i]); }
-    __device__ void operator()(long long i, double y, double sc, double rd, double* acc) const {
-        const XT ti = static_cast<XT>(y);
-        t[i] = ti;
-        const double td = static_cast<double>(ti);
-        acc[0] += rd * sc;
-        acc[1] += sc * sc;
-        acc[2] += sc * td;
-        acc[3] += td * td;
-        acc[4] += rd * td;
-    }
-};
-
-struct EpiResid {       // refinement: r = b - A x in FP64, r32 = float(r), <r,r>
-    static constexpr int NDOT = 1;
-    const double* b;
-    float* r32;
-    __device__ double load(long long i) const { return b[i]; }
-    __device__ void operator()(long long i, double y, double, double bi, double* acc) const {
-        const double ri = bi - y;
-        r32[i] = static_cast<float>(ri);
-        acc[0] += ri * ri;
-    }
-};
-
-template <typename XT>
-struct EpiStore {
-    static constexpr int NDOT = 1;
-    XT* out;
-    __device__ double load(long long) const { return 0.; }
-    __device__ void operator()(long long i, double y, double, double, double*) const { out[i] = static_cast<XT>(y); }
-};
-
-// ------------------------------------------------------------------------------ traversals
-// Generic CSR: one row per lane; the row's entries are fetched four at a time with clamped
-// indices (all loads in flight before the multiply-adds), summed in column order.
-template <typename VT, typename XT, class Epi>
-__global__ __launch_bounds__(kThreads) void k_apply_csr(CsrOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
-                                                       KState* st, int tail, int check_status) {
-    double acc[Epi::NDOT];
-#pragma unroll
-    for (int j = 0; j < Epi::NDOT; j++) acc[j] = 0.;
-    const bool run = !(check_status && st->status);
-    if (run) {
-        for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < A.n; i += (long long)gridDim.x * kThreads) {
-            const int b = A.rp[i], e = A.rp[i + 1];
-            const double pre = epi.load(i);
-            const double xi = static_cast<double>(x[i]);
-            double s = 0.;
-            for (int k = b; k < e; k += 4) {
-                double vv[4], xv[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int kk = (k + u < e) ? k + u : b;
-                    vv[u] = static_cast<double>(A.val[kk]);
-                    xv[u] = static_cast<double>(x[A.col[kk]]);
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (k + u < e) s += vv[u] * xv[u];
-            }
-            epi(i, s, xi, pre, acc);
-        }
-    }
-    if (tail >= 0) grid_reduce<Epi::NDOT>(acc, ws, st, run ? tail : TAIL_NONE);
-}
-
-// y_row = sum over the row's slots in the order W,S,B,[X],C,[X],T,N,E (the sorted column
-// order of a non-wrapped row; X is the cross entry, before C for component 1, after it for 0).
-// Every load of a plane is issued before the multiply-adds; entries removed by a boundary
-// edit read a valid in-range address and contribute +0.0 (the partial sum starts at +0.0 and
-// so is never -0.0: adding +0.0 leaves it bit-identical to skipping the term).
-template <int NC, typename VT, typename XT, class Epi>
-__global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
-                                                       KState* st, int tail, int check_status) {
-    double acc[Epi::NDOT];
-#pragma unroll
-    for (int j = 0; j < Epi::NDOT; j++) acc[j] = 0.;
-    const bool run = !(check_status && st->status);
-    const int tile = static_cast<int>(blockIdx.x % kXcd) * A.per_xcd + static_cast<int>(blockIdx.x / kXcd);
-    const int ntiles = A.tiles_k * A.tiles_j * A.tiles_i;
-    if (run && tile < ntiles) {
-        const DiaView& V = A.V;
-        const int tk = tile % A.tiles_k;
-        const int t2 = tile / A.tiles_k;
-        const int tj = t2 % A.tiles_j;
-        const int ti = t2 / A.tiles_j;
-        const int k = tk * kTileK + (threadIdx.x & 63);
-        const int j = tj * kTileJ + (threadIdx.x >> 6);
-        if (k < V.n[2] && j < V.n[1]) {
-            const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
-            const long long sI = static_cast<long long>(n1) * n2, sJ = n2;
-            const long long nr = V.nrows;
-            // j and k neighbours: presence (boundary edits) and wrap
-            const bool hasS = !(j == 0 && V.rm_lo[1]);
-            const bool hasN = !(j == n1 - 1 && V.rm_hi[1] && !(j == 0 && V.rm_lo[1]));
-            const bool hasB = !(k == 0 && V.rm_lo[2]);
-            const bool hasT = !(k == n2 - 1 && V.rm_hi[2] && !(k == 0 && V.rm_lo[2]));
-            const long long dS = !hasS ? 0 : (j == 0) ? (long long)(n1 - 1) * sJ : -sJ;
-            const long long dN = !hasN ? 0 : (j == n1 - 1) ? -(long long)(n1 - 1) * sJ : sJ;
-            const long long dB = !hasB ? 0 : (k == 0) ? (long long)(n2 - 1) : -1;
-            const long long dT = !hasT ? 0 : (k == n2 - 1) ? -(long long)(n2 - 1) : 1;
-            const int i0 = ti * A.tile_i;
-            const int i1 = (i0 + A.tile_i < n0) ? i0 + A.tile_i : n0;
-            long long node = (static_cast<long long>(i0) * n1 + j) * n2 + k;
-            double xp[NC], xc[NC];
-            {
-                const bool need = !(i0 == 0 && V.rm_lo[0]);
-                const long long nprev = !need ? node : (i0 == 0) ? node + (long long)(n0 - 1) * sI : node - sI;
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    xc[c] = static_cast<double>(x[node * NC + c]);
-                    xp[c] = static_cast<double>(x[nprev * NC + c]);
-                }
-            }
-            for (int i = i0; i < i1; i++) {
-                const bool hasW = !(i == 0 && V.rm_lo[0]);
-                const bool hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
-                const long long nnext = !hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
-                double xn[NC], xs[NC], xnn[NC], xb[NC], xt[NC], dv[NC][8], pre[NC];
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    const long long row = node * NC + c;
-                    xn[c] = static_cast<double>(x[nnext * NC + c]);
-                    xs[c] = static_cast<double>(x[(node + dS) * NC + c]);
-                    xnn[c] = static_cast<double>(x[(node + dN) * NC + c]);
-                    xb[c] = static_cast<double>(x[(node + dB) * NC + c]);
-                    xt[c] = static_cast<double>(x[(node + dT) * NC + c]);
-#pragma unroll
-                    for (int sl = 0; sl < 7; sl++) dv[c][sl] = static_cast<double>(A.D[sl * nr + row]);
-                    dv[c][7] = (NC == 2 && V.cross) ? static_cast<double>(A.D[7 * nr + row]) : 0.;
-                    pre[c] = epi.load(row);
-                }
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    double s = 0.;
-                    s += hasW ? dv[c][0] * xp[c] : 0.;
-                    s += hasS ? dv[c][1] * xs[c] : 0.;
-                    s += hasB ? dv[c][2] * xb[c] : 0.;
-                    if (NC == 2 && V.cross && c == 1) s += dv[c][7] * xc[0];
-                    s += dv[c][3] * xc[c];
-                    if (NC == 2 && V.cross && c == 0) s += dv[c][7] * xc[NC - 1];
-                    s += hasT ? dv[c][4] * xt[c] : 0.;
-                    s += hasN ? dv[c][5] * xnn[c] : 0.;
-                    s += hasE ? dv[c][6] * xn[c] : 0.;
-                    epi(node * NC + c, s, xc[c], pre[c], acc);
-                }
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    xp[c] = xc[c];
-                    xc[c] = xn[c];
-                }
-                node += sI;   // a tile never runs past plane n0 - 1, so xc is always a real plane
-            }
-        }
-    }
-    if (tail >= 0) grid_reduce<Epi::NDOT>(acc, ws, st, run ? tail : TAIL_NONE);
-}
-
-// ------------------------------------------------------------------------------ vector updates
-// Grid-stride over chunks of kU * kThreads elements; all loads of a chunk are issued before
-// its arithmetic so that every wave keeps several independent HBM requests in flight.
-// (status != 0 => the pass has broken down: every launch is a no-op)
-constexpr int kU = 4;
-
-#define DARE_CHUNK_LOOP(n)                                                                        \
-    for (long long base_ = blockIdx.x * (long long)(kThreads * kU); base_ < (n);                  \
-         base_ += (long long)gridDim.x * (kThreads * kU))
-#define DARE_IDX(u) (base_ + (long long)(u) * kThreads + threadIdx.x)
-
-template <typename XT>
-__global__ __launch_bounds__(kThreads) void k_upd_u(long long n, const XT* __restrict__ r, XT* __restrict__ u,
-                                                    const KState* __restrict__ st) {
-    if (st->status) return;
-    const double beta = st->beta;
-    DARE_CHUNK_LOOP(n) {
-        double rv[kU], uv[kU];
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
-            rv[q] = static_cast<double>(r[i]);
-            uv[q] = static_cast<double>(u[i]);
-        }
-#pragma unroll
-        for (int q = 0; q < kU; q++)
-            if (DARE_IDX(q) < n) u[DARE_IDX(q)] = static_cast<XT>(rv[q] - beta * uv[q]);          // (5)
-    }
-}
-
-template <typename XT>
-__global__ __launch_bounds__(kThreads) void k_upd_rx(long long n, XT* __restrict__ r, const XT* __restrict__ v,
-                                                     XT* __restrict__ x, const XT* __restrict__ u,
-                                                     const KState* __restrict__ st) {
-    if (st->status) return;
-    const double a = st->alpha;
-    DARE_CHUNK_LOOP(n) {
-        double rv[kU], vv[kU], xv[kU], uv[kU];
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
-            rv[q] = static_cast<double>(r[i]);
-            vv[q] = static_cast<double>(v[i]);
-            xv[q] = static_cast<double>(x[i]);
-            uv[q] = static_cast<double>(u[i]);
-        }
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q);
-            if (i < n) {
-                r[i] = static_cast<XT>(rv[q] - a * vv[q]);                                         // (8)
-                x[i] = static_cast<XT>(1. * xv[q] + a * uv[q]);                                    // (10)
-            }
-        }
-    }
-}
-
-template <typename XT>
-__global__ __launch_bounds__(kThreads) void k_upd_v(long long n, const XT* __restrict__ s, XT* __restrict__ v,
-                                                    const KState* __restrict__ st) {
-    if (st->status) return;
-    const double beta = st->beta;
-    DARE_CHUNK_LOOP(n) {
-        double sv[kU], vv[kU];
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
-            sv[q] = static_cast<double>(s[i]);
-            vv[q] = static_cast<double>(v[i]);
-        }
-#pragma unroll
-        for (int q = 0; q < kU; q++)
-            if (DARE_IDX(q) < n) v[DARE_IDX(q)] = static_cast<XT>(sv[q] - beta * vv[q]);          // (12)
-    }
-}
-
-template <typename XT>
-__global__ __launch_bounds__(kThreads) void k_upd_urs(long long n, XT* __restrict__ u, XT* __restrict__ r,
-                                                      const XT* __restrict__ v, XT* __restrict__ s,
-                                                      const XT* __restrict__ w, const KState* __restrict__ st) {
-    if (st->status) return;
-    const double a = st->alpha, beta = st->beta;
-    DARE_CHUNK_LOOP(n) {
-        double uv[kU], rv[kU], vv[kU], sv[kU], wv[kU];
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
-            uv[q] = static_cast<double>(u[i]);
-            rv[q] = static_cast<double>(r[i]);
-            vv[q] = static_cast<double>(v[i]);
-            sv[q] = static_cast<double>(s[i]);
-            wv[q] = static_cast<double>(w[i]);
-        }
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q);
-            if (i < n) {
-                u[i] = static_cast<XT>(rv[q] - beta * uv[q]);                                      // (15)
-                r[i] = static_cast<XT>(rv[q] - a * vv[q]);                                         // (16)
-                s[i] = static_cast<XT>(sv[q] - a * wv[q]);                                         // (17)
-            }
-        }
-    }
-}
-
-// x += om1 r + om2 s; x += alpha u; r -= om1 s + om2 t; u -= om1 v + om2 w; <r,r>, <rh,r>
-template <typename XT>
-__global__ __launch_bounds__(kThreads) void k_end(long long n, XT* __restrict__ x, XT* __restrict__ r,
-                                                  const XT* __restrict__ s, const XT* __restrict__ t,
-                                                  XT* __restrict__ u, const XT* __restrict__ v,
-                                                  const XT* __restrict__ w, const XT* __restrict__ rh, ReduceWs ws,
-                                                  KState* st) {
-    double acc[2] = {0., 0.};
-    const bool run = st->status == 0;
-    if (run) {
-        const double o1 = st->om1, o2 = st->om2, a = st->alpha;
-        DARE_CHUNK_LOOP(n) {
-            double xv[kU], rv[kU], sv[kU], tv[kU], uv[kU], vv[kU], wv[kU], hv[kU];
-#pragma unroll
-            for (int q = 0; q < kU; q++) {
-                const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
-                xv[q] = static_cast<double>(x[i]);
-                rv[q] = static_cast<double>(r[i]);
-                sv[q] = static_cast<double>(s[i]);
-                tv[q] = static_cast<double>(t[i]);
-                uv[q] = static_cast<double>(u[i]);
-                vv[q] = static_cast<double>(v[i]);
-                wv[q] = static_cast<double>(w[i]);
-                hv[q] = static_cast<double>(rh[i]);
-            }
-#pragma unroll
-            for (int q = 0; q < kU; q++) {
-                const long long i = DARE_IDX(q);
-                if (i < n) {
-                    double xi = 1. * xv[q] + o1 * rv[q] + o2 * sv[q];                           // (24)
-                    xi = 1. * xi + a * uv[q];
-                    x[i] = static_cast<XT>(xi);
-                    const XT rn = static_cast<XT>(1. * rv[q] + (-o1) * sv[q] + (-o2) * tv[q]);  // (25)
-                    r[i] = rn;
-                    u[i] = static_cast<XT>(1. * uv[q] + (-o1) * vv[q] + (-o2) * wv[q]);
-                    const double rd = static_cast<double>(rn);
-                    acc[0] += rd * rd;
-                    acc[1] += hv[q] * rd;
-                }
-            }
-        }
-    }
-    grid_reduce<2>(acc, ws, st, run ? TAIL_END : TAIL_NONE);
-}
-
-template <typename XT>
-__global__ __launch_bounds__(kThreads) void k_dot(long long n, const XT* __restrict__ a, const XT* __restrict__ b,
-                                                  ReduceWs ws, KState* st) {
-    double acc[1] = {0.};
-    DARE_CHUNK_LOOP(n) {
-        double av[kU], bv[kU];
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
-            av[q] = static_cast<double>(a[i]);
-            bv[q] = static_cast<double>(b[i]);
-        }
-#pragma unroll
-        for (int q = 0; q < kU; q++)
-            if (DARE_IDX(q) < n) acc[0] += av[q] * bv[q];
-    }
-    grid_reduce<1>(acc, ws, st, TAIL_STORE);
-}
-
-__global__ __launch_bounds__(kThreads) void k_axpby(long long n, double a, const double* __restrict__ x, double b,
-                                                    double* __restrict__ y) {
-    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
-        y[i] = a * x[i] + b * y[i];
-}
-
-__global__ __launch_bounds__(kThreads) void k_add_correction(long long n, double* __restrict__ x,
-                                                             const float* __restrict__ d) {
-    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
-        x[i] = x[i] + static_cast<double>(d[i]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_zero(long long n, T* __restrict__ p) {
-    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
-        p[i] = T(0);
-}
-
-__global__ __launch_bounds__(kThreads) void k_to_f32(long long n, const double* __restrict__ a, float* __restrict__ b) {
-    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
-        b[i] = static_cast<float>(a[i]);
-}
-
-__global__ void k_clear_state(KState* st) {
-    if (threadIdx.x == 0) {
-        st->status = 0;
-        st->next_status = 0;
-    }
-}
-
-inline unsigned grid_rows(long long n) {      // one row per lane per sweep (CSR traversal)
-    long long g = (n + kThreads - 1) / kThreads;
-    if (g > kGridMax) g = kGridMax;
-    if (g < 1) g = 1;
-    return static_cast<unsigned>(g);
-}
-
-inline unsigned grid_for(long long n) {
-    long long g = (n + kThreads * kU - 1) / (kThreads * kU);
-    if (g > kGridMax) g = kGridMax;
-    if (g < 1) g = 1;
-    return static_cast<unsigned>(g);
-}
 
 // An operator the solver can apply: CSR or diagonal view, value type VT.
 template <typename VT>

@@ -82,3 +82,65 @@ def c3_lognormal(n, device="cuda", seed=3, mean_diff_number=1.0):
     # pick dt so that the mean of D dt / dx^2 equals mean_diff_number
     dt = mean_diff_number * dx * dx / float(D.mean())
     return D, src, dt
+
+
+class C4Batch:
+    """Batch of independent implicit 2-component reaction-diffusion systems.
+
+    Per system: edge lengths n0, n1, n2 log-uniform in [nmin, nmax] (isotropic cells of width
+    1/64), D1, D2 ~ U[1e-4, 1e-2] (one per component), and per cell a linearised reaction
+    block [[-k1, k2], [k1, -k2]] with k1, k2 ~ U[0.1, 10]; implicit Euler with dt, Dirichlet 0
+    on every face; phi_old ~ U[0, 1) per component. Each system's structure is built and its
+    values assembled straight into the rows of the batch's concatenated b and x.
+    """
+
+    def __init__(self, ctx, nsys=512, seed=4, nmin=16, nmax=96, dt=1e-2, device="cuda"):
+        from . import Batch, Csr, Grid, Terms, build_structure, check, lib
+        torch = _torch()
+        rng = np.random.default_rng(seed)
+        dims = np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=(nsys, 3)))
+        self.dims = np.clip(np.rint(dims), nmin, nmax).astype(int)
+        self.D = rng.uniform(1e-4, 1e-2, size=(nsys, 2))
+        self.dt = dt
+        self.nsys = nsys
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed)
+        self.grids, self.terms, self.csr, self.phi, self.react = [], [], [], [], []
+        h = 1.0 / 64
+        for s in range(nsys):
+            n = [int(v) for v in self.dims[s]]
+            g = Grid.make(n, size=[v * h for v in n], ghost=1, ncomp=2)
+            ncell = n[0] * n[1] * n[2]
+            k = 0.1 + 9.9 * torch.rand((ncell, 2), generator=gen, dtype=torch.float64, device=device)
+            R = torch.stack([-k[:, 0], k[:, 1], k[:, 0], -k[:, 1]], dim=1).contiguous().view(-1)
+            t = Terms.make(dt=dt, diff_comp=[float(self.D[s, 0]), float(self.D[s, 1])], react=R,
+                           bc_type=[BC_DIRICHLET] * 6, bc_value=[0.0] * 6)
+            A = Csr(ctx)
+            build_structure(ctx, A, g, t)
+            npad = (n[0] + 2) * (n[1] + 2) * (n[2] + 2) * 2
+            phi = torch.rand(npad, generator=gen, dtype=torch.float64, device=device)
+            self.grids.append(g)
+            self.terms.append(t)
+            self.csr.append(A)
+            self.phi.append(phi)
+            self.react.append(R)
+        self.batch = Batch(ctx, self.csr)
+        self.ctx = ctx
+        self._check, self._lib = check, lib
+
+    @property
+    def cells(self):
+        return int(np.prod(self.dims, axis=1).sum())
+
+    def assemble(self):
+        """Values, rhs and x0 (= phi_old) of every system, into the batch's vectors."""
+        from . import fill_values
+        for s in range(self.nsys):
+            fill_values(self.ctx, self.csr[s], self.grids[s], self.terms[s], self.phi[s],
+                        self.batch.rhs_ptr(s), self.batch.x_ptr(s))
+        self.ctx.synchronize()
+
+    def close(self):
+        self.batch.close()
+        for A in self.csr:
+            A.close()

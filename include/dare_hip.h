@@ -185,6 +185,38 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* csr, const double* b, do
 int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* csr, const double* b, double* x,
                             const dare_solver_params* p, dare_solver_result* res);
 
+/* ---- batched solves ---------------------------------------------------------------------
+ * Many independent systems in one set of persistent launches. The reference solves one
+ * system per TrilinosSolver::Solve call (TrilinosSolver.inl:71-93); a batch is the same
+ * BiCGStab2 (BiCGStab2.inl:167-582) run for every system with a per-system convergence mask.
+ * Systems are dare_csr handles built by dare_csr_build_structure on 3-D grids (all with the
+ * same number of components, 1 or 2); their values come from dare_csr_fill_values. The batch
+ * owns the concatenated right-hand sides and solutions: system s occupies rows
+ * [row_offsets[s], row_offsets[s+1]) of b and x (fill them, e.g. by passing b + offset and
+ * x + offset to dare_csr_fill_values). */
+typedef struct dare_batch dare_batch;
+typedef struct {
+    int32_t tile_k;   /* lanes along k per work-group: 16, 32, 64, or 0 = by each system's nz */
+    int32_t tile_i;   /* planes per tile */
+    int32_t grid;     /* work-groups of the persistent launches (multiple of 8) */
+    int32_t order;    /* work list: 0 system-major, 1 XCD slabs, 2 XCD slabs largest first */
+} dare_batch_config;
+
+int dare_batch_create(dare_ctx* ctx, int32_t nsys, dare_csr* const* systems, dare_batch** out);
+int dare_batch_destroy(dare_batch* batch);
+int dare_batch_vectors(dare_batch* batch, double** b, double** x, int64_t* row_offsets /* nsys + 1 */);
+int dare_batch_set_config(dare_ctx* ctx, dare_batch* batch, const dare_batch_config* cfg);
+int dare_batch_get_config(const dare_batch* batch, dare_batch_config* cfg);
+/* Times every candidate configuration on this batch (FP32 operator sweep, `reps` launches
+ * each), keeps the fastest and returns it with its time per sweep in microseconds. */
+int dare_batch_autotune(dare_ctx* ctx, dare_batch* batch, int32_t reps, dare_batch_config* best, double* best_us);
+/* Solves every system. mixed = 0: FP64 BiCGStab2 (tol, tol_kind, max_it as in
+ * dare_bicgstab2_solve); mixed = 1: FP32 inner solves inside FP64 refinement (as in
+ * dare_refine_solve_mixed). graph_passes > 0 captures that many passes in a HIP graph that is
+ * replayed until every system has stopped; 0 launches pass by pass. results[nsys]. */
+int dare_batch_solve(dare_ctx* ctx, dare_batch* batch, const dare_solver_params* p, int32_t mixed,
+                     int32_t graph_passes, dare_solver_result* results);
+
 #ifdef __cplusplus
 }
 #endif
