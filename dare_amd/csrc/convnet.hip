@@ -1,0 +1,924 @@
+// Learned warm start for the implicit step: a 3-level 3-D convolutional encoder-decoder
+// (16/32/64 channels, 3x3x3 kernels, no biases) that predicts phi^{n+1} from phi^n,
+// phi^{n-1} and the right-hand side, used as BiCGStab2's initial guess. No counterpart in the
+// reference (its guess is phi^n, MatrixSystem/Trilinos.inl:316-317); the solver still iterates
+// to the same FP64 tolerance, so a poor guess costs passes, never accuracy.
+//
+// Tensors are channels-last BF16 ([voxel][channel], voxel = (z * H + y) * W + x with x = the
+// grid's fastest index k), weights BF16 with FP32 master copies, accumulation in FP32 on the
+// matrix cores. Every convolution is an implicit GEMM on v_mfma_f32_16x16x16_bf16: a wave
+// computes 16 consecutive voxels x all output channels, with K running over 27 taps x input
+// channels in blocks of 16. Back-propagation of the data gradient is the same kernel with the
+// weights flipped and transposed; the weight gradient is a GEMM over voxels (M = output
+// channels, N = input channels) written as per-chunk partials and summed in chunk order.
+//
+// Network (layer: in -> out channels, resolution):
+//   L1 16->16 (input channels padded to 16), L2 16->16            e1   full
+//   pool, L3 16->32, L4 32->32                                     e2   1/2
+//   pool, L5 32->64, L6 64->64                                     e3   1/4
+//   up,   L7 64->32, + e2, L8 32->32                               d2   1/2
+//   up,   L9 32->16, + e1, L10 16->16                              d1   full
+//   L11 16->16 (only output channel 0 is trained and read)        out  full, FP32
+// ReLU after every layer but L11. Prediction: phi^{n+1} = phi^n + S * out, S = max|phi^n - phi^{n-1}|.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace dare_hip {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNetThreads = 256;
+constexpr int kLayers = 11;
+constexpr int kInChannels = 16;
+
+struct LayerDef {
+    int cin, cout, level;    // level 0 full, 1 half, 2 quarter resolution
+};
+constexpr LayerDef kDefs[kLayers] = {{16, 16, 0}, {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2}, {64, 64, 2},
+                                     {64, 32, 1}, {32, 32, 1}, {32, 16, 0}, {16, 16, 0}, {16, 16, 0}};
+
+// FP32 -> BF16, round to nearest even; NaN stays NaN.
+__device__ __host__ inline uint16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+__device__ __host__ inline float bf2f(uint16_t h) {
+    return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16);
+}
+
+// ------------------------------------------------------------------------------ convolution
+// Y[v][co] = sum_tap sum_ci X[v + off(tap)][ci] * Wt[tap][co][ci], zero outside the grid.
+// Block = 4 waves = 64 voxels along x at one (z, y). MFMA 16x16x16 operand layout: lane l
+// holds A[l % 16][4 (l / 16) + e] (voxel, input channel), B[4 (l / 16) + e][l % 16]
+// (input channel, output channel) and D[4 (l / 16) + e][l % 16] (voxel, output channel).
+// OUT: 0 BF16 all channels, 1 FP32 channel 0 only.
+template <int CIN, int COUT, bool RELU, int OUT>
+__global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
+                                                     uint16_t* __restrict__ Y, float* __restrict__ Yf, int D, int H,
+                                                     int W) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 64 + wave * 16;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x0 >= W) return;                       // whole wave (no block barrier below)
+    const int i = lane & 15, kq = lane >> 4;
+    floatx4 acc[COUT / 16];
+#pragma unroll
+    for (int nb = 0; nb < COUT / 16; nb++) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int dz = -1; dz <= 1; dz++) {
+        const int zz = z + dz;
+        if (zz < 0 || zz >= D) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            const int yy = y + dy;
+            if (yy < 0 || yy >= H) continue;
+            const long long rowv = (static_cast<long long>(zz) * H + yy) * W;
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                const int xx = x0 + i + dx;
+                const bool inb = xx >= 0 && xx < W;
+                const uint16_t* xp = X + (rowv + (inb ? xx : 0)) * CIN + kq * 4;
+                const uint16_t* wp = Wt + (static_cast<long long>(tap) * COUT + i) * CIN + kq * 4;
+#pragma unroll
+                for (int cb = 0; cb < CIN / 16; cb++) {
+                    bf16x4 a = inb ? *reinterpret_cast<const bf16x4*>(xp + cb * 16) : bf16x4{0, 0, 0, 0};
+#pragma unroll
+                    for (int nb = 0; nb < COUT / 16; nb++) {
+                        const bf16x4 b = *reinterpret_cast<const bf16x4*>(wp + nb * 16 * CIN + cb * 16);
+                        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+    const long long vrow = (static_cast<long long>(z) * H + y) * W;
+#pragma unroll
+    for (int nb = 0; nb < COUT / 16; nb++) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int xv = x0 + kq * 4 + e;
+            if (xv >= W) continue;
+            float v = acc[nb][e];
+            if (RELU) v = v > 0.f ? v : 0.f;
+            if (OUT == 0) {
+                Y[(vrow + xv) * COUT + nb * 16 + i] = f2bf(v);
+            } else if (nb == 0 && i == 0) {
+                Yf[vrow + xv] = v;
+            }
+        }
+    }
+}
+
+// Weight gradient partials: P[c][tap][co][ci] = sum over the voxels of chunk c of
+// G[v][co] * X[v + off(tap)][ci]. Wave = one chunk of (z, y) rows for one 16x16 (co, ci)
+// block; A[co][voxel] = G, B[voxel][ci] = shifted X; 27 accumulators, one per tap.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(kNetThreads) void k_wgrad(const uint16_t* __restrict__ X, const uint16_t* __restrict__ G,
+                                                      int D, int H, int W, int rows_per_chunk, int nchunks,
+                                                      float* __restrict__ P) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int chunk = blockIdx.x * 4 + wave;
+    if (chunk >= nchunks) return;
+    constexpr int CB = CIN / 16;
+    const int cob = blockIdx.y / CB, cib = blockIdx.y % CB;
+    const int i = lane & 15, kq = lane >> 4;
+    floatx4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; t++) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int rows = D * H;
+    const int r0 = chunk * rows_per_chunk;
+    const int r1 = (r0 + rows_per_chunk < rows) ? r0 + rows_per_chunk : rows;
+    for (int r = r0; r < r1; r++) {
+        const int z = r / H, y = r % H;
+        const long long vrow = static_cast<long long>(r) * W;
+        for (int xs = 0; xs < W; xs += 16) {
+            bf16x4 a;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int xv = xs + kq * 4 + e;
+                a[e] = xv < W ? static_cast<short>(G[(vrow + xv) * COUT + cob * 16 + i]) : short(0);
+            }
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                const int zz = z + dz;
+                if (zz < 0 || zz >= D) continue;
+#pragma unroll
+                for (int dy = -1; dy <= 1; dy++) {
+                    const int yy = y + dy;
+                    if (yy < 0 || yy >= H) continue;
+                    const long long srow = (static_cast<long long>(zz) * H + yy) * W;
+#pragma unroll
+                    for (int dx = -1; dx <= 1; dx++) {
+                        bf16x4 b;
+#pragma unroll
+                        for (int e = 0; e < 4; e++) {
+                            const int xv = xs + kq * 4 + e;
+                            const int xx = xv + dx;
+                            const bool ok = xv < W && xx >= 0 && xx < W;
+                            b[e] = ok ? static_cast<short>(X[(srow + xx) * CIN + cib * 16 + i]) : short(0);
+                        }
+                        const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                        acc[tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tap], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 27; t++)
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            P[((static_cast<long long>(chunk) * 27 + t) * COUT + cob * 16 + kq * 4 + e) * CIN + cib * 16 + i] = acc[t][e];
+}
+
+// grad[idx] = sum over chunks in chunk order (deterministic)
+__global__ __launch_bounds__(kNetThreads) void k_chunk_sum(const float* __restrict__ P, int nchunks, int n,
+                                                          float* __restrict__ grad) {
+    const int idx = blockIdx.x * kNetThreads + threadIdx.x;
+    if (idx >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; c++) s += P[static_cast<long long>(c) * n + idx];
+    grad[idx] = s;
+}
+
+// ------------------------------------------------------------------------------ resampling
+// out (coarse, dims D x H x W) = mean of the 8 children of the fine grid (2D x 2H x 2W)
+__global__ __launch_bounds__(kNetThreads) void k_pool(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int C,
+                                                     int D, int H, int W) {
+    const long long n = static_cast<long long>(D) * H * W * C;
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int c = static_cast<int>(q % C);
+        long long v = q / C;
+        const int x = static_cast<int>(v % W);
+        v /= W;
+        const int y = static_cast<int>(v % H);
+        const int z = static_cast<int>(v / H);
+        float s = 0.f;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int d = 0; d < 2; d++) {
+                    const long long f = ((static_cast<long long>(2 * z + a) * (2 * H) + (2 * y + b)) * (2 * W) + (2 * x + d));
+                    s += bf2f(in[f * C + c]);
+                }
+        out[q] = f2bf(s * 0.125f);
+    }
+}
+
+// out (fine, D x H x W) = parent value (nearest upsampling)
+__global__ __launch_bounds__(kNetThreads) void k_up(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int C,
+                                                   int D, int H, int W) {
+    const long long n = static_cast<long long>(D) * H * W * C;
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int c = static_cast<int>(q % C);
+        long long v = q / C;
+        const int x = static_cast<int>(v % W);
+        v /= W;
+        const int y = static_cast<int>(v % H);
+        const int z = static_cast<int>(v / H);
+        const long long p = (static_cast<long long>(z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2;
+        out[q] = in[p * C + c];
+    }
+}
+
+// out[parent] = (y[parent] > 0) ? sum of the 8 children of du : 0   (up-sampling backward + ReLU)
+__global__ __launch_bounds__(kNetThreads) void k_up_bwd_mask(const uint16_t* __restrict__ du, const uint16_t* __restrict__ y,
+                                                            uint16_t* __restrict__ out, int C, int D, int H, int W) {
+    const long long n = static_cast<long long>(D) * H * W * C;
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int c = static_cast<int>(q % C);
+        long long v = q / C;
+        const int x = static_cast<int>(v % W);
+        v /= W;
+        const int yy = static_cast<int>(v % H);
+        const int z = static_cast<int>(v / H);
+        float s = 0.f;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int d = 0; d < 2; d++) {
+                    const long long f = ((static_cast<long long>(2 * z + a) * (2 * H) + (2 * yy + b)) * (2 * W) + (2 * x + d));
+                    s += bf2f(du[f * C + c]);
+                }
+        out[q] = bf2f(y[q]) > 0.f ? f2bf(s) : uint16_t(0);
+    }
+}
+
+// out[child] = (y[child] > 0) ? extra[child] + dp[parent] / 8 : 0   (pooling backward + skip + ReLU)
+__global__ __launch_bounds__(kNetThreads) void k_pool_bwd_add_mask(const uint16_t* __restrict__ dp,
+                                                                  const uint16_t* __restrict__ extra,
+                                                                  const uint16_t* __restrict__ y,
+                                                                  uint16_t* __restrict__ out, int C, int D, int H, int W) {
+    const long long n = static_cast<long long>(D) * H * W * C;
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int c = static_cast<int>(q % C);
+        long long v = q / C;
+        const int x = static_cast<int>(v % W);
+        v /= W;
+        const int yy = static_cast<int>(v % H);
+        const int z = static_cast<int>(v / H);
+        const long long p = (static_cast<long long>(z / 2) * (H / 2) + yy / 2) * (W / 2) + x / 2;
+        const float g = bf2f(extra[q]) + 0.125f * bf2f(dp[p * C + c]);
+        out[q] = bf2f(y[q]) > 0.f ? f2bf(g) : uint16_t(0);
+    }
+}
+
+__global__ __launch_bounds__(kNetThreads) void k_add(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b,
+                                                    uint16_t* __restrict__ out, long long n) {
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads)
+        out[q] = f2bf(bf2f(a[q]) + bf2f(b[q]));
+}
+
+__global__ __launch_bounds__(kNetThreads) void k_mask(const uint16_t* __restrict__ g, const uint16_t* __restrict__ y,
+                                                     uint16_t* __restrict__ out, long long n) {
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads)
+        out[q] = bf2f(y[q]) > 0.f ? g[q] : uint16_t(0);
+}
+
+// ------------------------------------------------------------------------------ solver side
+// Normalisation scales: s[0] = max|phi^n|, s[1] = max|phi^n - phi^{n-1}|,
+// s[2] = max|rhs * rhs_scale - phi^n| over the internal cells (block maxima, then one block).
+struct FieldGeo {
+    int n0, n1, n2, g;
+    long long s0, s1;    // padded strides of i and j
+};
+
+__device__ inline long long pad_index(const FieldGeo& F, int z, int y, int x) {
+    return (z + F.g) * F.s0 + (y + F.g) * F.s1 + (x + F.g);
+}
+
+__global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double* __restrict__ phi,
+                                                       const double* __restrict__ phim, const double* __restrict__ rhs,
+                                                       double rhs_scale, double* __restrict__ part) {
+    __shared__ double red[3][kNetThreads / 64];
+    double m0 = 0., m1 = 0., m2 = 0.;
+    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int x = static_cast<int>(q % F.n2);
+        const int y = static_cast<int>((q / F.n2) % F.n1);
+        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
+        const double p = phi[pad_index(F, z, y, x)];
+        const double pm = phim[pad_index(F, z, y, x)];
+        m0 = fmax(m0, fabs(p));
+        m1 = fmax(m1, fabs(p - pm));
+        m2 = fmax(m2, fabs(rhs[q] * rhs_scale - p));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmax(m0, __shfl_down(m0, off));
+        m1 = fmax(m1, __shfl_down(m1, off));
+        m2 = fmax(m2, __shfl_down(m2, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = m0;
+        red[1][threadIdx.x >> 6] = m1;
+        red[2][threadIdx.x >> 6] = m2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double m = 0.;
+        for (int w = 0; w < kNetThreads / 64; w++) m = fmax(m, red[threadIdx.x][w]);
+        part[threadIdx.x * 1024 + blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_scales_finish(const double* __restrict__ part, int nblocks, double* __restrict__ s) {
+    if (threadIdx.x >= 3) return;
+    double m = 0.;
+    for (int b = 0; b < nblocks; b++) m = fmax(m, part[threadIdx.x * 1024 + b]);
+    // floors keep the channels finite when a field is flat (first step of a trajectory)
+    if (threadIdx.x == 0) s[0] = fmax(m, 1e-300);
+    else s[threadIdx.x] = fmax(m, 1e-300);
+}
+
+// x0[v][c]: c0 = phi^n / s0, c1 = (phi^n - phi^{n-1}) / s1, c2 = (rhs * rhs_scale - phi^n) / s2,
+// c3.. = (extra_e - offset_e) * scale_e (static fields), the rest 0.
+struct Extras {
+    const double* f[8];
+    double scale[8], offset[8];
+    int n;
+};
+
+__global__ __launch_bounds__(kNetThreads) void k_pack(FieldGeo F, const double* __restrict__ phi,
+                                                     const double* __restrict__ phim, const double* __restrict__ rhs,
+                                                     double rhs_scale, Extras ex, const double* __restrict__ s,
+                                                     uint16_t* __restrict__ x0) {
+    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    const double i0 = 1. / s[0], i1 = 1. / s[1], i2 = 1. / s[2];
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int x = static_cast<int>(q % F.n2);
+        const int y = static_cast<int>((q / F.n2) % F.n1);
+        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
+        const long long pi = pad_index(F, z, y, x);
+        const double p = phi[pi];
+        uint16_t ch[kInChannels];
+        ch[0] = f2bf(static_cast<float>(p * i0));
+        ch[1] = f2bf(static_cast<float>((p - phim[pi]) * i1));
+        ch[2] = f2bf(static_cast<float>((rhs[q] * rhs_scale - p) * i2));
+        for (int c = 3; c < kInChannels; c++) {
+            const int e = c - 3;
+            ch[c] = e < ex.n ? f2bf(static_cast<float>((ex.f[e][pi] - ex.offset[e]) * ex.scale[e])) : uint16_t(0);
+        }
+        for (int c = 0; c < kInChannels; c++) x0[q * kInChannels + c] = ch[c];
+    }
+}
+
+// x[row] = phi^n + s1 * out[row] (the guess, solver row order = voxel order)
+__global__ __launch_bounds__(kNetThreads) void k_unpack(FieldGeo F, const double* __restrict__ phi,
+                                                       const float* __restrict__ out, const double* __restrict__ s,
+                                                       double* __restrict__ xg) {
+    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    const double s1 = s[1];
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int x = static_cast<int>(q % F.n2);
+        const int y = static_cast<int>((q / F.n2) % F.n1);
+        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
+        xg[q] = phi[pad_index(F, z, y, x)] + s1 * static_cast<double>(out[q]);
+    }
+}
+
+// loss = mean((out - t)^2), t = (x_true - phi^n) / s1; dout[v][0] = 2 (out - t) / n, other channels 0.
+__global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* __restrict__ phi,
+                                                    const double* __restrict__ xt, const float* __restrict__ out,
+                                                    const double* __restrict__ s, uint16_t* __restrict__ dout,
+                                                    double* __restrict__ part) {
+    __shared__ double red[kNetThreads / 64];
+    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    const double inv = 1. / s[1];
+    double acc = 0.;
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
+        const int x = static_cast<int>(q % F.n2);
+        const int y = static_cast<int>((q / F.n2) % F.n1);
+        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
+        const double t = (xt[q] - phi[pad_index(F, z, y, x)]) * inv;
+        const double d = static_cast<double>(out[q]) - t;
+        acc += d * d;
+        dout[q * kInChannels] = f2bf(static_cast<float>(2. * d / static_cast<double>(n)));
+        for (int c = 1; c < kInChannels; c++) dout[q * kInChannels + c] = 0;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        for (int w = 0; w < kNetThreads / 64; w++) t += red[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_mse_finish(const double* __restrict__ part, int nblocks, long long n,
+                                                  double* __restrict__ loss) {
+    if (threadIdx.x != 0) return;
+    double t = 0.;
+    for (int b = 0; b < nblocks; b++) t += part[b];
+    loss[0] = t / static_cast<double>(n);
+}
+
+// ------------------------------------------------------------------------------ parameters
+// Adam (Kingma & Ba 2015) on the FP32 master weights of one layer, then the two BF16 copies:
+// forward [tap][co][ci] and data-backward [26 - tap][ci][co].
+__global__ __launch_bounds__(kNetThreads) void k_adam_cast(float* __restrict__ w, const float* __restrict__ g,
+                                                          float* __restrict__ m, float* __restrict__ v,
+                                                          uint16_t* __restrict__ wf, uint16_t* __restrict__ wb,
+                                                          int cin, int cout, int update, float lr, float b1, float b2,
+                                                          float eps, float c1, float c2) {
+    const int n = 27 * cin * cout;
+    const int idx = blockIdx.x * kNetThreads + threadIdx.x;
+    if (idx >= n) return;
+    float wv = w[idx];
+    if (update) {
+        const float gv = g[idx];
+        const float mv = b1 * m[idx] + (1.f - b1) * gv;
+        const float vv = b2 * v[idx] + (1.f - b2) * gv * gv;
+        m[idx] = mv;
+        v[idx] = vv;
+        wv -= lr * (mv / c1) / (sqrtf(vv / c2) + eps);
+        w[idx] = wv;
+    }
+    const int ci = idx % cin;
+    const int co = (idx / cin) % cout;
+    const int tap = idx / (cin * cout);
+    const uint16_t h = f2bf(wv);
+    wf[idx] = h;
+    wb[((26 - tap) * cin + ci) * cout + co] = h;
+}
+
+inline unsigned ew_grid(long long n) {
+    long long g = (n + kNetThreads - 1) / kNetThreads;
+    if (g > 8192) g = 8192;
+    return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+}  // namespace
+}  // namespace dare_hip
+
+using namespace dare_hip;
+
+struct dare_convnet {
+    int device = 0;
+    int in_channels = 3;
+    int max_dims[3] = {0, 0, 0};
+    int dims[3] = {0, 0, 0};              // current full-resolution dims (divisible by 4)
+    long long max_v0 = 0;
+    // parameters
+    long long off[kLayers + 1] = {};
+    long long nparams = 0;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    uint16_t *wf = nullptr, *wb = nullptr;
+    int adam_t = 0;
+    // forward activations
+    uint16_t *x0 = nullptr, *a1 = nullptr, *e1 = nullptr, *p1 = nullptr, *a3 = nullptr, *e2 = nullptr, *p2 = nullptr,
+             *a5 = nullptr, *e3 = nullptr, *u2 = nullptr, *a7 = nullptr, *s2 = nullptr, *d2 = nullptr, *u1 = nullptr,
+             *a9 = nullptr, *s1 = nullptr, *d1 = nullptr;
+    float* out = nullptr;
+    // backward buffers (allocated on first use)
+    bool bwd_ready = false;
+    uint16_t *dout = nullptr, *t0a = nullptr, *t0b = nullptr, *t0c = nullptr, *t0w = nullptr, *h1a = nullptr,
+             *h1b = nullptr, *h1c = nullptr, *h1w = nullptr, *q2a = nullptr, *q2b = nullptr;
+    float* wpart = nullptr;
+    long long wpart_floats = 0;
+    // scalars
+    double* scales = nullptr;             // 3
+    double* part = nullptr;               // 3 * 1024
+    double* loss = nullptr;
+    double* h_pinned = nullptr;           // 4
+};
+
+namespace {
+
+constexpr long long kWpartFloats = 16LL << 20;
+
+template <typename T>
+int dalloc(T** p, long long n) {
+    if (hipMalloc(p, sizeof(T) * (n > 0 ? n : 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("convnet allocation failed");
+        return DARE_ERR_OOM;
+    }
+    return DARE_OK;
+}
+
+long long level_voxels(const int* d, int level) {
+    return static_cast<long long>(d[0] >> level) * (d[1] >> level) * (d[2] >> level);
+}
+
+template <int CIN, int COUT, bool RELU, int OUT>
+void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, float* Yf, const int* d, int level) {
+    const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
+    dim3 grid((W + 63) / 64, H, D);
+    hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT>), grid, dim3(kNetThreads), 0, sm, X, Wt, Y, Yf, D, H, W);
+}
+
+// dispatch on (cin, cout) of the network's layers
+void conv_any(hipStream_t sm, int cin, int cout, bool relu, const uint16_t* X, const uint16_t* Wt, uint16_t* Y,
+              const int* d, int level) {
+#define DARE_CONV_CASE(CI, CO)                                                     \
+    if (cin == CI && cout == CO) {                                                 \
+        if (relu) conv<CI, CO, true, 0>(sm, X, Wt, Y, nullptr, d, level);          \
+        else conv<CI, CO, false, 0>(sm, X, Wt, Y, nullptr, d, level);              \
+        return;                                                                    \
+    }
+    DARE_CONV_CASE(16, 16)
+    DARE_CONV_CASE(16, 32)
+    DARE_CONV_CASE(32, 16)
+    DARE_CONV_CASE(32, 32)
+    DARE_CONV_CASE(32, 64)
+    DARE_CONV_CASE(64, 32)
+    DARE_CONV_CASE(64, 64)
+#undef DARE_CONV_CASE
+}
+
+int wgrad_any(dare_convnet* net, hipStream_t sm, int layer, const uint16_t* X, const uint16_t* G) {
+    const int cin = kDefs[layer].cin, cout = kDefs[layer].cout, level = kDefs[layer].level;
+    const int D = net->dims[0] >> level, H = net->dims[1] >> level, W = net->dims[2] >> level;
+    const int rows = D * H;
+    const int pairs = (cin / 16) * (cout / 16);
+    long long cap = std::min<long long>(512, kWpartFloats / (27LL * cin * cout));
+    cap = std::min<long long>(cap, std::max(1, 4096 / pairs));
+    const int chunks0 = static_cast<int>(std::max<long long>(1, std::min<long long>(rows, cap)));
+    const int rpc = (rows + chunks0 - 1) / chunks0;
+    const int nchunks = (rows + rpc - 1) / rpc;
+    dim3 grid((nchunks + 3) / 4, pairs);
+#define DARE_WG_CASE(CI, CO)                                                                              \
+    if (cin == CI && cout == CO)                                                                          \
+        hipLaunchKernelGGL((k_wgrad<CI, CO>), grid, dim3(kNetThreads), 0, sm, X, G, D, H, W, rpc, nchunks, \
+                           net->wpart);
+    DARE_WG_CASE(16, 16)
+    DARE_WG_CASE(16, 32)
+    DARE_WG_CASE(32, 16)
+    DARE_WG_CASE(32, 32)
+    DARE_WG_CASE(32, 64)
+    DARE_WG_CASE(64, 32)
+    DARE_WG_CASE(64, 64)
+#undef DARE_WG_CASE
+    const int n = 27 * cin * cout;
+    hipLaunchKernelGGL(k_chunk_sum, dim3((n + kNetThreads - 1) / kNetThreads), dim3(kNetThreads), 0, sm, net->wpart,
+                       nchunks, n, net->g + net->off[layer]);
+    return DARE_OK;
+}
+
+FieldGeo field_geo(const dare_grid* g) {
+    FieldGeo F;
+    F.n0 = g->n[0];
+    F.n1 = g->n[1];
+    F.n2 = g->n[2];
+    F.g = g->ghost;
+    const long long p1 = g->n[1] + 2 * g->ghost, p2 = g->n[2] + 2 * g->ghost;
+    F.s1 = p2;
+    F.s0 = p1 * p2;
+    return F;
+}
+
+int check_grid(const dare_convnet* net, const dare_grid* g) {
+    DARE_CHECK_ARG(g && g->dim == 3 && g->ncomp == 1, "the warm-start network takes 3-D scalar grids");
+    for (int d = 0; d < 3; d++) {
+        DARE_CHECK_ARG(g->n[d] % 4 == 0 && g->n[d] >= 4, "grid sizes must be multiples of 4");
+        DARE_CHECK_ARG(g->n[d] <= net->max_dims[d], "grid larger than the network was created for");
+    }
+    DARE_CHECK_ARG(static_cast<long long>(g->n[0]) * g->n[1] * g->n[2] <= net->max_v0, "grid too large");
+    return DARE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dare_convnet_create(dare_ctx* ctx, int32_t in_channels, const int32_t* max_dims, dare_convnet** out) {
+    DARE_CHECK_ARG(ctx && max_dims && out, "NULL argument");
+    DARE_CHECK_ARG(in_channels >= 3 && in_channels <= kInChannels, "in_channels must be in [3, 16]");
+    for (int d = 0; d < 3; d++) DARE_CHECK_ARG(max_dims[d] >= 4 && max_dims[d] % 4 == 0, "dims must be multiples of 4");
+    *out = nullptr;
+    DARE_HIP_TRY(hipSetDevice(ctx->device));
+    dare_convnet* net = new dare_convnet();
+    net->device = ctx->device;
+    net->in_channels = in_channels;
+    for (int d = 0; d < 3; d++) net->max_dims[d] = net->dims[d] = max_dims[d];
+    const long long V0 = static_cast<long long>(max_dims[0]) * max_dims[1] * max_dims[2];
+    net->max_v0 = V0;
+    const long long V1 = V0 / 8, V2 = V0 / 64;
+    for (int l = 0; l < kLayers; l++) net->off[l + 1] = net->off[l] + 27LL * kDefs[l].cin * kDefs[l].cout;
+    net->nparams = net->off[kLayers];
+    int rc = DARE_OK;
+    auto A = [&](auto** p, long long n) {
+        if (rc == DARE_OK) rc = dalloc(p, n);
+    };
+    A(&net->w, net->nparams);
+    A(&net->g, net->nparams);
+    A(&net->m, net->nparams);
+    A(&net->v, net->nparams);
+    A(&net->wf, net->nparams);
+    A(&net->wb, net->nparams);
+    A(&net->x0, V0 * 16);
+    A(&net->a1, V0 * 16);
+    A(&net->e1, V0 * 16);
+    A(&net->p1, V1 * 16);
+    A(&net->a3, V1 * 32);
+    A(&net->e2, V1 * 32);
+    A(&net->p2, V2 * 32);
+    A(&net->a5, V2 * 64);
+    A(&net->e3, V2 * 64);
+    A(&net->u2, V1 * 64);
+    A(&net->a7, V1 * 32);
+    A(&net->s2, V1 * 32);
+    A(&net->d2, V1 * 32);
+    A(&net->u1, V0 * 32);
+    A(&net->a9, V0 * 16);
+    A(&net->s1, V0 * 16);
+    A(&net->d1, V0 * 16);
+    A(&net->out, V0);
+    A(&net->scales, 3);
+    A(&net->part, 3 * 1024);
+    A(&net->loss, 1);
+    if (rc == DARE_OK && hipHostMalloc(reinterpret_cast<void**>(&net->h_pinned), 4 * sizeof(double), 0) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("pinned allocation failed");
+        rc = DARE_ERR_OOM;
+    }
+    if (rc == DARE_OK) {
+        // zero parameters and Adam moments (kernels, so that nothing here is a memset node)
+        std::vector<float> z(net->nparams, 0.f);
+        if (hipMemcpy(net->w, z.data(), sizeof(float) * net->nparams, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(net->m, z.data(), sizeof(float) * net->nparams, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(net->v, z.data(), sizeof(float) * net->nparams, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(net->g, z.data(), sizeof(float) * net->nparams, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("convnet initialisation failed");
+            rc = DARE_ERR_HIP;
+        }
+    }
+    if (rc != DARE_OK) {
+        dare_convnet_destroy(net);
+        return rc;
+    }
+    *out = net;
+    return DARE_OK;
+}
+
+int dare_convnet_destroy(dare_convnet* net) {
+    if (!net) return DARE_OK;
+    (void)hipSetDevice(net->device);
+    (void)hipDeviceSynchronize();
+    void* ptrs[] = {net->w, net->g, net->m, net->v, net->wf, net->wb, net->x0, net->a1, net->e1, net->p1, net->a3,
+                    net->e2, net->p2, net->a5, net->e3, net->u2, net->a7, net->s2, net->d2, net->u1, net->a9,
+                    net->s1, net->d1, net->out, net->dout, net->t0a, net->t0b, net->t0c, net->t0w, net->h1a,
+                    net->h1b, net->h1c, net->h1w, net->q2a, net->q2b, net->wpart, net->scales, net->part, net->loss};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (net->h_pinned) (void)hipHostFree(net->h_pinned);
+    delete net;
+    return DARE_OK;
+}
+
+int dare_convnet_num_params(const dare_convnet* net, int64_t* n, int64_t* layer_offsets) {
+    DARE_CHECK_ARG(net && n, "NULL argument");
+    *n = net->nparams;
+    if (layer_offsets)
+        for (int l = 0; l <= kLayers; l++) layer_offsets[l] = net->off[l];
+    return DARE_OK;
+}
+
+// host FP32 master weights in, both BF16 copies refreshed; Adam state reset
+int dare_convnet_set_params(dare_ctx* ctx, dare_convnet* net, const float* host) {
+    DARE_CHECK_ARG(ctx && net && host, "NULL argument");
+    DARE_HIP_TRY(hipMemcpyAsync(net->w, host, sizeof(float) * net->nparams, hipMemcpyHostToDevice, ctx->stream));
+    DARE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<float> z(net->nparams, 0.f);
+    DARE_HIP_TRY(hipMemcpy(net->m, z.data(), sizeof(float) * net->nparams, hipMemcpyHostToDevice));
+    DARE_HIP_TRY(hipMemcpy(net->v, z.data(), sizeof(float) * net->nparams, hipMemcpyHostToDevice));
+    net->adam_t = 0;
+    for (int l = 0; l < kLayers; l++) {
+        const int n = 27 * kDefs[l].cin * kDefs[l].cout;
+        hipLaunchKernelGGL(k_adam_cast, dim3((n + kNetThreads - 1) / kNetThreads), dim3(kNetThreads), 0, ctx->stream,
+                           net->w + net->off[l], net->g + net->off[l], net->m + net->off[l], net->v + net->off[l],
+                           net->wf + net->off[l], net->wb + net->off[l], kDefs[l].cin, kDefs[l].cout, 0, 0.f, 0.f, 0.f,
+                           0.f, 1.f, 1.f);
+    }
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_w, float* host_grad) {
+    DARE_CHECK_ARG(ctx && net, "NULL argument");
+    if (host_w)
+        DARE_HIP_TRY(hipMemcpyAsync(host_w, net->w, sizeof(float) * net->nparams, hipMemcpyDeviceToHost, ctx->stream));
+    if (host_grad)
+        DARE_HIP_TRY(hipMemcpyAsync(host_grad, net->g, sizeof(float) * net->nparams, hipMemcpyDeviceToHost, ctx->stream));
+    DARE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DARE_OK;
+}
+
+// Input tensor from the solver's fields. phi, phi_prev: padded grid fields; rhs: the assembled
+// right-hand side (row order); rhs_scale maps it to field units (dt / V for the implicit
+// Euler term); extras: up to in_channels - 3 static padded fields, (f - offset) * scale.
+int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
+                      const double* rhs, double rhs_scale, int32_t nextra, const double* const* extras,
+                      const double* extra_scale, const double* extra_offset) {
+    DARE_CHECK_ARG(ctx && net && phi && phi_prev && rhs, "NULL argument");
+    int rc = check_grid(net, g);
+    if (rc) return rc;
+    DARE_CHECK_ARG(nextra >= 0 && nextra <= net->in_channels - 3 && nextra <= 8, "too many extra channels");
+    for (int d = 0; d < 3; d++) net->dims[d] = g->n[d];
+    const FieldGeo F = field_geo(g);
+    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    Extras ex{};
+    ex.n = nextra;
+    for (int e = 0; e < nextra; e++) {
+        DARE_CHECK_ARG(extras && extras[e] && extra_scale && extra_offset, "NULL extra field");
+        ex.f[e] = extras[e];
+        ex.scale[e] = extra_scale[e];
+        ex.offset[e] = extra_offset[e];
+    }
+    const unsigned gs = std::min<unsigned>(ew_grid(n), 1024);
+    hipLaunchKernelGGL(k_scales, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, rhs, rhs_scale,
+                       net->part);
+    hipLaunchKernelGGL(k_scales_finish, dim3(1), dim3(64), 0, ctx->stream, net->part, static_cast<int>(gs), net->scales);
+    hipLaunchKernelGGL(k_pack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, rhs, rhs_scale,
+                       ex, net->scales, net->x0);
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
+    DARE_CHECK_ARG(ctx && net, "NULL argument");
+    hipStream_t sm = ctx->stream;
+    const int* d = net->dims;
+    const long long V1 = level_voxels(d, 1);
+    const long long V0 = level_voxels(d, 0);
+    const uint16_t* W = net->wf;
+    const long long* o = net->off;
+    conv<16, 16, true, 0>(sm, net->x0, W + o[0], net->a1, nullptr, d, 0);
+    conv<16, 16, true, 0>(sm, net->a1, W + o[1], net->e1, nullptr, d, 0);
+    hipLaunchKernelGGL(k_pool, dim3(ew_grid(V1 * 16)), dim3(kNetThreads), 0, sm, net->e1, net->p1, 16, d[0] >> 1,
+                       d[1] >> 1, d[2] >> 1);
+    conv<16, 32, true, 0>(sm, net->p1, W + o[2], net->a3, nullptr, d, 1);
+    conv<32, 32, true, 0>(sm, net->a3, W + o[3], net->e2, nullptr, d, 1);
+    hipLaunchKernelGGL(k_pool, dim3(ew_grid(level_voxels(d, 2) * 32)), dim3(kNetThreads), 0, sm, net->e2, net->p2, 32,
+                       d[0] >> 2, d[1] >> 2, d[2] >> 2);
+    conv<32, 64, true, 0>(sm, net->p2, W + o[4], net->a5, nullptr, d, 2);
+    conv<64, 64, true, 0>(sm, net->a5, W + o[5], net->e3, nullptr, d, 2);
+    hipLaunchKernelGGL(k_up, dim3(ew_grid(V1 * 64)), dim3(kNetThreads), 0, sm, net->e3, net->u2, 64, d[0] >> 1,
+                       d[1] >> 1, d[2] >> 1);
+    conv<64, 32, true, 0>(sm, net->u2, W + o[6], net->a7, nullptr, d, 1);
+    hipLaunchKernelGGL(k_add, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->a7, net->e2, net->s2, V1 * 32);
+    conv<32, 32, true, 0>(sm, net->s2, W + o[7], net->d2, nullptr, d, 1);
+    hipLaunchKernelGGL(k_up, dim3(ew_grid(V0 * 32)), dim3(kNetThreads), 0, sm, net->d2, net->u1, 32, d[0], d[1], d[2]);
+    conv<32, 16, true, 0>(sm, net->u1, W + o[8], net->a9, nullptr, d, 0);
+    hipLaunchKernelGGL(k_add, dim3(ew_grid(V0 * 16)), dim3(kNetThreads), 0, sm, net->a9, net->e1, net->s1, V0 * 16);
+    conv<16, 16, true, 0>(sm, net->s1, W + o[9], net->d1, nullptr, d, 0);
+    conv<16, 16, false, 1>(sm, net->d1, W + o[10], nullptr, net->out, d, 0);
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+// x_guess[row] = phi[row's cell] + S * out[row]
+int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess) {
+    DARE_CHECK_ARG(ctx && net && phi && x_guess, "NULL argument");
+    int rc = check_grid(net, g);
+    if (rc) return rc;
+    const FieldGeo F = field_geo(g);
+    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    hipLaunchKernelGGL(k_unpack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, net->out, net->scales,
+                       x_guess);
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+// MSE against the solver's solution x_true (row order) and the output gradient; loss to host.
+int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* x_true,
+                      double* loss_host) {
+    DARE_CHECK_ARG(ctx && net && phi && x_true, "NULL argument");
+    int rc = check_grid(net, g);
+    if (rc) return rc;
+    if (!net->dout) {
+        rc = dalloc(&net->dout, net->max_v0 * 16);
+        if (rc) return rc;
+    }
+    const FieldGeo F = field_geo(g);
+    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    const unsigned gs = std::min<unsigned>(ew_grid(n), 1024);
+    hipLaunchKernelGGL(k_mse, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, x_true, net->out, net->scales,
+                       net->dout, net->part);
+    hipLaunchKernelGGL(k_mse_finish, dim3(1), dim3(64), 0, ctx->stream, net->part, static_cast<int>(gs), n, net->loss);
+    DARE_LAUNCH_CHECK();
+    if (loss_host) {
+        DARE_HIP_TRY(hipMemcpyAsync(net->h_pinned, net->loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        DARE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        *loss_host = net->h_pinned[0];
+    }
+    return DARE_OK;
+}
+
+// Back-propagation of the last loss through the last forward pass: weight gradients (FP32).
+int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net) {
+    DARE_CHECK_ARG(ctx && net && net->dout, "call dare_convnet_loss first");
+    if (!net->bwd_ready) {
+        const long long V0 = net->max_v0, V1 = V0 / 8, V2 = V0 / 64;
+        int rc = DARE_OK;
+        auto A = [&](auto** p, long long n) {
+            if (rc == DARE_OK) rc = dalloc(p, n);
+        };
+        A(&net->t0a, V0 * 16);
+        A(&net->t0b, V0 * 16);
+        A(&net->t0c, V0 * 16);
+        A(&net->t0w, V0 * 32);
+        A(&net->h1a, V1 * 32);
+        A(&net->h1b, V1 * 32);
+        A(&net->h1c, V1 * 32);
+        A(&net->h1w, V1 * 64);
+        A(&net->q2a, V2 * 64);
+        A(&net->q2b, V2 * 64);
+        A(&net->wpart, kWpartFloats);
+        if (rc) return rc;
+        net->bwd_ready = true;
+    }
+    hipStream_t sm = ctx->stream;
+    const int* d = net->dims;
+    const long long V0 = level_voxels(d, 0), V1 = level_voxels(d, 1), V2 = level_voxels(d, 2);
+    const uint16_t* WB = net->wb;
+    const long long* o = net->off;
+    // L11 (no ReLU): dd1 = conv_bwd(dout)
+    wgrad_any(net, sm, 10, net->d1, net->dout);
+    conv_any(sm, 16, 16, false, net->dout, WB + o[10], net->t0a, d, 0);            // dd1
+    hipLaunchKernelGGL(k_mask, dim3(ew_grid(V0 * 16)), dim3(kNetThreads), 0, sm, net->t0a, net->d1, net->t0b, V0 * 16);
+    // L10: g10 = t0b
+    wgrad_any(net, sm, 9, net->s1, net->t0b);
+    conv_any(sm, 16, 16, false, net->t0b, WB + o[9], net->t0c, d, 0);              // ds1 (kept)
+    hipLaunchKernelGGL(k_mask, dim3(ew_grid(V0 * 16)), dim3(kNetThreads), 0, sm, net->t0c, net->a9, net->t0a, V0 * 16);
+    // L9: g9 = t0a
+    wgrad_any(net, sm, 8, net->u1, net->t0a);
+    conv_any(sm, 16, 32, false, net->t0a, WB + o[8], net->t0w, d, 0);              // du1 (32 ch)
+    hipLaunchKernelGGL(k_up_bwd_mask, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->t0w, net->d2, net->h1a,
+                       32, d[0] >> 1, d[1] >> 1, d[2] >> 1);                       // g8
+    // L8
+    wgrad_any(net, sm, 7, net->s2, net->h1a);
+    conv_any(sm, 32, 32, false, net->h1a, WB + o[7], net->h1b, d, 1);              // ds2 (kept)
+    hipLaunchKernelGGL(k_mask, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->h1b, net->a7, net->h1c, V1 * 32);
+    // L7: g7 = h1c
+    wgrad_any(net, sm, 6, net->u2, net->h1c);
+    conv_any(sm, 32, 64, false, net->h1c, WB + o[6], net->h1w, d, 1);              // du2 (64 ch)
+    hipLaunchKernelGGL(k_up_bwd_mask, dim3(ew_grid(V2 * 64)), dim3(kNetThreads), 0, sm, net->h1w, net->e3, net->q2a,
+                       64, d[0] >> 2, d[1] >> 2, d[2] >> 2);                       // g6
+    // L6
+    wgrad_any(net, sm, 5, net->a5, net->q2a);
+    conv_any(sm, 64, 64, false, net->q2a, WB + o[5], net->q2b, d, 2);              // da5
+    hipLaunchKernelGGL(k_mask, dim3(ew_grid(V2 * 64)), dim3(kNetThreads), 0, sm, net->q2b, net->a5, net->q2a, V2 * 64);
+    // L5: g5 = q2a
+    wgrad_any(net, sm, 4, net->p2, net->q2a);
+    conv_any(sm, 64, 32, false, net->q2a, WB + o[4], net->q2b, d, 2);              // dp2 (32 ch, quarter)
+    hipLaunchKernelGGL(k_pool_bwd_add_mask, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->q2b, net->h1b,
+                       net->e2, net->h1a, 32, d[0] >> 1, d[1] >> 1, d[2] >> 1);     // g4
+    // L4
+    wgrad_any(net, sm, 3, net->a3, net->h1a);
+    conv_any(sm, 32, 32, false, net->h1a, WB + o[3], net->h1c, d, 1);              // da3
+    hipLaunchKernelGGL(k_mask, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->h1c, net->a3, net->h1a, V1 * 32);
+    // L3: g3 = h1a
+    wgrad_any(net, sm, 2, net->p1, net->h1a);
+    conv_any(sm, 32, 16, false, net->h1a, WB + o[2], net->h1c, d, 1);              // dp1 (16 ch, half)
+    hipLaunchKernelGGL(k_pool_bwd_add_mask, dim3(ew_grid(V0 * 16)), dim3(kNetThreads), 0, sm, net->h1c, net->t0c,
+                       net->e1, net->t0a, 16, d[0], d[1], d[2]);                   // g2
+    // L2
+    wgrad_any(net, sm, 1, net->a1, net->t0a);
+    conv_any(sm, 16, 16, false, net->t0a, WB + o[1], net->t0b, d, 0);              // da1
+    hipLaunchKernelGGL(k_mask, dim3(ew_grid(V0 * 16)), dim3(kNetThreads), 0, sm, net->t0b, net->a1, net->t0a, V0 * 16);
+    // L1: g1 = t0a
+    wgrad_any(net, sm, 0, net->x0, net->t0a);
+    DARE_LAUNCH_CHECK();
+    (void)V2;
+    return DARE_OK;
+}
+
+// One Adam step on every layer with the gradients of the last backward pass.
+int dare_conv_net_step(dare_ctx* ctx, dare_convnet* net, float lr, float beta1, float beta2, float eps) {
+    DARE_CHECK_ARG(ctx && net, "NULL argument");
+    DARE_CHECK_ARG(lr >= 0.f && beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f,
+                   "invalid Adam parameters");
+    net->adam_t++;
+    const float c1 = 1.f - std::pow(beta1, static_cast<float>(net->adam_t));
+    const float c2 = 1.f - std::pow(beta2, static_cast<float>(net->adam_t));
+    for (int l = 0; l < kLayers; l++) {
+        const int n = 27 * kDefs[l].cin * kDefs[l].cout;
+        hipLaunchKernelGGL(k_adam_cast, dim3((n + kNetThreads - 1) / kNetThreads), dim3(kNetThreads), 0, ctx->stream,
+                           net->w + net->off[l], net->g + net->off[l], net->m + net->off[l], net->v + net->off[l],
+                           net->wf + net->off[l], net->wb + net->off[l], kDefs[l].cin, kDefs[l].cout, 1, lr, beta1,
+                           beta2, eps, c1, c2);
+    }
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+// Device pointers of the current tensors (tests): x0, out, dout, and the forward activations
+// in network order (a1, e1, p1, a3, e2, p2, a5, e3, u2, a7, s2, d2, u1, a9, s1, d1).
+int dare_convnet_tensors(const dare_convnet* net, void** ptrs /* 19 */, double* scales_host) {
+    DARE_CHECK_ARG(net && ptrs, "NULL argument");
+    void* p[19] = {net->x0, net->out, net->dout, net->a1, net->e1, net->p1, net->a3, net->e2, net->p2, net->a5,
+                   net->e3, net->u2, net->a7, net->s2, net->d2, net->u1, net->a9, net->s1, net->d1};
+    for (int q = 0; q < 19; q++) ptrs[q] = p[q];
+    if (scales_host) DARE_HIP_TRY(hipMemcpy(scales_host, net->scales, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return DARE_OK;
+}
+
+}  // extern "C"

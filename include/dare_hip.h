@@ -217,6 +217,33 @@ int dare_batch_autotune(dare_ctx* ctx, dare_batch* batch, int32_t reps, dare_bat
 int dare_batch_solve(dare_ctx* ctx, dare_batch* batch, const dare_solver_params* p, int32_t mixed,
                      int32_t graph_passes, dare_solver_result* results);
 
+/* ---- learned warm start ------------------------------------------------------------------
+ * A 3-level 3-D convolutional encoder-decoder (16/32/64 channels, 3x3x3 kernels, BF16 weights
+ * and activations, FP32 accumulation on the matrix cores) that predicts the solution of the
+ * next implicit step from phi^n, phi^{n-1} and the right-hand side. No reference counterpart:
+ * the reference starts BiCGStab2 from phi^n (MatrixSystem/Trilinos.inl:316-317); the guess
+ * only replaces x0, the solver still iterates to its FP64 tolerance.
+ * Sequence: pack -> fwd -> unpack (inference); pack -> fwd -> loss -> bwd -> step (training).
+ * Grids: 3-D, one component, every size a multiple of 4 and at most max_dims. */
+typedef struct dare_convnet dare_convnet;
+int dare_convnet_create(dare_ctx* ctx, int32_t in_channels, const int32_t* max_dims, dare_convnet** out);
+int dare_convnet_destroy(dare_convnet* net);
+/* number of FP32 master parameters; layer_offsets[12] (may be NULL). Layer l holds
+ * [27][cout][cin] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1), dx along the fastest grid index). */
+int dare_convnet_num_params(const dare_convnet* net, int64_t* n, int64_t* layer_offsets);
+int dare_convnet_set_params(dare_ctx* ctx, dare_convnet* net, const float* host);
+int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_w, float* host_grad);
+int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
+                      const double* rhs, double rhs_scale, int32_t nextra, const double* const* extras,
+                      const double* extra_scale, const double* extra_offset);
+int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net);
+int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess);
+int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* x_true,
+                      double* loss_host);
+int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net);
+int dare_conv_net_step(dare_ctx* ctx, dare_convnet* net, float lr, float beta1, float beta2, float eps);
+int dare_convnet_tensors(const dare_convnet* net, void** ptrs, double* scales_host);
+
 #ifdef __cplusplus
 }
 #endif

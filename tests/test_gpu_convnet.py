@@ -1,0 +1,144 @@
+"""Warm-start network kernels (csrc/convnet.hip) against the CPU oracle (oracle/convnet.py).
+
+Packing of the solver fields is bit-exact (same double arithmetic, same BF16 rounding).
+Forward activations and weight gradients agree with the oracle run with BF16 rounding at the
+same storage points; the remaining difference is the FP32 summation order of the matrix-core
+accumulations (and BF16 rounding flips it causes), bounded here in relative L2 norm.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import dare_amd as da
+from dare_amd.learned import WarmStartNet, he_init
+from oracle import convnet as cn
+
+pytestmark = pytest.mark.gpu
+
+
+def d2h_bf16(ctx, addr, shape):
+    n = int(np.prod(shape))
+    u = np.empty(n, np.uint16)
+    da.check(da.lib().dare_memcpy_d2h(ctx.h, u.ctypes.data, addr, 2 * n), "d2h")
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64).reshape(shape)
+
+
+def d2h_f32(ctx, addr, shape):
+    n = int(np.prod(shape))
+    a = np.empty(n, np.float32)
+    da.check(da.lib().dare_memcpy_d2h(ctx.h, a.ctypes.data, addr, 4 * n), "d2h")
+    return a.astype(np.float64).reshape(shape)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def fields(rng, n, ghost=2):
+    import torch
+    pad = [m + 2 * ghost for m in n]
+    phi = rng.normal(size=pad)
+    phim = phi + 0.1 * rng.normal(size=pad)
+    rhs = rng.normal(size=n)
+    ext = rng.uniform(size=pad)
+    to = lambda a: torch.from_numpy(np.ascontiguousarray(a).ravel()).cuda()
+    sl = tuple(slice(ghost, ghost + m) for m in n)
+    return (to(phi), to(phim), to(rhs), to(ext)), (phi[sl], phim[sl], rhs, ext[sl])
+
+
+N = [8, 12, 16]
+
+
+def setup(ctx, rng, in_ch=4):
+    net = WarmStartNet(ctx, N, in_channels=in_ch, seed=3)
+    w = he_init(in_ch, seed=3, out_scale=1.0)
+    net.set_params(w)
+    g = da.Grid.make(N, size=[1, 1, 1], ghost=2)
+    dev, hostf = fields(rng, N)
+    extras = [(dev[3], 2.0, 0.5)]
+    net.pack(g, dev[0], dev[1], dev[2], 0.7, extras)
+    return net, g, dev, hostf, w, extras
+
+
+def test_pack_bit_exact(ctx, rng):
+    net, g, dev, (phi, phim, rhs, ext), w, _ = setup(ctx, rng)
+    ptr, scales = net.tensors()
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    exp, s = cn.pack(phi, phim, rhs.reshape(N), 0.7, extras=[(ext, 2.0, 0.5)])
+    assert np.allclose(scales, s, rtol=0, atol=0)
+    assert np.array_equal(x0, exp)
+    net.close()
+
+
+def test_forward_matches_oracle(ctx, rng):
+    net, g, dev, hostf, w, _ = setup(ctx, rng)
+    net.forward()
+    ctx.synchronize()
+    ptr, _ = net.tensors()
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    out_ref, A = cn.forward(x0, cn.split(cn.bf16(w)), exact=False)
+    shapes = {"a1": (0, 16), "e1": (0, 16), "p1": (1, 16), "a3": (1, 32), "e2": (1, 32), "p2": (2, 32),
+              "a5": (2, 64), "e3": (2, 64), "a7": (1, 32), "s2": (1, 32), "d2": (1, 32), "a9": (0, 16),
+              "s1": (0, 16), "d1": (0, 16)}
+    for name, (lv, c) in shapes.items():
+        got = d2h_bf16(ctx, ptr[name], [m >> lv for m in N] + [c])
+        assert rel(got, A[name]) <= 1e-2, (name, rel(got, A[name]))
+    out = d2h_f32(ctx, ptr["out"], N)
+    assert rel(out, out_ref) <= 1e-2, rel(out, out_ref)
+    net.close()
+
+
+def test_backward_matches_oracle(ctx, rng):
+    import torch
+    net, g, dev, (phi, phim, rhs, ext), w, _ = setup(ctx, rng)
+    net.forward()
+    x_true = torch.from_numpy(rng.normal(size=int(np.prod(N)))).cuda()
+    loss = net.loss(g, dev[0], x_true)
+    net.backward()
+    grads = net.grads()
+    ptr, scales = net.tensors()
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    Wb = cn.split(cn.bf16(w))
+    out_ref, A = cn.forward(x0, Wb, exact=False)
+    out_gpu = d2h_f32(ctx, ptr["out"], N)
+    loss_ref, dout = cn.loss_and_dout(out_gpu, phi, x_true.cpu().numpy().reshape(N), scales[1])
+    assert abs(loss - loss_ref) <= 1e-9 * abs(loss_ref)
+    dW = cn.backward(dout, A, Wb, exact=False)
+    G = cn.split(grads.astype(np.float64))
+    for l in range(11):
+        e = rel(G[l], dW[l])
+        assert e <= 3e-2, (l, e)
+    net.close()
+
+
+def test_training_reduces_loss(ctx, rng):
+    """Adam on one fixed sample: the loss falls by an order of magnitude."""
+    import torch
+    net = WarmStartNet(ctx, N, in_channels=3, seed=5)
+    g = da.Grid.make(N, size=[1, 1, 1], ghost=2)
+    dev, (phi, phim, rhs, ext) = fields(rng, N)
+    # a target that is a smooth function of the inputs
+    x_true = dev[0].view(*[m + 4 for m in N])[2:-2, 2:-2, 2:-2].contiguous().view(-1) * 1.05
+    losses = [net.train_step(g, dev[0], dev[1], dev[2], 0.7, x_true, lr=3e-3) for _ in range(60)]
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+    # the guess is phi^n + S * out
+    xg = torch.empty_like(x_true)
+    net.predict(g, dev[0], dev[1], dev[2], 0.7, xg)
+    ptr, s = net.tensors()
+    out = d2h_f32(ctx, ptr["out"], N)
+    assert np.allclose(xg.cpu().numpy().reshape(N), phi + s[1] * out, rtol=1e-14, atol=1e-14)
+    net.close()
+
+
+def test_save_load_roundtrip(ctx, tmp_path):
+    net = WarmStartNet(ctx, N, in_channels=3, seed=9)
+    p = str(tmp_path / "w.npz")
+    net.save(p, meta={"note": "test"})
+    w, meta = WarmStartNet.load_params(p)
+    assert np.array_equal(w, net.params()) and meta["in_channels"] == 3
+    net2 = WarmStartNet(ctx, N, in_channels=3, params=w)
+    assert np.array_equal(net2.params(), w)
+    net.close()
+    net2.close()

@@ -1,0 +1,82 @@
+"""The warm-start network's CPU oracle, pinned by the mathematics: its analytic weight
+gradients agree with central finite differences of its loss (float64, no rounding), and its
+building blocks satisfy their defining identities (the data gradient of a convolution is the
+adjoint of the convolution; pooling and upsampling are adjoint up to the factor 8)."""
+import numpy as np
+
+from oracle import convnet as cn
+
+
+def _weights(rng, in_ch=3):
+    ws = []
+    for l, (cin, cout, _) in enumerate(cn.LAYERS):
+        fan = 27 * (in_ch if l == 0 else cin)
+        w = rng.normal(0, np.sqrt(2.0 / fan), size=(27, cout, cin))
+        if l == 0:
+            w[:, :, in_ch:] = 0
+        if l == 10:
+            w[:, 1:, :] = 0
+        ws.append(w)
+    return ws
+
+
+def test_conv_adjoint_and_resampling_identities():
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(4, 6, 8, 16))
+    g = rng.normal(size=(4, 6, 8, 32))
+    w = rng.normal(size=(27, 32, 16))
+    # <conv(x), g> = <x, conv_t(g)>
+    assert np.isclose(np.sum(cn.conv(x, w) * g), np.sum(x * cn.conv_t(g, w)), rtol=1e-12)
+    # <conv(x), g> = <w, wgrad(x, g)>
+    assert np.isclose(np.sum(cn.conv(x, w) * g), np.sum(w * cn.wgrad(x, g)), rtol=1e-12)
+    # centre tap only = a per-voxel matrix product
+    wc = np.zeros_like(w)
+    wc[13] = w[13]
+    assert np.allclose(cn.conv(x, wc), x @ w[13].T)
+    # pooling and upsampling: <pool(a), b> = <a, up(b)> / 8, up_bwd = 8 * adjoint of pool
+    a = rng.normal(size=(4, 6, 8, 5))
+    b = rng.normal(size=(2, 3, 4, 5))
+    assert np.isclose(np.sum(cn.pool(a) * b), np.sum(a * cn.up(b)) / 8)
+    assert np.isclose(np.sum(cn.up(b) * a), np.sum(b * cn.up_bwd(a)))
+
+
+def test_oracle_gradients_match_finite_differences():
+    rng = np.random.default_rng(2)
+    W = _weights(rng)
+    x0 = np.zeros((4, 4, 8, 16))
+    x0[..., :3] = rng.normal(size=(4, 4, 8, 3))
+    t = rng.normal(size=(4, 4, 8))
+
+    def loss(ws):
+        out, _ = cn.forward(x0, ws, exact=True)
+        return np.mean((out - t) ** 2)
+
+    out, A = cn.forward(x0, W, exact=True)
+    dout = 2 * (out - t) / out.size
+    dW = cn.backward(dout, A, W, exact=True)
+    checked = 0
+    for l in range(11):
+        cin, cout, _ = cn.LAYERS[l]
+        for _ in range(3):
+            tap = int(rng.integers(27))
+            co = 0 if l == 10 else int(rng.integers(cout))
+            ci = int(rng.integers(3 if l == 0 else cin))
+            h = 1e-6
+            Wp = [w.copy() for w in W]
+            Wm = [w.copy() for w in W]
+            Wp[l][tap, co, ci] += h
+            Wm[l][tap, co, ci] -= h
+            fd = (loss(Wp) - loss(Wm)) / (2 * h)
+            an = dW[l][tap, co, ci]
+            # FD noise floor: eps * loss / h ~ 1e-10
+            assert abs(fd - an) <= 1e-5 * abs(an) + 1e-9, (l, tap, co, ci, fd, an)
+            checked += 1
+    assert checked == 33
+
+
+def test_bf16_rounding_is_round_to_nearest_even():
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 0.0])
+    got = cn.bf16(vals)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: even mantissa -> 1; 1 + 3*2^-8 -> 1 + 2^-6
+    assert got[0] == 1.0 and got[1] == 1.0 and got[2] == 1.0 + 2 ** -6 and got[3] == 1.0
+    assert got[4] == -2.5 and got[5] == 0.0
