@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -35,6 +37,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the target / batch / learned legs")
+    ap.add_argument("--target-steps", type=int, default=4)
+    ap.add_argument("--batch-systems", type=int, default=512)
     return ap.parse_args()
 
 
@@ -166,6 +171,16 @@ def main():
     cpu = None
     if not args.no_cpu and not args.profile:
         cpu = cpu_baseline(args, W, ctx)
+    extra = {}
+    if not args.no_extra and not args.profile:
+        A.close()
+        del W
+        torch.cuda.empty_cache()
+        extra["target"] = target_leg(ctx, args.n, args.target_steps)
+        torch.cuda.empty_cache()
+        extra["batch"] = batch_leg(ctx, args.batch_systems)
+        torch.cuda.empty_cache()
+        extra["learned"] = learned_leg(ctx)
 
     line = {
         "metric": "256^3 solve time to 1e-8 resid: mixed/learned vs plain FP64 GPU; HBM GB/s",
@@ -203,10 +218,169 @@ def main():
             "avg_launch_us": round(tvd_s * 1e6, 2), "cell_updates_per_s": round(ncell / tvd_s, 1),
         },
         "cpu_baseline": cpu,
+        **extra,
     }
     print(json.dumps(line), flush=True)
     A.close()
     ctx.close()
+
+
+WEIGHTS = os.path.join(ROOT, "dare_amd", "weights")
+
+
+def _true_rel(ctx, A, b, x):
+    import torch
+    import dare_amd as da
+    y = torch.empty_like(x)
+    da.spmv(ctx, A, x, y)
+    return float(torch.linalg.norm(b - y) / torch.linalg.norm(b))
+
+
+def target_leg(ctx, n, steps):
+    """256^3 variable-diffusivity implicit steps (ln D ~ N(ln 1e-3, 1), harmonic face values,
+    source U[0,1), Dirichlet 0, mean D dt/dx^2 = 1) on a held-out seed: plain FP64 BiCGStab2
+    from phi^n (the reference's guess) against mixed precision and the learned guess, every
+    form to FP64 true relative residual 1e-8. Step 0 (no phi^{n-1}) is a warm-up."""
+    import torch
+    import dare_amd as da
+    from dare_amd.learned import WarmStartNet
+    from dare_amd.workloads import VarDiffusion
+    log(f"[bench] target: {n}^3 variable-diffusivity steps")
+    W = VarDiffusion(ctx, seed=5000, n=n)
+    net = None
+    wpath = os.path.join(WEIGHTS, "ws_vardiff_128.npz")
+    if os.path.exists(wpath):
+        w, meta = WarmStartNet.load_params(wpath)
+        net = WarmStartNet(ctx, [n] * 3, in_channels=meta["in_channels"], params=w)
+    forms = ["fp64", "mixed"] + (["fp64+learned", "mixed+learned"] if net else [])
+    st = {f: {"ms": [], "passes": [], "refine": [], "rel": [], "guess_ms": []} for f in forms}
+    xg = torch.empty_like(W.x)
+    for k in range(steps + 1):
+        W.assemble()
+        ctx.synchronize()
+        x_prev = W.x.clone()
+        keep = None
+        for f in forms:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if f.endswith("learned"):
+                net.predict(W.grid, W.phi, W.phi_prev, W.rhs, W.rhs_scale, xg, extras=W.extras)
+            else:
+                xg.copy_(x_prev)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if f.startswith("fp64"):
+                r = da.bicgstab2(ctx, W.A, W.rhs, xg, tol=TOL, tol_kind=1, max_it=2000)
+            else:
+                r = da.refine_mixed(ctx, W.A, W.rhs, xg, tol=TOL, tol_kind=1, inner_tol=1e-4, max_refine=30,
+                                    inner_max_it=2000, max_it=4000)
+            ctx.synchronize()
+            t2 = time.perf_counter()
+            if k >= 1:
+                st[f]["ms"].append(1e3 * (t2 - t0))
+                st[f]["guess_ms"].append(1e3 * (t1 - t0))
+                st[f]["passes"].append(r["iterations"])
+                st[f]["refine"].append(r["refinements"])
+                st[f]["rel"].append(_true_rel(ctx, W.A, W.rhs, xg))
+            if f == "fp64":
+                keep = xg.clone()
+        W.advance(keep)
+    out = {"workload": f"{n}^3 variable-diffusivity implicit step (held-out seed 5000), {steps} steps after 1 warm-up",
+           "dt": W.dt}
+    for f in forms:
+        out[f] = {"ms_mean": round(float(np.mean(st[f]["ms"])), 3),
+                  "guess_ms_mean": round(float(np.mean(st[f]["guess_ms"])), 3),
+                  "passes_mean": float(np.mean(st[f]["passes"])), "refinements_mean": float(np.mean(st[f]["refine"])),
+                  "true_rel_residual_max": float(np.max(st[f]["rel"]))}
+    base = out["fp64"]["ms_mean"]
+    for f in forms:
+        out[f]["speedup_vs_fp64"] = round(base / out[f]["ms_mean"], 3)
+    if net:
+        net.close()
+    W.A.close()
+    return out
+
+
+def batch_leg(ctx, nsys):
+    """512 independent 2-component reaction-diffusion systems (edges log-uniform in [16, 96]):
+    one batched solve (persistent launches, per-system masks, graph-captured passes, autotuned
+    configuration) against the same systems solved one after the other by the plain solver."""
+    import torch
+    import dare_amd as da
+    from dare_amd import autotune
+    from dare_amd.workloads import C4Batch
+    log(f"[bench] batch: {nsys} systems")
+    t0 = time.perf_counter()
+    Wb = C4Batch(ctx, nsys=nsys)
+    Wb.assemble()
+    B = Wb.batch
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cfg, us = autotune.tune(B, Wb.dims.tolist(), 2, torch.cuda.get_device_name(0), reps=3)
+    tune_s = time.perf_counter() - t0
+    x0 = torch.empty(B.rows, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(x0)
+    da.check(da.lib().dare_memcpy_d2d(ctx.h, x0.data_ptr(), B.x_addr, 8 * B.rows))
+    da.check(da.lib().dare_memcpy_d2d(ctx.h, b.data_ptr(), B.b_addr, 8 * B.rows))
+    ctx.synchronize()
+    res_t = {}
+    results = {}
+    for mixed in (False, True):
+        for rep in range(2):          # first call captures the graphs
+            da.check(da.lib().dare_memcpy_d2d(ctx.h, B.x_addr, x0.data_ptr(), 8 * B.rows))
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            res = B.solve(tol=TOL, tol_kind=1, max_it=1000, mixed=mixed, graph_passes=4)
+            ctx.synchronize()
+            el = time.perf_counter() - t0
+        key = "batched_mixed" if mixed else "batched_fp64"
+        res_t[key] = el
+        results[key] = res
+    xb = torch.empty_like(x0)
+    da.check(da.lib().dare_memcpy_d2d(ctx.h, xb.data_ptr(), B.x_addr, 8 * B.rows))
+    xs = x0.clone()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    seq = []
+    for s in range(nsys):
+        o0, o1 = int(B.offsets[s]), int(B.offsets[s + 1])
+        seq.append(da.bicgstab2(ctx, Wb.csr[s], b[o0:o1], xs[o0:o1], tol=TOL, tol_kind=1, max_it=1000))
+    ctx.synchronize()
+    seq_s = time.perf_counter() - t0
+    worst = 0.0
+    for s in range(nsys):
+        o0, o1 = int(B.offsets[s]), int(B.offsets[s + 1])
+        worst = max(worst, _true_rel(ctx, Wb.csr[s], b[o0:o1], xb[o0:o1]))
+    out = {"workload": f"{nsys} reaction-diffusion systems, N=2, edges log-uniform [16,96], "
+                       f"{Wb.cells} cells, {B.rows} rows, tol 1e-8 per system",
+           "sequential_plain_ms": round(1e3 * seq_s, 2),
+           "batched_fp64_ms": round(1e3 * res_t["batched_fp64"], 2),
+           "batched_mixed_ms": round(1e3 * res_t["batched_mixed"], 2),
+           "speedup_mixed_vs_sequential": round(seq_s / res_t["batched_mixed"], 2),
+           "speedup_fp64_vs_sequential": round(seq_s / res_t["batched_fp64"], 2),
+           "converged": {k: int(sum(r["converged"] for r in v)) for k, v in results.items()},
+           "sequential_passes_mean": float(np.mean([r["iterations"] for r in seq])),
+           "batched_mixed_worst_true_rel_residual": worst,
+           "autotuned_config": cfg, "autotune_s": round(tune_s, 2), "build_s": round(build_s, 2)}
+    Wb.close()
+    return out
+
+
+def learned_leg(ctx, seeds=2, steps=30):
+    """Held-out advection-diffusion trajectories at 128^3: passes and time per step from the
+    zero, previous-step, extrapolated and learned initial guesses (a short sample of the full
+    evaluation the training report holds)."""
+    from dare_amd.learned import WarmStartNet
+    from dare_amd.warmstart import evaluate
+    wpath = os.path.join(WEIGHTS, "ws_advdiff_128.npz")
+    if not os.path.exists(wpath):
+        return None
+    log("[bench] learned: held-out 128^3 advection-diffusion steps")
+    w, meta = WarmStartNet.load_params(wpath)
+    net = WarmStartNet(ctx, [128] * 3, in_channels=meta["in_channels"], params=w)
+    ev = evaluate(ctx, net, n=128, eval_seeds=range(5000, 5000 + seeds), steps=steps, workload="advdiff")
+    net.close()
+    return {"workload": f"128^3 advection-diffusion, {seeds} held-out seeds x {steps} steps", **ev}
 
 
 def cpu_baseline(args, W, ctx):

@@ -1,0 +1,187 @@
+"""Training and evaluation of the learned warm start on solver trajectories.
+
+    python -m dare_amd.warmstart train --n 128 --train-seeds 200 --steps 200 --out ws.npz
+    python -m dare_amd.warmstart eval  --n 128 --weights ws.npz --eval-seeds 20
+
+train: for every training seed, the plain FP64 BiCGStab2 advances the advection-diffusion
+trajectory (C5AdvDiff) from the reference's guess (phi^n); each step's (phi^n, phi^{n-1},
+rhs) -> solution pair is one online Adam step of the network (hand-written BF16 MFMA forward
+and backward kernels). Nothing is stored between steps.
+
+eval: on held-out seeds, every step is solved from four initial guesses — zero, phi^n (the
+reference's choice), the linear extrapolation 2 phi^n - phi^{n-1}, and the network's
+prediction — to the same FP64 tolerance; passes and wall time (inference included for the
+network) are reported per guess.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import Context, bicgstab2
+from .learned import WarmStartNet
+from .workloads import WORKLOADS
+
+TOL = 1e-8
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_final=1e-4, net=None,
+          train_every=1, log_every=60.0, max_seconds=None, seed=0, workload="advdiff"):
+    """Online training on plain-solver trajectories. Returns (net, info dict)."""
+    import torch
+    make = WORKLOADS[workload]
+    net = net or WarmStartNet(ctx, [n, n, n], in_channels=make.in_channels_for(), seed=seed)
+    seeds = list(train_seeds)
+    total = len(seeds) * max(steps - 1, 1) // train_every
+    done, t_start, t_log = 0, time.perf_counter(), time.perf_counter()
+    A = None
+    losses, passes = [], []
+    for si, s in enumerate(seeds):
+        W = make(ctx, s, n=n, A=A)
+        A = W.A
+        for k in range(steps):
+            W.assemble()
+            r = bicgstab2(ctx, A, W.rhs, W.x, tol=TOL, tol_kind=1, max_it=1000)
+            passes.append(r["iterations"])
+            if k >= 1 and k % train_every == 0:
+                frac = min(done / max(total, 1), 1.0)
+                cur_lr = lr * (lr_final / lr) ** frac
+                losses.append(net.train_step(W.grid, W.phi, W.phi_prev, W.rhs, W.rhs_scale, W.x, lr=cur_lr,
+                                             extras=W.extras))
+                done += 1
+            W.advance(W.x)
+            now = time.perf_counter()
+            if now - t_log > log_every:
+                t_log = now
+                _log(f"[train] seed {si + 1}/{len(seeds)} step {k} samples {done} "
+                     f"loss(last 50) {np.mean(losses[-50:]):.4e} elapsed {now - t_start:.0f}s")
+            if max_seconds and now - t_start > max_seconds:
+                break
+        if max_seconds and time.perf_counter() - t_start > max_seconds:
+            break
+    torch.cuda.synchronize()
+    info = {"workload": workload, "samples": done, "seeds_used": si + 1, "steps": steps, "n": n,
+            "seconds": round(time.perf_counter() - t_start, 1),
+            "loss_first_100": float(np.mean(losses[:100])) if losses else None,
+            "loss_last_100": float(np.mean(losses[-100:])) if losses else None,
+            "plain_passes_mean": float(np.mean(passes)) if passes else None}
+    return net, info
+
+
+def evaluate(ctx, net, n=128, eval_seeds=range(5000, 5020), steps=200, tol=TOL, log_every=60.0,
+             workload="advdiff"):
+    """Passes and time per step for each initial guess on held-out trajectories."""
+    import torch
+    make = WORKLOADS[workload]
+    modes = ["zero", "previous", "extrapolated", "learned"]
+    stats = {m: {"passes": [], "solve_s": [], "guess_s": []} for m in modes}
+    A = None
+    xg = None
+    t_log = time.perf_counter()
+    for si, s in enumerate(eval_seeds):
+        W = make(ctx, s, n=n, A=A)
+        A = W.A
+        if xg is None:
+            xg = torch.empty_like(W.x)
+        for k in range(steps):
+            W.assemble()
+            x_prev = W.x.clone()                    # phi^n in solver order (fill wrote x0 = phi^n)
+            keep = None
+            for m in modes:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if m == "zero":
+                    xg.zero_()
+                elif m == "previous":
+                    xg.copy_(x_prev)
+                elif m == "extrapolated":
+                    if k == 0:
+                        xg.copy_(x_prev)
+                    else:
+                        prev_int = W.phi_prev.view(n + 4, n + 4, n + 4)[2:-2, 2:-2, 2:-2].reshape(-1)
+                        torch.sub(x_prev * 2.0, prev_int, out=xg)
+                else:
+                    net.predict(W.grid, W.phi, W.phi_prev, W.rhs, W.rhs_scale, xg, extras=W.extras)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                r = bicgstab2(ctx, A, W.rhs, xg, tol=tol, tol_kind=1, max_it=1000)
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                if k >= 1:                          # step 0 has no phi^{n-1}: all guesses coincide
+                    stats[m]["passes"].append(r["iterations"])
+                    stats[m]["solve_s"].append(t2 - t1)
+                    stats[m]["guess_s"].append(t1 - t0)
+                if m == "previous":
+                    keep = xg.clone()
+            W.advance(keep)
+            if time.perf_counter() - t_log > log_every:
+                t_log = time.perf_counter()
+                _log(f"[eval] seed {si + 1}/{len(eval_seeds)} step {k}: "
+                     + ", ".join(f"{m} {np.mean(stats[m]['passes'] or [0]):.2f}" for m in modes))
+    out = {}
+    for m in modes:
+        p = np.array(stats[m]["passes"], float)
+        out[m] = {"passes_mean": float(p.mean()), "passes_total": int(p.sum()),
+                  "solve_ms_mean": 1e3 * float(np.mean(stats[m]["solve_s"])),
+                  "guess_ms_mean": 1e3 * float(np.mean(stats[m]["guess_s"])),
+                  "step_ms_mean": 1e3 * float(np.mean(stats[m]["solve_s"]) + np.mean(stats[m]["guess_s"]))}
+    base = out["previous"]
+    for m in modes:
+        out[m]["passes_saved_vs_previous"] = 1.0 - out[m]["passes_mean"] / base["passes_mean"]
+        out[m]["time_saved_vs_previous"] = 1.0 - out[m]["step_ms_mean"] / base["step_ms_mean"]
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train", "eval", "train-eval"])
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="advdiff")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--train-seeds", type=int, default=200)
+    ap.add_argument("--eval-seeds", type=int, default=20)
+    ap.add_argument("--eval-steps", type=int, default=None)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--train-every", type=int, default=1)
+    ap.add_argument("--max-seconds", type=float, default=None)
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--report", default=None)
+    a = ap.parse_args(argv)
+    ctx = Context(0)
+    report = {}
+    net = None
+    if a.cmd in ("train", "train-eval"):
+        init = None
+        if a.weights:
+            init, _ = WarmStartNet.load_params(a.weights)
+        net = WarmStartNet(ctx, [a.n] * 3, in_channels=WORKLOADS[a.workload].in_channels_for(), params=init)
+        net, info = train(ctx, n=a.n, train_seeds=range(1000, 1000 + a.train_seeds), steps=a.steps, lr=a.lr,
+                          net=net, train_every=a.train_every, max_seconds=a.max_seconds, workload=a.workload)
+        report["train"] = info
+        _log(f"[train] {info}")
+        if a.out:
+            net.save(a.out, meta={"train": info, "workload": a.workload, "tol": TOL})
+    if a.cmd in ("eval", "train-eval"):
+        if net is None:
+            w, meta = WarmStartNet.load_params(a.weights)
+            net = WarmStartNet(ctx, [a.n] * 3, in_channels=meta["in_channels"], params=w)
+        ev = evaluate(ctx, net, n=a.n, eval_seeds=range(5000, 5000 + a.eval_seeds), steps=a.eval_steps or a.steps,
+                      workload=a.workload)
+        report["eval"] = ev
+        _log(f"[eval] {json.dumps(ev)}")
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(report, f, indent=1)
+    print(json.dumps(report))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -144,3 +144,131 @@ class C4Batch:
         self.batch.close()
         for A in self.csr:
             A.close()
+
+
+def gaussian_sum(n, rng, count=8, ghost=2, device="cuda"):
+    """sum of `count` Gaussians, centres U(unit cube), sigma U[0.05, 0.2], amplitude U[-1, 1],
+    at the cell centres of the padded grid (ghosts included; callers overwrite them)."""
+    torch = _torch()
+    xc = torch.tensor(cell_centers_1d(n, 1.0, ghost), dtype=torch.float64, device=device)
+    X, Y, Z = xc.view(-1, 1, 1), xc.view(1, -1, 1), xc.view(1, 1, -1)
+    phi = torch.zeros((n + 2 * ghost,) * 3, dtype=torch.float64, device=device)
+    c = rng.uniform(0.0, 1.0, size=(count, 3))
+    sig = rng.uniform(0.05, 0.2, size=count)
+    amp = rng.uniform(-1.0, 1.0, size=count)
+    for q in range(count):
+        r2 = (X - c[q, 0]) ** 2 + (Y - c[q, 1]) ** 2 + (Z - c[q, 2]) ** 2
+        phi += amp[q] * torch.exp(-r2 / (2 * sig[q] ** 2))
+    return phi.contiguous().view(-1)
+
+
+class C5AdvDiff:
+    """Transient advection-diffusion trajectory on an n^3 unit cube (learned-form workload).
+
+    Per seed: D ~ U[1e-4, 1e-2], constant velocity u ~ U[-1, 1]^3 on every face, initial field
+    a sum of 8 Gaussians; implicit Euler with dt, van Leer TVD convection (upwind implicit part
+    + deferred correction from phi^n), Dirichlet 0 walls (ghost cells held at 0). One shared
+    matrix structure serves every seed of a resolution (pass it as `A`).
+    """
+
+    @staticmethod
+    def in_channels_for():
+        return 3
+
+
+    def __init__(self, ctx, seed, n=128, dt=5e-3, A=None, device="cuda"):
+        from . import VANLEER, Csr, build_structure, lib as _lib
+        import ctypes
+        torch = _torch()
+        rng = np.random.default_rng(seed)
+        self.ctx, self.n, self.dt, self.seed = ctx, n, dt, seed
+        self.D = float(rng.uniform(1e-4, 1e-2))
+        self.u = rng.uniform(-1.0, 1.0, size=3)
+        self.grid = Grid.make([n, n, n], size=[1.0, 1.0, 1.0], ghost=2)
+        self.vel = []
+        for d in range(3):
+            nf = [n, n, n]
+            nf[d] += 1
+            self.vel.append(torch.full((nf[0] * nf[1] * nf[2],), float(self.u[d]), dtype=torch.float64,
+                                       device=device))
+        self.terms = Terms.make(dt=dt, diff=self.D, limiter=VANLEER, vel=self.vel, bc_type=[BC_DIRICHLET] * 6,
+                                bc_value=[0.0] * 6)
+        self.phi = gaussian_sum(n, rng, device=device)
+        self.ghost_modes, self.ghost_values = [GHOST_CONSTANT] * 6, [0.0] * 6
+        self.phi_prev = self.phi.clone()
+        self.rhs = torch.zeros(n ** 3, dtype=torch.float64, device=device)
+        self.x = torch.zeros_like(self.rhs)
+        self.A = A
+        if self.A is None:
+            self.A = Csr(ctx)
+            build_structure(ctx, self.A, self.grid, self.terms)
+        self.rhs_scale = dt / (1.0 / n) ** 3      # rhs * dt / V: the V/dt phi^n term in field units
+        self.step_index = 0
+        self.extras = []
+        self.in_channels = 3
+
+    def assemble(self):
+        """ghost fill, then values / rhs / x0 = phi^n of the next implicit step"""
+        from . import fill_values, ghost_fill
+        ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
+        fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
+
+    def advance(self, x_solution):
+        """phi^{n-1} <- phi^n, phi^n <- solution of the step"""
+        from . import copy_to_field
+        self.phi_prev.copy_(self.phi)
+        copy_to_field(self.ctx, self.grid, x_solution, self.phi)
+        self.step_index += 1
+
+
+class VarDiffusion:
+    """Transient variable-diffusivity diffusion on an n^3 unit cube (the precision/learned target).
+
+    Per seed: cell diffusivity ln D ~ N(ln 1e-3, 1) (face value: harmonic mean), source
+    U[0, 1) per cell (rhs += V s), Dirichlet 0 walls, initial field a sum of 8 Gaussians, and
+    dt chosen so that the mean of D dt / dx^2 is `mean_diff_number`. The network sees ln D as a
+    fourth input channel ((ln D - ln 1e-3) / 1).
+    """
+
+    @staticmethod
+    def in_channels_for():
+        return 4
+
+
+    def __init__(self, ctx, seed, n=256, mean_diff_number=1.0, A=None, device="cuda"):
+        from . import Csr, build_structure
+        torch = _torch()
+        rng = np.random.default_rng(seed)
+        self.ctx, self.n, self.seed = ctx, n, seed
+        self.D, self.src, self.dt = c3_lognormal(n, device=device, seed=seed, mean_diff_number=mean_diff_number)
+        self.lnD = torch.log(self.D)
+        self.grid = Grid.make([n, n, n], size=[1.0, 1.0, 1.0], ghost=2)
+        self.terms = Terms.make(dt=self.dt, diff_cell=self.D, source=self.src, bc_type=[BC_DIRICHLET] * 6,
+                                bc_value=[0.0] * 6)
+        self.phi = gaussian_sum(n, rng, device=device)
+        self.ghost_modes, self.ghost_values = [GHOST_CONSTANT] * 6, [0.0] * 6
+        self.phi_prev = self.phi.clone()
+        self.rhs = torch.zeros(n ** 3, dtype=torch.float64, device=device)
+        self.x = torch.zeros_like(self.rhs)
+        self.A = A
+        if self.A is None:
+            self.A = Csr(ctx)
+            build_structure(ctx, self.A, self.grid, self.terms)
+        self.rhs_scale = self.dt / (1.0 / n) ** 3
+        self.step_index = 0
+        self.extras = [(self.lnD, 1.0, math.log(1e-3))]
+        self.in_channels = 4
+
+    def assemble(self):
+        from . import fill_values, ghost_fill
+        ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
+        fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
+
+    def advance(self, x_solution):
+        from . import copy_to_field
+        self.phi_prev.copy_(self.phi)
+        copy_to_field(self.ctx, self.grid, x_solution, self.phi)
+        self.step_index += 1
+
+
+WORKLOADS = {"advdiff": C5AdvDiff, "vardiff": VarDiffusion}
