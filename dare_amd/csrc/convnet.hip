@@ -6,9 +6,9 @@
 //
 // Tensors are channels-last BF16 ([voxel][channel], voxel = (z * H + y) * W + x with x = the
 // grid's fastest index k), weights BF16 with FP32 master copies, accumulation in FP32 on the
-// matrix cores. Every convolution is an implicit GEMM on v_mfma_f32_16x16x16_bf16: a wave
-// computes 16 consecutive voxels x all output channels, with K running over 27 taps x input
-// channels in blocks of 16. Back-propagation of the data gradient is the same kernel with the
+// matrix cores. Every convolution is an implicit GEMM on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16): a wave computes up to 64 consecutive voxels x all output
+// channels, with K running over 27 taps x input channels. Back-propagation of the data gradient is the same kernel with the
 // weights flipped and transposed; the weight gradient is a GEMM over voxels (M = output
 // channels, N = input channels) written as per-chunk partials and summed in chunk order.
 //
@@ -57,61 +57,88 @@ __device__ __host__ inline float bf2f(uint16_t h) {
 
 // ------------------------------------------------------------------------------ convolution
 // Y[v][co] = sum_tap sum_ci X[v + off(tap)][ci] * Wt[tap][co][ci], zero outside the grid.
-// Block = 4 waves = 64 voxels along x at one (z, y). MFMA 16x16x16 operand layout: lane l
-// holds A[l % 16][4 (l / 16) + e] (voxel, input channel), B[4 (l / 16) + e][l % 16]
-// (input channel, output channel) and D[4 (l / 16) + e][l % 16] (voxel, output channel).
+//
+// Computed transposed, D[co][v] = sum_k Wt[co][k] X^T[k][v], on v_mfma_f32_16x16x32_bf16
+// (K = 32 per instruction). Operand layout: lane l holds A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16] (j = 0..7, 16 contiguous bytes each: eight channels of one output
+// channel's weights / of one voxel), and D[4 (l / 16) + j][l % 16] (j = 0..3: four consecutive
+// output channels of one voxel, stored as one 8-byte write). K runs over 32-channel slices of
+// one tap, or, for 16-channel inputs, over two taps at once. A wave owns NSEG x 16 voxels of one
+// (z, y) row, so each weight fragment feeds NSEG instructions; a block is 4 such rows.
 // OUT: 0 BF16 all channels, 1 FP32 channel 0 only.
-template <int CIN, int COUT, bool RELU, int OUT>
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline bf16x8 load8(const uint16_t* p, bool ok) {
+    if (!ok) return bf16x8{};
+    const uint4 u = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+template <int CIN, int COUT, bool RELU, int OUT, int NSEG>
 __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
                                                      uint16_t* __restrict__ Y, float* __restrict__ Yf, int D, int H,
                                                      int W) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 64 + wave * 16;
-    const int y = blockIdx.y, z = blockIdx.z;
-    if (x0 >= W) return;                       // whole wave (no block barrier below)
+    const int y = blockIdx.y * 4 + wave;
+    const int z = blockIdx.z;
+    const int xw = blockIdx.x * (16 * NSEG);
+    if (y >= H) return;                        // whole wave (no block barrier below)
     const int i = lane & 15, kq = lane >> 4;
-    floatx4 acc[COUT / 16];
+    constexpr int NB = COUT / 16;
+    constexpr int NCHUNK = (CIN == 16) ? 14 : 27 * (CIN / 32);
+    floatx4 acc[NSEG][NB];
 #pragma unroll
-    for (int nb = 0; nb < COUT / 16; nb++) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int dz = -1; dz <= 1; dz++) {
-        const int zz = z + dz;
-        if (zz < 0 || zz >= D) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-            const int yy = y + dy;
-            if (yy < 0 || yy >= H) continue;
-            const long long rowv = (static_cast<long long>(zz) * H + yy) * W;
+    for (int s = 0; s < NSEG; s++)
 #pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-                const int xx = x0 + i + dx;
-                const bool inb = xx >= 0 && xx < W;
-                const uint16_t* xp = X + (rowv + (inb ? xx : 0)) * CIN + kq * 4;
-                const uint16_t* wp = Wt + (static_cast<long long>(tap) * COUT + i) * CIN + kq * 4;
+        for (int nb = 0; nb < NB; nb++) acc[s][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < NCHUNK; c++) {
+        int tap, ch;
+        if (CIN == 16) {
+            tap = 2 * c + (kq >> 1);
+            ch = 8 * (kq & 1);
+        } else {
+            tap = c / (CIN / 32);
+            ch = 32 * (c % (CIN / 32)) + 8 * kq;
+        }
+        const bool tap_ok = tap < 27;
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int zz = z + dz, yy = y + dy;
+        const bool row_ok = tap_ok && zz >= 0 && zz < D && yy >= 0 && yy < H;
+        bf16x8 wa[NB];
 #pragma unroll
-                for (int cb = 0; cb < CIN / 16; cb++) {
-                    bf16x4 a = inb ? *reinterpret_cast<const bf16x4*>(xp + cb * 16) : bf16x4{0, 0, 0, 0};
+        for (int nb = 0; nb < NB; nb++)
+            wa[nb] = load8(Wt + (static_cast<long long>(tap_ok ? tap : 0) * COUT + nb * 16 + i) * CIN + ch, tap_ok);
+        const long long rowv = (static_cast<long long>(row_ok ? zz : 0) * H + (row_ok ? yy : 0)) * W;
 #pragma unroll
-                    for (int nb = 0; nb < COUT / 16; nb++) {
-                        const bf16x4 b = *reinterpret_cast<const bf16x4*>(wp + nb * 16 * CIN + cb * 16);
-                        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
-                    }
-                }
-            }
+        for (int s = 0; s < NSEG; s++) {
+            const int xx = xw + 16 * s + i + dx;
+            const bool ok = row_ok && xx >= 0 && xx < W;
+            const bf16x8 xb = load8(X + (rowv + (ok ? xx : 0)) * CIN + ch, ok);
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+                acc[s][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[nb], xb, acc[s][nb], 0, 0, 0);
         }
     }
     const long long vrow = (static_cast<long long>(z) * H + y) * W;
 #pragma unroll
-    for (int nb = 0; nb < COUT / 16; nb++) {
+    for (int s = 0; s < NSEG; s++) {
+        const int v = xw + 16 * s + i;
+        if (v >= W) continue;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int xv = x0 + kq * 4 + e;
-            if (xv >= W) continue;
-            float v = acc[nb][e];
-            if (RELU) v = v > 0.f ? v : 0.f;
+        for (int nb = 0; nb < NB; nb++) {
+            float o[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                o[j] = acc[s][nb][j];
+                if (RELU) o[j] = o[j] > 0.f ? o[j] : 0.f;
+            }
             if (OUT == 0) {
-                Y[(vrow + xv) * COUT + nb * 16 + i] = f2bf(v);
-            } else if (nb == 0 && i == 0) {
-                Yf[vrow + xv] = v;
+                uint2 pk;
+                pk.x = static_cast<uint32_t>(f2bf(o[0])) | (static_cast<uint32_t>(f2bf(o[1])) << 16);
+                pk.y = static_cast<uint32_t>(f2bf(o[2])) | (static_cast<uint32_t>(f2bf(o[3])) << 16);
+                *reinterpret_cast<uint2*>(Y + (vrow + v) * COUT + nb * 16 + kq * 4) = pk;
+            } else if (nb == 0 && kq == 0) {
+                Yf[vrow + v] = o[0];
             }
         }
     }
@@ -284,13 +311,20 @@ __global__ __launch_bounds__(kNetThreads) void k_mask(const uint16_t* __restrict
 // ------------------------------------------------------------------------------ solver side
 // Normalisation scales: s[0] = max|phi^n|, s[1] = max|phi^n - phi^{n-1}|,
 // s[2] = max|rhs * rhs_scale - phi^n| over the internal cells (block maxima, then one block).
+// The network may see a sub-box (crop) of the grid: n0..n2 are its sizes, o0..o2 its origin
+// among the internal cells, m1, m2 the grid's internal sizes (row order of the solver vectors).
 struct FieldGeo {
     int n0, n1, n2, g;
+    int o0, o1, o2, m1, m2;
     long long s0, s1;    // padded strides of i and j
 };
 
 __device__ inline long long pad_index(const FieldGeo& F, int z, int y, int x) {
-    return (z + F.g) * F.s0 + (y + F.g) * F.s1 + (x + F.g);
+    return (z + F.o0 + F.g) * F.s0 + (y + F.o1 + F.g) * F.s1 + (x + F.o2 + F.g);
+}
+
+__device__ inline long long row_index(const FieldGeo& F, int z, int y, int x) {
+    return (static_cast<long long>(z + F.o0) * F.m1 + (y + F.o1)) * F.m2 + (x + F.o2);
 }
 
 __global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double* __restrict__ phi,
@@ -307,7 +341,7 @@ __global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double
         const double pm = phim[pad_index(F, z, y, x)];
         m0 = fmax(m0, fabs(p));
         m1 = fmax(m1, fabs(p - pm));
-        m2 = fmax(m2, fabs(rhs[q] * rhs_scale - p));
+        m2 = fmax(m2, fabs(rhs[row_index(F, z, y, x)] * rhs_scale - p));
     }
     for (int off = 32; off > 0; off >>= 1) {
         m0 = fmax(m0, __shfl_down(m0, off));
@@ -359,7 +393,7 @@ __global__ __launch_bounds__(kNetThreads) void k_pack(FieldGeo F, const double* 
         uint16_t ch[kInChannels];
         ch[0] = f2bf(static_cast<float>(p * i0));
         ch[1] = f2bf(static_cast<float>((p - phim[pi]) * i1));
-        ch[2] = f2bf(static_cast<float>((rhs[q] * rhs_scale - p) * i2));
+        ch[2] = f2bf(static_cast<float>((rhs[row_index(F, z, y, x)] * rhs_scale - p) * i2));
         for (int c = 3; c < kInChannels; c++) {
             const int e = c - 3;
             ch[c] = e < ex.n ? f2bf(static_cast<float>((ex.f[e][pi] - ex.offset[e]) * ex.scale[e])) : uint16_t(0);
@@ -378,7 +412,7 @@ __global__ __launch_bounds__(kNetThreads) void k_unpack(FieldGeo F, const double
         const int x = static_cast<int>(q % F.n2);
         const int y = static_cast<int>((q / F.n2) % F.n1);
         const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
-        xg[q] = phi[pad_index(F, z, y, x)] + s1 * static_cast<double>(out[q]);
+        xg[row_index(F, z, y, x)] = phi[pad_index(F, z, y, x)] + s1 * static_cast<double>(out[q]);
     }
 }
 
@@ -395,7 +429,7 @@ __global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* _
         const int x = static_cast<int>(q % F.n2);
         const int y = static_cast<int>((q / F.n2) % F.n1);
         const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
-        const double t = (xt[q] - phi[pad_index(F, z, y, x)]) * inv;
+        const double t = (xt[row_index(F, z, y, x)] - phi[pad_index(F, z, y, x)]) * inv;
         const double d = static_cast<double>(out[q]) - t;
         acc += d * d;
         dout[q * kInChannels] = f2bf(static_cast<float>(2. * d / static_cast<double>(n)));
@@ -510,8 +544,17 @@ long long level_voxels(const int* d, int level) {
 template <int CIN, int COUT, bool RELU, int OUT>
 void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, float* Yf, const int* d, int level) {
     const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
-    dim3 grid((W + 63) / 64, H, D);
-    hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT>), grid, dim3(kNetThreads), 0, sm, X, Wt, Y, Yf, D, H, W);
+    const dim3 blk(kNetThreads);
+    if (W >= 64) {
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 4>), dim3((W + 63) / 64, (H + 3) / 4, D), blk, 0, sm, X, Wt,
+                           Y, Yf, D, H, W);
+    } else if (W >= 32) {
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 2>), dim3((W + 31) / 32, (H + 3) / 4, D), blk, 0, sm, X, Wt,
+                           Y, Yf, D, H, W);
+    } else {
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1>), dim3((W + 15) / 16, (H + 3) / 4, D), blk, 0, sm, X, Wt,
+                           Y, Yf, D, H, W);
+    }
 }
 
 // dispatch on (cin, cout) of the network's layers
@@ -562,11 +605,16 @@ int wgrad_any(dare_convnet* net, hipStream_t sm, int layer, const uint16_t* X, c
     return DARE_OK;
 }
 
-FieldGeo field_geo(const dare_grid* g) {
+FieldGeo field_geo(const dare_grid* g, const int32_t* crop) {
     FieldGeo F;
-    F.n0 = g->n[0];
-    F.n1 = g->n[1];
-    F.n2 = g->n[2];
+    F.n0 = crop ? crop[3] : g->n[0];
+    F.n1 = crop ? crop[4] : g->n[1];
+    F.n2 = crop ? crop[5] : g->n[2];
+    F.o0 = crop ? crop[0] : 0;
+    F.o1 = crop ? crop[1] : 0;
+    F.o2 = crop ? crop[2] : 0;
+    F.m1 = g->n[1];
+    F.m2 = g->n[2];
     F.g = g->ghost;
     const long long p1 = g->n[1] + 2 * g->ghost, p2 = g->n[2] + 2 * g->ghost;
     F.s1 = p2;
@@ -574,13 +622,18 @@ FieldGeo field_geo(const dare_grid* g) {
     return F;
 }
 
-int check_grid(const dare_convnet* net, const dare_grid* g) {
+int check_grid(const dare_convnet* net, const dare_grid* g, const int32_t* crop) {
     DARE_CHECK_ARG(g && g->dim == 3 && g->ncomp == 1, "the warm-start network takes 3-D scalar grids");
+    int box[3];
     for (int d = 0; d < 3; d++) {
-        DARE_CHECK_ARG(g->n[d] % 4 == 0 && g->n[d] >= 4, "grid sizes must be multiples of 4");
-        DARE_CHECK_ARG(g->n[d] <= net->max_dims[d], "grid larger than the network was created for");
+        box[d] = crop ? crop[3 + d] : g->n[d];
+        if (crop)
+            DARE_CHECK_ARG(crop[d] >= 0 && crop[3 + d] >= 1 && crop[d] + crop[3 + d] <= g->n[d],
+                           "crop outside the grid");
+        DARE_CHECK_ARG(box[d] % 4 == 0 && box[d] >= 4, "network box sizes must be multiples of 4");
+        DARE_CHECK_ARG(box[d] <= net->max_dims[d], "box larger than the network was created for");
     }
-    DARE_CHECK_ARG(static_cast<long long>(g->n[0]) * g->n[1] * g->n[2] <= net->max_v0, "grid too large");
+    DARE_CHECK_ARG(static_cast<long long>(box[0]) * box[1] * box[2] <= net->max_v0, "box too large");
     return DARE_OK;
 }
 
@@ -716,13 +769,15 @@ int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_
 // Euler term); extras: up to in_channels - 3 static padded fields, (f - offset) * scale.
 int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
                       const double* rhs, double rhs_scale, int32_t nextra, const double* const* extras,
-                      const double* extra_scale, const double* extra_offset) {
+                      const double* extra_scale, const double* extra_offset, const int32_t* crop) {
     DARE_CHECK_ARG(ctx && net && phi && phi_prev && rhs, "NULL argument");
-    int rc = check_grid(net, g);
+    int rc = check_grid(net, g, crop);
     if (rc) return rc;
     DARE_CHECK_ARG(nextra >= 0 && nextra <= net->in_channels - 3 && nextra <= 8, "too many extra channels");
-    for (int d = 0; d < 3; d++) net->dims[d] = g->n[d];
-    const FieldGeo F = field_geo(g);
+    const FieldGeo F = field_geo(g, crop);
+    net->dims[0] = F.n0;
+    net->dims[1] = F.n1;
+    net->dims[2] = F.n2;
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
     Extras ex{};
     ex.n = nextra;
@@ -775,11 +830,14 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
 }
 
 // x_guess[row] = phi[row's cell] + S * out[row]
-int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess) {
+int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess,
+                        const int32_t* crop) {
     DARE_CHECK_ARG(ctx && net && phi && x_guess, "NULL argument");
-    int rc = check_grid(net, g);
+    int rc = check_grid(net, g, crop);
     if (rc) return rc;
-    const FieldGeo F = field_geo(g);
+    const FieldGeo F = field_geo(g, crop);
+    DARE_CHECK_ARG(F.n0 == net->dims[0] && F.n1 == net->dims[1] && F.n2 == net->dims[2],
+                   "box differs from the one of the last forward pass");
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
     hipLaunchKernelGGL(k_unpack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, net->out, net->scales,
                        x_guess);
@@ -789,15 +847,17 @@ int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, co
 
 // MSE against the solver's solution x_true (row order) and the output gradient; loss to host.
 int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* x_true,
-                      double* loss_host) {
+                      double* loss_host, const int32_t* crop) {
     DARE_CHECK_ARG(ctx && net && phi && x_true, "NULL argument");
-    int rc = check_grid(net, g);
+    int rc = check_grid(net, g, crop);
     if (rc) return rc;
     if (!net->dout) {
         rc = dalloc(&net->dout, net->max_v0 * 16);
         if (rc) return rc;
     }
-    const FieldGeo F = field_geo(g);
+    const FieldGeo F = field_geo(g, crop);
+    DARE_CHECK_ARG(F.n0 == net->dims[0] && F.n1 == net->dims[1] && F.n2 == net->dims[2],
+                   "box differs from the one of the last forward pass");
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
     const unsigned gs = std::min<unsigned>(ew_grid(n), 1024);
     hipLaunchKernelGGL(k_mse, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, x_true, net->out, net->scales,

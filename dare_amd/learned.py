@@ -33,10 +33,10 @@ def _sig():
         "dare_convnet_num_params": (i32, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "dare_convnet_set_params": (i32, [vp, vp, vp]),
         "dare_convnet_get_params": (i32, [vp, vp, vp, vp]),
-        "dare_convnet_pack": (i32, [vp, vp, G, vp, vp, vp, dbl, i32, ctypes.POINTER(vp), vp, vp]),
+        "dare_convnet_pack": (i32, [vp, vp, G, vp, vp, vp, dbl, i32, ctypes.POINTER(vp), vp, vp, vp]),
         "dare_conv_net_fwd": (i32, [vp, vp]),
-        "dare_convnet_unpack": (i32, [vp, vp, G, vp, vp]),
-        "dare_convnet_loss": (i32, [vp, vp, G, vp, vp, ctypes.POINTER(dbl)]),
+        "dare_convnet_unpack": (i32, [vp, vp, G, vp, vp, vp]),
+        "dare_convnet_loss": (i32, [vp, vp, G, vp, vp, ctypes.POINTER(dbl), vp]),
         "dare_conv_net_bwd": (i32, [vp, vp]),
         "dare_conv_net_step": (i32, [vp, vp, flt, flt, flt, flt]),
         "dare_convnet_tensors": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(dbl)]),
@@ -109,21 +109,25 @@ class WarmStartNet:
             return z["params"].astype(np.float32), json.loads(str(z["meta"]))
 
     # ------------------------------------------------------------------ passes
-    def pack(self, grid, phi, phi_prev, rhs, rhs_scale, extras=()):
+    @staticmethod
+    def _crop(crop):
+        return None if crop is None else (ctypes.c_int32 * 6)(*crop)
+
+    def pack(self, grid, phi, phi_prev, rhs, rhs_scale, extras=(), crop=None):
         ne = len(extras)
         fptr = (ctypes.c_void_p * max(ne, 1))(*[_ptr(f) for f, _, _ in extras])
         sc = np.array([s for _, s, _ in extras] or [0.0], np.float64)
         of = np.array([o for _, _, o in extras] or [0.0], np.float64)
         check(_sig().dare_convnet_pack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(phi_prev), _ptr(rhs),
-                                       float(rhs_scale), ne, fptr, sc.ctypes.data, of.ctypes.data),
-              "dare_convnet_pack")
+                                       float(rhs_scale), ne, fptr, sc.ctypes.data, of.ctypes.data,
+                                       self._crop(crop)), "dare_convnet_pack")
 
     def forward(self):
         check(_sig().dare_conv_net_fwd(self.ctx.h, self.h), "dare_conv_net_fwd")
 
-    def unpack(self, grid, phi, x_out):
-        check(_sig().dare_convnet_unpack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(x_out)),
-              "dare_convnet_unpack")
+    def unpack(self, grid, phi, x_out, crop=None):
+        check(_sig().dare_convnet_unpack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(x_out),
+                                         self._crop(crop)), "dare_convnet_unpack")
 
     def predict(self, grid, phi, phi_prev, rhs, rhs_scale, x_out, extras=()):
         """x_out (solver rows) = the network's guess of the next step."""
@@ -131,10 +135,10 @@ class WarmStartNet:
         self.forward()
         self.unpack(grid, phi, x_out)
 
-    def loss(self, grid, phi, x_true, readback=True):
+    def loss(self, grid, phi, x_true, readback=True, crop=None):
         v = ctypes.c_double()
         check(_sig().dare_convnet_loss(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(x_true),
-                                       ctypes.byref(v) if readback else None), "dare_convnet_loss")
+                                       ctypes.byref(v) if readback else None, self._crop(crop)), "dare_convnet_loss")
         return v.value if readback else None
 
     def backward(self):
@@ -143,10 +147,11 @@ class WarmStartNet:
     def step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         check(_sig().dare_conv_net_step(self.ctx.h, self.h, lr, beta1, beta2, eps), "dare_conv_net_step")
 
-    def train_step(self, grid, phi, phi_prev, rhs, rhs_scale, x_true, lr=1e-3, extras=(), readback=True):
-        self.pack(grid, phi, phi_prev, rhs, rhs_scale, extras)
+    def train_step(self, grid, phi, phi_prev, rhs, rhs_scale, x_true, lr=1e-3, extras=(), readback=True,
+                   crop=None):
+        self.pack(grid, phi, phi_prev, rhs, rhs_scale, extras, crop)
         self.forward()
-        v = self.loss(grid, phi, x_true, readback)
+        v = self.loss(grid, phi, x_true, readback, crop)
         self.backward()
         self.step(lr)
         return v

@@ -32,10 +32,12 @@ def _log(msg):
 
 
 def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_final=1e-4, net=None,
-          train_every=1, log_every=60.0, max_seconds=None, seed=0, workload="advdiff"):
-    """Online training on plain-solver trajectories. Returns (net, info dict)."""
+          train_every=1, log_every=60.0, max_seconds=None, seed=0, workload="advdiff", crop=None):
+    """Online training on plain-solver trajectories. With `crop` = c, each step trains on one
+    random c^3 sub-box of the grid (the network is fully convolutional). Returns (net, info)."""
     import torch
     make = WORKLOADS[workload]
+    crng = np.random.default_rng(seed + 17)
     net = net or WarmStartNet(ctx, [n, n, n], in_channels=make.in_channels_for(), seed=seed)
     seeds = list(train_seeds)
     total = len(seeds) * max(steps - 1, 1) // train_every
@@ -52,8 +54,12 @@ def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_fina
             if k >= 1 and k % train_every == 0:
                 frac = min(done / max(total, 1), 1.0)
                 cur_lr = lr * (lr_final / lr) ** frac
+                box = None
+                if crop:
+                    o = [int(crng.integers(0, n - crop + 1)) for _ in range(3)]
+                    box = o + [crop] * 3
                 losses.append(net.train_step(W.grid, W.phi, W.phi_prev, W.rhs, W.rhs_scale, W.x, lr=cur_lr,
-                                             extras=W.extras))
+                                             extras=W.extras, crop=box))
                 done += 1
             W.advance(W.x)
             now = time.perf_counter()
@@ -66,7 +72,7 @@ def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_fina
         if max_seconds and time.perf_counter() - t_start > max_seconds:
             break
     torch.cuda.synchronize()
-    info = {"workload": workload, "samples": done, "seeds_used": si + 1, "steps": steps, "n": n,
+    info = {"workload": workload, "samples": done, "seeds_used": si + 1, "steps": steps, "n": n, "crop": crop,
             "seconds": round(time.perf_counter() - t_start, 1),
             "loss_first_100": float(np.mean(losses[:100])) if losses else None,
             "loss_last_100": float(np.mean(losses[-100:])) if losses else None,
@@ -149,6 +155,7 @@ def main(argv=None):
     ap.add_argument("--eval-steps", type=int, default=None)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--train-every", type=int, default=1)
+    ap.add_argument("--crop", type=int, default=None)
     ap.add_argument("--max-seconds", type=float, default=None)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--out", default=None)
@@ -163,7 +170,8 @@ def main(argv=None):
             init, _ = WarmStartNet.load_params(a.weights)
         net = WarmStartNet(ctx, [a.n] * 3, in_channels=WORKLOADS[a.workload].in_channels_for(), params=init)
         net, info = train(ctx, n=a.n, train_seeds=range(1000, 1000 + a.train_seeds), steps=a.steps, lr=a.lr,
-                          net=net, train_every=a.train_every, max_seconds=a.max_seconds, workload=a.workload)
+                          net=net, train_every=a.train_every, max_seconds=a.max_seconds, workload=a.workload,
+                          crop=a.crop)
         report["train"] = info
         _log(f"[train] {info}")
         if a.out:

@@ -233,13 +233,16 @@ int dare_convnet_destroy(dare_convnet* net);
 int dare_convnet_num_params(const dare_convnet* net, int64_t* n, int64_t* layer_offsets);
 int dare_convnet_set_params(dare_ctx* ctx, dare_convnet* net, const float* host);
 int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_w, float* host_grad);
+/* crop: NULL for the whole grid, else {origin[3], size[3]} of a sub-box of internal cells
+ * (training on crops; the network's zero padding then applies at the crop's faces). */
 int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
                       const double* rhs, double rhs_scale, int32_t nextra, const double* const* extras,
-                      const double* extra_scale, const double* extra_offset);
+                      const double* extra_scale, const double* extra_offset, const int32_t* crop);
 int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net);
-int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess);
+int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess,
+                        const int32_t* crop);
 int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* x_true,
-                      double* loss_host);
+                      double* loss_host, const int32_t* crop);
 int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net);
 int dare_conv_net_step(dare_ctx* ctx, dare_convnet* net, float lr, float beta1, float beta2, float eps);
 int dare_convnet_tensors(const dare_convnet* net, void** ptrs, double* scales_host);

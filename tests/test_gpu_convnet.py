@@ -71,6 +71,23 @@ def test_pack_bit_exact(ctx, rng):
     net.close()
 
 
+def test_pack_crop_matches_oracle(ctx, rng):
+    """A crop packs the sub-box: same values as the oracle on the sliced fields."""
+    net = WarmStartNet(ctx, N, in_channels=3, seed=3)
+    big = [16, 16, 20]
+    g = da.Grid.make(big, size=[1, 1, 1], ghost=2)
+    dev, (phi, phim, rhs, ext) = fields(rng, big)
+    o = [4, 2, 3]
+    net.pack(g, dev[0], dev[1], dev[2], 0.7, crop=o + N)
+    ptr, scales = net.tensors()
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    sl = tuple(slice(o[d], o[d] + N[d]) for d in range(3))
+    exp, s = cn.pack(phi[sl], phim[sl], rhs.reshape(big)[sl], 0.7)
+    assert np.allclose(scales, s, rtol=0, atol=0)
+    assert np.array_equal(x0, exp)
+    net.close()
+
+
 def test_forward_matches_oracle(ctx, rng):
     net, g, dev, hostf, w, _ = setup(ctx, rng)
     net.forward()
