@@ -155,6 +155,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise DareError(f"{LIB_PATH} not found: build it with `python -m dare_amd.build`")
+    # torch's ROCm wheel ships its own HIP runtime with the same soname as the system one.
+    # Loading torch first makes this library bind to that runtime: with two HIP runtimes in
+    # one process, the one loaded second sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     G = ctypes.POINTER(Grid)
@@ -193,6 +200,7 @@ def lib():
         "dare_dot": (i32, [vp, i64, vp, vp, ctypes.POINTER(dbl)]),
         "dare_spmv_dot": (i32, [vp, vp, vp, vp, vp, ctypes.POINTER(dbl)]),
         "dare_axpby": (i32, [vp, i64, dbl, vp, dbl, vp]),
+        "dare_residual": (i32, [vp, vp, vp, vp, vp, ctypes.POINTER(dbl)]),
         "dare_bicgstab2_solve": (i32, [vp, vp, vp, vp, P, R]),
         "dare_refine_solve_mixed": (i32, [vp, vp, vp, vp, P, R]),
         "dare_batch_create": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
@@ -359,6 +367,14 @@ def dot(ctx, a, b):
     out = ctypes.c_double()
     check(lib().dare_dot(ctx.h, a.numel(), _ptr(a), _ptr(b), ctypes.byref(out)), "dare_dot")
     return out.value
+
+
+def residual(ctx, csr, b, x, r, norm=False):
+    """r = b - A x with the solver's operator; returns ||r|| when norm is True."""
+    v = ctypes.c_double()
+    check(lib().dare_residual(ctx.h, csr.h, _ptr(b), _ptr(x), _ptr(r), ctypes.byref(v) if norm else None),
+          "dare_residual")
+    return v.value if norm else None
 
 
 def copy_to_field(ctx, grid, x, phi):

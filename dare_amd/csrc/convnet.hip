@@ -19,7 +19,8 @@
 //   up,   L7 64->32, + e2, L8 32->32                               d2   1/2
 //   up,   L9 32->16, + e1, L10 16->16                              d1   full
 //   L11 16->16 (only output channel 0 is trained and read)        out  full, FP32
-// ReLU after every layer but L11. Prediction: phi^{n+1} = phi^n + S * out, S = max|phi^n - phi^{n-1}|.
+// ReLU after every layer but L11. Prediction: phi^{n+1} = phi^n + S * out with S = max|r0| dt / V,
+// r0 = b - A phi^n the residual of the reference's guess (the size of one Richardson step).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -310,7 +311,8 @@ __global__ __launch_bounds__(kNetThreads) void k_mask(const uint16_t* __restrict
 
 // ------------------------------------------------------------------------------ solver side
 // Normalisation scales: s[0] = max|phi^n|, s[1] = max|phi^n - phi^{n-1}|,
-// s[2] = max|rhs * rhs_scale - phi^n| over the internal cells (block maxima, then one block).
+// s[2] = max|resid * resid_scale| over the cells of the box (block maxima, then one block);
+// s[2] is also the scale of the output: the guess is phi^n + s[2] * out.
 // The network may see a sub-box (crop) of the grid: n0..n2 are its sizes, o0..o2 its origin
 // among the internal cells, m1, m2 the grid's internal sizes (row order of the solver vectors).
 struct FieldGeo {
@@ -328,8 +330,8 @@ __device__ inline long long row_index(const FieldGeo& F, int z, int y, int x) {
 }
 
 __global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double* __restrict__ phi,
-                                                       const double* __restrict__ phim, const double* __restrict__ rhs,
-                                                       double rhs_scale, double* __restrict__ part) {
+                                                       const double* __restrict__ phim, const double* __restrict__ resid,
+                                                       double resid_scale, double* __restrict__ part) {
     __shared__ double red[3][kNetThreads / 64];
     double m0 = 0., m1 = 0., m2 = 0.;
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
@@ -341,7 +343,7 @@ __global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double
         const double pm = phim[pad_index(F, z, y, x)];
         m0 = fmax(m0, fabs(p));
         m1 = fmax(m1, fabs(p - pm));
-        m2 = fmax(m2, fabs(rhs[row_index(F, z, y, x)] * rhs_scale - p));
+        m2 = fmax(m2, fabs(resid[row_index(F, z, y, x)] * resid_scale));
     }
     for (int off = 32; off > 0; off >>= 1) {
         m0 = fmax(m0, __shfl_down(m0, off));
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(64) void k_scales_finish(const double* __restrict__
     else s[threadIdx.x] = fmax(m, 1e-300);
 }
 
-// x0[v][c]: c0 = phi^n / s0, c1 = (phi^n - phi^{n-1}) / s1, c2 = (rhs * rhs_scale - phi^n) / s2,
+// x0[v][c]: c0 = phi^n / s0, c1 = (phi^n - phi^{n-1}) / s1, c2 = resid * resid_scale / s2,
 // c3.. = (extra_e - offset_e) * scale_e (static fields), the rest 0.
 struct Extras {
     const double* f[8];
@@ -379,8 +381,8 @@ struct Extras {
 };
 
 __global__ __launch_bounds__(kNetThreads) void k_pack(FieldGeo F, const double* __restrict__ phi,
-                                                     const double* __restrict__ phim, const double* __restrict__ rhs,
-                                                     double rhs_scale, Extras ex, const double* __restrict__ s,
+                                                     const double* __restrict__ phim, const double* __restrict__ resid,
+                                                     double resid_scale, Extras ex, const double* __restrict__ s,
                                                      uint16_t* __restrict__ x0) {
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
     const double i0 = 1. / s[0], i1 = 1. / s[1], i2 = 1. / s[2];
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(kNetThreads) void k_pack(FieldGeo F, const double* 
         uint16_t ch[kInChannels];
         ch[0] = f2bf(static_cast<float>(p * i0));
         ch[1] = f2bf(static_cast<float>((p - phim[pi]) * i1));
-        ch[2] = f2bf(static_cast<float>((rhs[row_index(F, z, y, x)] * rhs_scale - p) * i2));
+        ch[2] = f2bf(static_cast<float>(resid[row_index(F, z, y, x)] * resid_scale * i2));
         for (int c = 3; c < kInChannels; c++) {
             const int e = c - 3;
             ch[c] = e < ex.n ? f2bf(static_cast<float>((ex.f[e][pi] - ex.offset[e]) * ex.scale[e])) : uint16_t(0);
@@ -402,28 +404,28 @@ __global__ __launch_bounds__(kNetThreads) void k_pack(FieldGeo F, const double* 
     }
 }
 
-// x[row] = phi^n + s1 * out[row] (the guess, solver row order = voxel order)
+// x[row] = phi^n + s2 * out[row] (the guess, solver row order)
 __global__ __launch_bounds__(kNetThreads) void k_unpack(FieldGeo F, const double* __restrict__ phi,
                                                        const float* __restrict__ out, const double* __restrict__ s,
                                                        double* __restrict__ xg) {
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
-    const double s1 = s[1];
+    const double s2 = s[2];
     for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
         const int x = static_cast<int>(q % F.n2);
         const int y = static_cast<int>((q / F.n2) % F.n1);
         const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
-        xg[row_index(F, z, y, x)] = phi[pad_index(F, z, y, x)] + s1 * static_cast<double>(out[q]);
+        xg[row_index(F, z, y, x)] = phi[pad_index(F, z, y, x)] + s2 * static_cast<double>(out[q]);
     }
 }
 
-// loss = mean((out - t)^2), t = (x_true - phi^n) / s1; dout[v][0] = 2 (out - t) / n, other channels 0.
+// loss = mean((out - t)^2), t = (x_true - phi^n) / s2; dout[v][0] = 2 (out - t) / n, other channels 0.
 __global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* __restrict__ phi,
                                                     const double* __restrict__ xt, const float* __restrict__ out,
                                                     const double* __restrict__ s, uint16_t* __restrict__ dout,
                                                     double* __restrict__ part) {
     __shared__ double red[kNetThreads / 64];
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
-    const double inv = 1. / s[1];
+    const double inv = 1. / s[2];
     double acc = 0.;
     for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
         const int x = static_cast<int>(q % F.n2);
@@ -764,13 +766,12 @@ int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_
     return DARE_OK;
 }
 
-// Input tensor from the solver's fields. phi, phi_prev: padded grid fields; rhs: the assembled
-// right-hand side (row order); rhs_scale maps it to field units (dt / V for the implicit
-// Euler term); extras: up to in_channels - 3 static padded fields, (f - offset) * scale.
+// Input tensor from the solver's fields (channels as declared in dare_hip.h); extras: up to
+// in_channels - 3 static padded fields, (f - offset) * scale.
 int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
-                      const double* rhs, double rhs_scale, int32_t nextra, const double* const* extras,
+                      const double* resid, double resid_scale, int32_t nextra, const double* const* extras,
                       const double* extra_scale, const double* extra_offset, const int32_t* crop) {
-    DARE_CHECK_ARG(ctx && net && phi && phi_prev && rhs, "NULL argument");
+    DARE_CHECK_ARG(ctx && net && phi && phi_prev && resid, "NULL argument");
     int rc = check_grid(net, g, crop);
     if (rc) return rc;
     DARE_CHECK_ARG(nextra >= 0 && nextra <= net->in_channels - 3 && nextra <= 8, "too many extra channels");
@@ -788,11 +789,11 @@ int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
         ex.offset[e] = extra_offset[e];
     }
     const unsigned gs = std::min<unsigned>(ew_grid(n), 1024);
-    hipLaunchKernelGGL(k_scales, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, rhs, rhs_scale,
+    hipLaunchKernelGGL(k_scales, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid, resid_scale,
                        net->part);
     hipLaunchKernelGGL(k_scales_finish, dim3(1), dim3(64), 0, ctx->stream, net->part, static_cast<int>(gs), net->scales);
-    hipLaunchKernelGGL(k_pack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, rhs, rhs_scale,
-                       ex, net->scales, net->x0);
+    hipLaunchKernelGGL(k_pack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid,
+                       resid_scale, ex, net->scales, net->x0);
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }

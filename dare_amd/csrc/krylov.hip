@@ -210,6 +210,28 @@ int dare_spmv(dare_ctx* ctx, const dare_csr* A, const double* x, double* y) {
     return DARE_OK;
 }
 
+int dare_residual(dare_ctx* ctx, const dare_csr* A, const double* b, const double* x, double* r, double* norm_host) {
+    DARE_CHECK_ARG(ctx && A && b && x && r, "NULL argument");
+    Workspace W;
+    int rc = ws_reserve(ctx, 0, &W);
+    if (rc) return rc;
+    SolveOp<double> op = op64(A, true);
+    if (!op.dia) {
+        rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(64), 0, ctx->stream, W.st);
+    op.launch(ctx->stream, x, EpiResid64{b, r}, ctx->ws, W.st, TAIL_STORE, 0);
+    DARE_LAUNCH_CHECK();
+    if (norm_host) {
+        KState h;
+        rc = read_state(ctx, W.st, &h);
+        if (rc) return rc;
+        *norm_host = std::sqrt(h.dots[0]);
+    }
+    return DARE_OK;
+}
+
 int dare_spmv_dot(dare_ctx* ctx, const dare_csr* A, const double* x, double* y, const double* w,
                   double* out_host) {
     DARE_CHECK_ARG(ctx && A && x && y && w, "NULL argument");

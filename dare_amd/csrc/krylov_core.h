@@ -295,6 +295,18 @@ struct EpiResid {       // refinement: r = b - A x in FP64, r32 = float(r), <r,r
     }
 };
 
+struct EpiResid64 {     // r = b - A x in FP64, <r,r>
+    static constexpr int NDOT = 1;
+    const double* b;
+    double* r;
+    __device__ double load(long long i) const { return b[i]; }
+    __device__ void operator()(long long i, double y, double, double bi, double* acc) const {
+        const double ri = bi - y;
+        r[i] = ri;
+        acc[0] += ri * ri;
+    }
+};
+
 template <typename XT>
 struct EpiStore {
     static constexpr int NDOT = 1;

@@ -3,9 +3,9 @@
 WarmStartNet wraps the C-ABI network: a 3-level 3-D convolutional encoder-decoder
 (16/32/64 channels, 3x3x3, BF16 weights and activations, FP32 accumulation, hand-written
 matrix-core kernels in csrc/convnet.hip). Inputs per cell: phi^n / max|phi^n|,
-(phi^n - phi^{n-1}) / S, (rhs * dt / V - phi^n) / max|.| and optional static fields; output:
-the change of the step in units of S = max|phi^n - phi^{n-1}|, so the guess is
-phi^n + S * out. Training is online Adam on (inputs, solver solution) pairs that the plain
+(phi^n - phi^{n-1}) / max|.|, r0 dt / V / S with r0 = b - A phi^n the residual of the
+reference's guess and S = max|r0| dt / V, and optional static fields; output: the change of
+the step in units of S, so the guess is phi^n + S * out. Training is online Adam on (inputs, solver solution) pairs that the plain
 solver produces along its own trajectories; the solver then starts from the prediction and
 still iterates to its FP64 tolerance.
 """
@@ -113,13 +113,13 @@ class WarmStartNet:
     def _crop(crop):
         return None if crop is None else (ctypes.c_int32 * 6)(*crop)
 
-    def pack(self, grid, phi, phi_prev, rhs, rhs_scale, extras=(), crop=None):
+    def pack(self, grid, phi, phi_prev, resid, resid_scale, extras=(), crop=None):
         ne = len(extras)
         fptr = (ctypes.c_void_p * max(ne, 1))(*[_ptr(f) for f, _, _ in extras])
         sc = np.array([s for _, s, _ in extras] or [0.0], np.float64)
         of = np.array([o for _, _, o in extras] or [0.0], np.float64)
-        check(_sig().dare_convnet_pack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(phi_prev), _ptr(rhs),
-                                       float(rhs_scale), ne, fptr, sc.ctypes.data, of.ctypes.data,
+        check(_sig().dare_convnet_pack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(phi_prev), _ptr(resid),
+                                       float(resid_scale), ne, fptr, sc.ctypes.data, of.ctypes.data,
                                        self._crop(crop)), "dare_convnet_pack")
 
     def forward(self):
@@ -129,9 +129,9 @@ class WarmStartNet:
         check(_sig().dare_convnet_unpack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(x_out),
                                          self._crop(crop)), "dare_convnet_unpack")
 
-    def predict(self, grid, phi, phi_prev, rhs, rhs_scale, x_out, extras=()):
-        """x_out (solver rows) = the network's guess of the next step."""
-        self.pack(grid, phi, phi_prev, rhs, rhs_scale, extras)
+    def predict(self, grid, phi, phi_prev, resid, resid_scale, x_out, extras=()):
+        """x_out (solver rows) = the network's guess of the next step; resid = b - A phi^n."""
+        self.pack(grid, phi, phi_prev, resid, resid_scale, extras)
         self.forward()
         self.unpack(grid, phi, x_out)
 
@@ -147,9 +147,9 @@ class WarmStartNet:
     def step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         check(_sig().dare_conv_net_step(self.ctx.h, self.h, lr, beta1, beta2, eps), "dare_conv_net_step")
 
-    def train_step(self, grid, phi, phi_prev, rhs, rhs_scale, x_true, lr=1e-3, extras=(), readback=True,
+    def train_step(self, grid, phi, phi_prev, resid, resid_scale, x_true, lr=1e-3, extras=(), readback=True,
                    crop=None):
-        self.pack(grid, phi, phi_prev, rhs, rhs_scale, extras, crop)
+        self.pack(grid, phi, phi_prev, resid, resid_scale, extras, crop)
         self.forward()
         v = self.loss(grid, phi, x_true, readback, crop)
         self.backward()

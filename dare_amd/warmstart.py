@@ -58,7 +58,7 @@ def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_fina
                 if crop:
                     o = [int(crng.integers(0, n - crop + 1)) for _ in range(3)]
                     box = o + [crop] * 3
-                losses.append(net.train_step(W.grid, W.phi, W.phi_prev, W.rhs, W.rhs_scale, W.x, lr=cur_lr,
+                losses.append(net.train_step(W.grid, W.phi, W.phi_prev, W.r0, W.rhs_scale, W.x, lr=cur_lr,
                                              extras=W.extras, crop=box))
                 done += 1
             W.advance(W.x)
@@ -113,7 +113,7 @@ def evaluate(ctx, net, n=128, eval_seeds=range(5000, 5020), steps=200, tol=TOL, 
                         prev_int = W.phi_prev.view(n + 4, n + 4, n + 4)[2:-2, 2:-2, 2:-2].reshape(-1)
                         torch.sub(x_prev * 2.0, prev_int, out=xg)
                 else:
-                    net.predict(W.grid, W.phi, W.phi_prev, W.rhs, W.rhs_scale, xg, extras=W.extras)
+                    net.predict(W.grid, W.phi, W.phi_prev, W.r0, W.rhs_scale, xg, extras=W.extras)
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
                 r = bicgstab2(ctx, A, W.rhs, xg, tol=tol, tol_kind=1, max_it=1000)

@@ -198,20 +198,23 @@ class C5AdvDiff:
         self.phi_prev = self.phi.clone()
         self.rhs = torch.zeros(n ** 3, dtype=torch.float64, device=device)
         self.x = torch.zeros_like(self.rhs)
+        self.r0 = torch.zeros_like(self.rhs)
         self.A = A
         if self.A is None:
             self.A = Csr(ctx)
             build_structure(ctx, self.A, self.grid, self.terms)
-        self.rhs_scale = dt / (1.0 / n) ** 3      # rhs * dt / V: the V/dt phi^n term in field units
+        self.rhs_scale = dt / (1.0 / n) ** 3      # dt / V: residuals of the step in field units
         self.step_index = 0
         self.extras = []
         self.in_channels = 3
 
     def assemble(self):
-        """ghost fill, then values / rhs / x0 = phi^n of the next implicit step"""
-        from . import fill_values, ghost_fill
+        """ghost fill, then values / rhs / x0 = phi^n of the next implicit step, and the
+        residual r0 = rhs - A phi^n of that guess (the network's input)"""
+        from . import fill_values, ghost_fill, residual
         ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
         fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
+        residual(self.ctx, self.A, self.rhs, self.x, self.r0)
 
     def advance(self, x_solution):
         """phi^{n-1} <- phi^n, phi^n <- solution of the step"""
@@ -250,6 +253,7 @@ class VarDiffusion:
         self.phi_prev = self.phi.clone()
         self.rhs = torch.zeros(n ** 3, dtype=torch.float64, device=device)
         self.x = torch.zeros_like(self.rhs)
+        self.r0 = torch.zeros_like(self.rhs)
         self.A = A
         if self.A is None:
             self.A = Csr(ctx)
@@ -260,9 +264,10 @@ class VarDiffusion:
         self.in_channels = 4
 
     def assemble(self):
-        from . import fill_values, ghost_fill
+        from . import fill_values, ghost_fill, residual
         ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
         fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
+        residual(self.ctx, self.A, self.rhs, self.x, self.r0)
 
     def advance(self, x_solution):
         from . import copy_to_field

@@ -174,6 +174,8 @@ int dare_dot(dare_ctx* ctx, int64_t n, const double* a, const double* b, double*
 int dare_spmv_dot(dare_ctx* ctx, const dare_csr* csr, const double* x, double* y, const double* w,
                   double* out_host);
 int dare_axpby(dare_ctx* ctx, int64_t n, double a, const double* x, double b, double* y);
+/* r = b - A x (the operator the solver uses) and, if norm_host != NULL, ||r|| (BiCGStab2.inl:265-266). */
+int dare_residual(dare_ctx* ctx, const dare_csr* csr, const double* b, const double* x, double* r, double* norm_host);
 
 /* ---- solvers ------------------------------------------------------------------------- */
 /* FP64 BiCGStab(2). Replaces TrilinosSolver::Solve(BumbleBee, "BICGSTAB2", ...) ->
@@ -235,8 +237,12 @@ int dare_convnet_set_params(dare_ctx* ctx, dare_convnet* net, const float* host)
 int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_w, float* host_grad);
 /* crop: NULL for the whole grid, else {origin[3], size[3]} of a sub-box of internal cells
  * (training on crops; the network's zero padding then applies at the crop's faces). */
+/* Inputs: phi^n, phi^{n-1} (padded fields) and resid = b - A phi^n in solver row order, which
+ * resid_scale (dt / V for the implicit Euler term) maps to field units. Channels:
+ * phi^n / max|phi^n|, (phi^n - phi^{n-1}) / max|.|, resid * resid_scale / S with
+ * S = max|resid * resid_scale|, then the extras. The guess is phi^n + S * out. */
 int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
-                      const double* rhs, double rhs_scale, int32_t nextra, const double* const* extras,
+                      const double* resid, double resid_scale, int32_t nextra, const double* const* extras,
                       const double* extra_scale, const double* extra_offset, const int32_t* crop);
 int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net);
 int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess,

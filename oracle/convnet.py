@@ -172,27 +172,29 @@ def backward(dout, A, weights, exact=False):
 
 
 # ---------------------------------------------------------------- solver-side packing
-def scales(phi_int, phim_int, rhs_int, rhs_scale):
+def scales(phi_int, phim_int, resid_int, resid_scale):
     s0 = max(np.max(np.abs(phi_int)), 1e-300)
     s1 = max(np.max(np.abs(phi_int - phim_int)), 1e-300)
-    s2 = max(np.max(np.abs(rhs_int * rhs_scale - phi_int)), 1e-300)
+    s2 = max(np.max(np.abs(resid_int * resid_scale)), 1e-300)
     return s0, s1, s2
 
 
-def pack(phi_int, phim_int, rhs_int, rhs_scale, extras=(), in_ch=16):
-    """Input tensor [D, H, W, 16] as BF16 values (internal cells, arrays of shape [D, H, W])."""
-    s0, s1, s2 = scales(phi_int, phim_int, rhs_int, rhs_scale)
+def pack(phi_int, phim_int, resid_int, resid_scale, extras=(), in_ch=16):
+    """Input tensor [D, H, W, 16] as BF16 values (arrays of shape [D, H, W] over the box);
+    resid = b - A phi^n, resid_scale maps it to field units."""
+    s0, s1, s2 = scales(phi_int, phim_int, resid_int, resid_scale)
     x = np.zeros(phi_int.shape + (16,))
     x[..., 0] = bf16(phi_int * (1.0 / s0))
     x[..., 1] = bf16((phi_int - phim_int) * (1.0 / s1))
-    x[..., 2] = bf16((rhs_int * rhs_scale - phi_int) * (1.0 / s2))
+    x[..., 2] = bf16(resid_int * resid_scale * (1.0 / s2))
     for e, (f, scale, offset) in enumerate(extras):
         x[..., 3 + e] = bf16((f - offset) * scale)
     return x, (s0, s1, s2)
 
 
-def loss_and_dout(out, phi_int, x_true_int, s1, exact=False):
-    t = (x_true_int - phi_int) * (1.0 / s1)
+def loss_and_dout(out, phi_int, x_true_int, s2, exact=False):
+    """MSE of the output against the step in units of s2 (the output scale)."""
+    t = (x_true_int - phi_int) * (1.0 / s2)
     d = out - t
     n = d.size
     g = 2.0 * d / n

@@ -119,7 +119,7 @@ def test_backward_matches_oracle(ctx, rng):
     Wb = cn.split(cn.bf16(w))
     out_ref, A = cn.forward(x0, Wb, exact=False)
     out_gpu = d2h_f32(ctx, ptr["out"], N)
-    loss_ref, dout = cn.loss_and_dout(out_gpu, phi, x_true.cpu().numpy().reshape(N), scales[1])
+    loss_ref, dout = cn.loss_and_dout(out_gpu, phi, x_true.cpu().numpy().reshape(N), scales[2])
     assert abs(loss - loss_ref) <= 1e-9 * abs(loss_ref)
     dW = cn.backward(dout, A, Wb, exact=False)
     G = cn.split(grads.astype(np.float64))
@@ -140,12 +140,12 @@ def test_training_reduces_loss(ctx, rng):
     losses = [net.train_step(g, dev[0], dev[1], dev[2], 0.7, x_true, lr=3e-3) for _ in range(120)]
     assert np.all(np.isfinite(losses))
     assert np.mean(losses[-10:]) < 0.25 * np.mean(losses[:10]), (losses[:10], losses[-10:])
-    # the guess is phi^n + S * out
+    # the guess is phi^n + S * out, S = max|resid * resid_scale|
     xg = torch.empty_like(x_true)
     net.predict(g, dev[0], dev[1], dev[2], 0.7, xg)
     ptr, s = net.tensors()
     out = d2h_f32(ctx, ptr["out"], N)
-    assert np.allclose(xg.cpu().numpy().reshape(N), phi + s[1] * out, rtol=1e-14, atol=1e-14)
+    assert np.allclose(xg.cpu().numpy().reshape(N), phi + s[2] * out, rtol=1e-14, atol=1e-14)
     net.close()
 
 
