@@ -264,7 +264,7 @@ def target_leg(ctx, n, steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             if f.endswith("learned"):
-                net.predict(W.grid, W.phi, W.phi_prev, W.r0, W.rhs_scale, xg, extras=W.extras)
+                net.predict(W.grid, W.phi, W.phi_prev, W.xb, W.r0, W.rhs_scale, xg, extras=W.extras)
             else:
                 xg.copy_(x_prev)
             torch.cuda.synchronize()

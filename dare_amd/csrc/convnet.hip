@@ -19,8 +19,9 @@
 //   up,   L7 64->32, + e2, L8 32->32                               d2   1/2
 //   up,   L9 32->16, + e1, L10 16->16                              d1   full
 //   L11 16->16 (only output channel 0 is trained and read)        out  full, FP32
-// ReLU after every layer but L11. Prediction: phi^{n+1} = phi^n + S * out with S = max|r0| dt / V,
-// r0 = b - A phi^n the residual of the reference's guess (the size of one Richardson step).
+// ReLU after every layer but L11. Prediction: x = x_base + S * out, where x_base is a classical
+// guess (phi^n, or the extrapolation 2 phi^n - phi^{n-1}), r0 = b - A x_base its residual (an
+// input channel) and S = max|r0| dt / V the size of one Richardson step from it.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -404,8 +405,9 @@ __global__ __launch_bounds__(kNetThreads) void k_pack(FieldGeo F, const double* 
     }
 }
 
-// x[row] = phi^n + s2 * out[row] (the guess, solver row order)
-__global__ __launch_bounds__(kNetThreads) void k_unpack(FieldGeo F, const double* __restrict__ phi,
+// x[row] = base[row] + s2 * out (the guess, solver row order; base = the classical guess
+// whose residual the network saw)
+__global__ __launch_bounds__(kNetThreads) void k_unpack(FieldGeo F, const double* __restrict__ base,
                                                        const float* __restrict__ out, const double* __restrict__ s,
                                                        double* __restrict__ xg) {
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
@@ -414,12 +416,13 @@ __global__ __launch_bounds__(kNetThreads) void k_unpack(FieldGeo F, const double
         const int x = static_cast<int>(q % F.n2);
         const int y = static_cast<int>((q / F.n2) % F.n1);
         const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
-        xg[row_index(F, z, y, x)] = phi[pad_index(F, z, y, x)] + s2 * static_cast<double>(out[q]);
+        const long long r = row_index(F, z, y, x);
+        xg[r] = base[r] + s2 * static_cast<double>(out[q]);
     }
 }
 
-// loss = mean((out - t)^2), t = (x_true - phi^n) / s2; dout[v][0] = 2 (out - t) / n, other channels 0.
-__global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* __restrict__ phi,
+// loss = mean((out - t)^2), t = (x_true - base) / s2; dout[v][0] = 2 (out - t) / n, other channels 0.
+__global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* __restrict__ base,
                                                     const double* __restrict__ xt, const float* __restrict__ out,
                                                     const double* __restrict__ s, uint16_t* __restrict__ dout,
                                                     double* __restrict__ part) {
@@ -431,7 +434,8 @@ __global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* _
         const int x = static_cast<int>(q % F.n2);
         const int y = static_cast<int>((q / F.n2) % F.n1);
         const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
-        const double t = (xt[row_index(F, z, y, x)] - phi[pad_index(F, z, y, x)]) * inv;
+        const long long r = row_index(F, z, y, x);
+        const double t = (xt[r] - base[r]) * inv;
         const double d = static_cast<double>(out[q]) - t;
         acc += d * d;
         dout[q * kInChannels] = f2bf(static_cast<float>(2. * d / static_cast<double>(n)));
@@ -830,26 +834,26 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
     return DARE_OK;
 }
 
-// x_guess[row] = phi[row's cell] + S * out[row]
-int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess,
+// x_guess[row] = base[row] + S * out[row]
+int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* base, double* x_guess,
                         const int32_t* crop) {
-    DARE_CHECK_ARG(ctx && net && phi && x_guess, "NULL argument");
+    DARE_CHECK_ARG(ctx && net && base && x_guess, "NULL argument");
     int rc = check_grid(net, g, crop);
     if (rc) return rc;
     const FieldGeo F = field_geo(g, crop);
     DARE_CHECK_ARG(F.n0 == net->dims[0] && F.n1 == net->dims[1] && F.n2 == net->dims[2],
                    "box differs from the one of the last forward pass");
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
-    hipLaunchKernelGGL(k_unpack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, net->out, net->scales,
+    hipLaunchKernelGGL(k_unpack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, base, net->out, net->scales,
                        x_guess);
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }
 
 // MSE against the solver's solution x_true (row order) and the output gradient; loss to host.
-int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* x_true,
+int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* base, const double* x_true,
                       double* loss_host, const int32_t* crop) {
-    DARE_CHECK_ARG(ctx && net && phi && x_true, "NULL argument");
+    DARE_CHECK_ARG(ctx && net && base && x_true, "NULL argument");
     int rc = check_grid(net, g, crop);
     if (rc) return rc;
     if (!net->dout) {
@@ -861,7 +865,7 @@ int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
                    "box differs from the one of the last forward pass");
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
     const unsigned gs = std::min<unsigned>(ew_grid(n), 1024);
-    hipLaunchKernelGGL(k_mse, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, x_true, net->out, net->scales,
+    hipLaunchKernelGGL(k_mse, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, base, x_true, net->out, net->scales,
                        net->dout, net->part);
     hipLaunchKernelGGL(k_mse_finish, dim3(1), dim3(64), 0, ctx->stream, net->part, static_cast<int>(gs), n, net->loss);
     DARE_LAUNCH_CHECK();

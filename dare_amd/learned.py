@@ -2,10 +2,11 @@
 
 WarmStartNet wraps the C-ABI network: a 3-level 3-D convolutional encoder-decoder
 (16/32/64 channels, 3x3x3, BF16 weights and activations, FP32 accumulation, hand-written
-matrix-core kernels in csrc/convnet.hip). Inputs per cell: phi^n / max|phi^n|,
-(phi^n - phi^{n-1}) / max|.|, r0 dt / V / S with r0 = b - A phi^n the residual of the
-reference's guess and S = max|r0| dt / V, and optional static fields; output: the change of
-the step in units of S, so the guess is phi^n + S * out. Training is online Adam on (inputs, solver solution) pairs that the plain
+matrix-core kernels in csrc/convnet.hip). It corrects a classical guess x_base — by default
+the second-order extrapolation 2 phi^n - phi^{n-1}, computed in FP64. Inputs per cell:
+phi^n / max|phi^n|, (phi^n - phi^{n-1}) / max|.|, r0 dt / V / S with r0 = b - A x_base the
+residual of x_base and S = max|r0| dt / V, and optional static fields; the guess is
+x_base + S * out. Training is online Adam on (inputs, solver solution) pairs that the plain
 solver produces along its own trajectories; the solver then starts from the prediction and
 still iterates to its FP64 tolerance.
 """
@@ -125,19 +126,19 @@ class WarmStartNet:
     def forward(self):
         check(_sig().dare_conv_net_fwd(self.ctx.h, self.h), "dare_conv_net_fwd")
 
-    def unpack(self, grid, phi, x_out, crop=None):
-        check(_sig().dare_convnet_unpack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(x_out),
+    def unpack(self, grid, x_base, x_out, crop=None):
+        check(_sig().dare_convnet_unpack(self.ctx.h, self.h, ctypes.byref(grid), _ptr(x_base), _ptr(x_out),
                                          self._crop(crop)), "dare_convnet_unpack")
 
-    def predict(self, grid, phi, phi_prev, resid, resid_scale, x_out, extras=()):
-        """x_out (solver rows) = the network's guess of the next step; resid = b - A phi^n."""
+    def predict(self, grid, phi, phi_prev, x_base, resid, resid_scale, x_out, extras=()):
+        """x_out (solver rows) = x_base + the network's correction; resid = b - A x_base."""
         self.pack(grid, phi, phi_prev, resid, resid_scale, extras)
         self.forward()
-        self.unpack(grid, phi, x_out)
+        self.unpack(grid, x_base, x_out)
 
-    def loss(self, grid, phi, x_true, readback=True, crop=None):
+    def loss(self, grid, x_base, x_true, readback=True, crop=None):
         v = ctypes.c_double()
-        check(_sig().dare_convnet_loss(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(x_true),
+        check(_sig().dare_convnet_loss(self.ctx.h, self.h, ctypes.byref(grid), _ptr(x_base), _ptr(x_true),
                                        ctypes.byref(v) if readback else None, self._crop(crop)), "dare_convnet_loss")
         return v.value if readback else None
 
@@ -147,11 +148,11 @@ class WarmStartNet:
     def step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         check(_sig().dare_conv_net_step(self.ctx.h, self.h, lr, beta1, beta2, eps), "dare_conv_net_step")
 
-    def train_step(self, grid, phi, phi_prev, resid, resid_scale, x_true, lr=1e-3, extras=(), readback=True,
-                   crop=None):
+    def train_step(self, grid, phi, phi_prev, x_base, resid, resid_scale, x_true, lr=1e-3, extras=(),
+                   readback=True, crop=None):
         self.pack(grid, phi, phi_prev, resid, resid_scale, extras, crop)
         self.forward()
-        v = self.loss(grid, phi, x_true, readback, crop)
+        v = self.loss(grid, x_base, x_true, readback, crop)
         self.backward()
         self.step(lr)
         return v

@@ -146,6 +146,16 @@ class C4Batch:
             A.close()
 
 
+def _base_guess(W):
+    """xb = 2 phi^n - phi^{n-1} in solver rows (x holds phi^n after the fill) and r0 = rhs - A xb."""
+    from . import residual
+    g = W.grid.ghost
+    n = W.n
+    prev_int = W.phi_prev.view(n + 2 * g, n + 2 * g, n + 2 * g)[g:-g, g:-g, g:-g].reshape(-1)
+    _torch().sub(W.x * 2.0, prev_int, out=W.xb)
+    residual(W.ctx, W.A, W.rhs, W.xb, W.r0)
+
+
 def gaussian_sum(n, rng, count=8, ghost=2, device="cuda"):
     """sum of `count` Gaussians, centres U(unit cube), sigma U[0.05, 0.2], amplitude U[-1, 1],
     at the cell centres of the padded grid (ghosts included; callers overwrite them)."""
@@ -199,6 +209,7 @@ class C5AdvDiff:
         self.rhs = torch.zeros(n ** 3, dtype=torch.float64, device=device)
         self.x = torch.zeros_like(self.rhs)
         self.r0 = torch.zeros_like(self.rhs)
+        self.xb = torch.zeros_like(self.rhs)
         self.A = A
         if self.A is None:
             self.A = Csr(ctx)
@@ -209,12 +220,13 @@ class C5AdvDiff:
         self.in_channels = 3
 
     def assemble(self):
-        """ghost fill, then values / rhs / x0 = phi^n of the next implicit step, and the
-        residual r0 = rhs - A phi^n of that guess (the network's input)"""
-        from . import fill_values, ghost_fill, residual
+        """ghost fill, then values / rhs / x0 = phi^n of the next implicit step; also the
+        extrapolated guess xb = 2 phi^n - phi^{n-1} and its residual r0 = rhs - A xb (the
+        warm-start network's base guess and input)"""
+        from . import fill_values, ghost_fill
         ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
         fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
-        residual(self.ctx, self.A, self.rhs, self.x, self.r0)
+        _base_guess(self)
 
     def advance(self, x_solution):
         """phi^{n-1} <- phi^n, phi^n <- solution of the step"""
@@ -254,6 +266,7 @@ class VarDiffusion:
         self.rhs = torch.zeros(n ** 3, dtype=torch.float64, device=device)
         self.x = torch.zeros_like(self.rhs)
         self.r0 = torch.zeros_like(self.rhs)
+        self.xb = torch.zeros_like(self.rhs)
         self.A = A
         if self.A is None:
             self.A = Csr(ctx)
@@ -264,10 +277,10 @@ class VarDiffusion:
         self.in_channels = 4
 
     def assemble(self):
-        from . import fill_values, ghost_fill, residual
+        from . import fill_values, ghost_fill
         ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
         fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
-        residual(self.ctx, self.A, self.rhs, self.x, self.r0)
+        _base_guess(self)
 
     def advance(self, x_solution):
         from . import copy_to_field

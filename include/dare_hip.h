@@ -237,17 +237,19 @@ int dare_convnet_set_params(dare_ctx* ctx, dare_convnet* net, const float* host)
 int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_w, float* host_grad);
 /* crop: NULL for the whole grid, else {origin[3], size[3]} of a sub-box of internal cells
  * (training on crops; the network's zero padding then applies at the crop's faces). */
-/* Inputs: phi^n, phi^{n-1} (padded fields) and resid = b - A phi^n in solver row order, which
- * resid_scale (dt / V for the implicit Euler term) maps to field units. Channels:
- * phi^n / max|phi^n|, (phi^n - phi^{n-1}) / max|.|, resid * resid_scale / S with
- * S = max|resid * resid_scale|, then the extras. The guess is phi^n + S * out. */
+/* Inputs: phi^n, phi^{n-1} (padded fields) and resid = b - A x_base in solver row order, the
+ * residual of a classical guess x_base (phi^n, or 2 phi^n - phi^{n-1}), which resid_scale
+ * (dt / V for the implicit Euler term) maps to field units. Channels: phi^n / max|phi^n|,
+ * (phi^n - phi^{n-1}) / max|.|, resid * resid_scale / S with S = max|resid * resid_scale|,
+ * then the extras. The guess (unpack) is x_base + S * out; the loss compares out with
+ * (x_true - x_base) / S. */
 int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
                       const double* resid, double resid_scale, int32_t nextra, const double* const* extras,
                       const double* extra_scale, const double* extra_offset, const int32_t* crop);
 int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net);
-int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, double* x_guess,
+int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* x_base, double* x_guess,
                         const int32_t* crop);
-int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* x_true,
+int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* x_base, const double* x_true,
                       double* loss_host, const int32_t* crop);
 int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net);
 int dare_conv_net_step(dare_ctx* ctx, dare_convnet* net, float lr, float beta1, float beta2, float eps);

@@ -111,7 +111,8 @@ def test_backward_matches_oracle(ctx, rng):
     net, g, dev, (phi, phim, rhs, ext), w, _ = setup(ctx, rng)
     net.forward()
     x_true = torch.from_numpy(rng.normal(size=int(np.prod(N)))).cuda()
-    loss = net.loss(g, dev[0], x_true)
+    base = dev[0].view(*[m + 4 for m in N])[2:-2, 2:-2, 2:-2].contiguous().view(-1)
+    loss = net.loss(g, base, x_true)
     net.backward()
     grads = net.grads()
     ptr, scales = net.tensors()
@@ -137,12 +138,13 @@ def test_training_reduces_loss(ctx, rng):
     dev, (phi, phim, rhs, ext) = fields(rng, N)
     # a target that is a smooth function of the inputs
     x_true = dev[0].view(*[m + 4 for m in N])[2:-2, 2:-2, 2:-2].contiguous().view(-1) * 1.05
-    losses = [net.train_step(g, dev[0], dev[1], dev[2], 0.7, x_true, lr=3e-3) for _ in range(120)]
+    base = dev[0].view(*[m + 4 for m in N])[2:-2, 2:-2, 2:-2].contiguous().view(-1)
+    losses = [net.train_step(g, dev[0], dev[1], base, dev[2], 0.7, x_true, lr=3e-3) for _ in range(120)]
     assert np.all(np.isfinite(losses))
     assert np.mean(losses[-10:]) < 0.25 * np.mean(losses[:10]), (losses[:10], losses[-10:])
     # the guess is phi^n + S * out, S = max|resid * resid_scale|
     xg = torch.empty_like(x_true)
-    net.predict(g, dev[0], dev[1], dev[2], 0.7, xg)
+    net.predict(g, dev[0], dev[1], base, dev[2], 0.7, xg)
     ptr, s = net.tensors()
     out = d2h_f32(ctx, ptr["out"], N)
     assert np.allclose(xg.cpu().numpy().reshape(N), phi + s[2] * out, rtol=1e-14, atol=1e-14)

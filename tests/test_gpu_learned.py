@@ -44,7 +44,7 @@ def test_learned_guess_reaches_fp64_tolerance(ctx):
     xg = torch.empty_like(W.x)
     for k in range(4):
         W.assemble()
-        net.predict(W.grid, W.phi, W.phi_prev, W.r0, W.rhs_scale, xg)
+        net.predict(W.grid, W.phi, W.phi_prev, W.xb, W.r0, W.rhs_scale, xg)
         r = da.bicgstab2(ctx, W.A, W.rhs, xg, tol=1e-8, tol_kind=1, max_it=500)
         assert r["converged"] == 1
         y = torch.empty_like(xg)
