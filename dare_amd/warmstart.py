@@ -86,7 +86,7 @@ def evaluate(ctx, net, n=128, eval_seeds=range(5000, 5020), steps=200, tol=TOL, 
     import torch
     make = WORKLOADS[workload]
     modes = ["zero", "previous", "extrapolated", "learned"]
-    stats = {m: {"passes": [], "solve_s": [], "guess_s": [], "r0": []} for m in modes}
+    stats = {m: {"passes": [], "solve_s": [], "guess_s": [], "r0": [], "conv": []} for m in modes}
     A = None
     xg = None
     t_log = time.perf_counter()
@@ -123,6 +123,7 @@ def evaluate(ctx, net, n=128, eval_seeds=range(5000, 5020), steps=200, tol=TOL, 
                 t2 = time.perf_counter()
                 if k >= 1:                          # step 0 has no phi^{n-1}: all guesses coincide
                     stats[m]["passes"].append(r["iterations"])
+                    stats[m]["conv"].append(r["converged"])
                     stats[m]["solve_s"].append(t2 - t1)
                     stats[m]["guess_s"].append(guess_s)
                 if m == "previous":
@@ -139,7 +140,8 @@ def evaluate(ctx, net, n=128, eval_seeds=range(5000, 5020), steps=200, tol=TOL, 
                   "solve_ms_mean": 1e3 * float(np.mean(stats[m]["solve_s"])),
                   "guess_ms_mean": 1e3 * float(np.mean(stats[m]["guess_s"])),
                   "step_ms_mean": 1e3 * float(np.mean(stats[m]["solve_s"]) + np.mean(stats[m]["guess_s"])),
-                  "initial_rel_residual_geomean": float(np.exp(np.mean(np.log(np.maximum(stats[m]["r0"], 1e-300)))))}
+                  "initial_rel_residual_geomean": float(np.exp(np.mean(np.log(np.maximum(stats[m]["r0"], 1e-300))))),
+                  "all_converged": bool(all(stats[m]["conv"]))}
     base = out["previous"]
     for m in modes:
         out[m]["passes_saved_vs_previous"] = 1.0 - out[m]["passes_mean"] / base["passes_mean"]
