@@ -118,7 +118,7 @@ class SolverParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("tol_kind", ctypes.c_int32), ("max_it", ctypes.c_int32),
                 ("inner_tol", ctypes.c_double), ("max_refine", ctypes.c_int32),
                 ("inner_max_it", ctypes.c_int32), ("history", ctypes.POINTER(ctypes.c_double)),
-                ("history_cap", ctypes.c_int32)]
+                ("history_cap", ctypes.c_int32), ("precond", ctypes.c_int32)]
 
 
 class SolverResult(ctypes.Structure):
@@ -381,8 +381,9 @@ def copy_to_field(ctx, grid, x, phi):
     check(lib().dare_copy_to_field(ctx.h, ctypes.byref(grid), _ptr(x), _ptr(phi)), "dare_copy_to_field")
 
 
-def _params(tol, tol_kind, max_it, inner_tol, max_refine, inner_max_it, hist_cap):
+def _params(tol, tol_kind, max_it, inner_tol, max_refine, inner_max_it, hist_cap, precond=0):
     p = SolverParams()
+    p.precond = precond
     p.tol, p.tol_kind, p.max_it = tol, tol_kind, max_it
     p.inner_tol, p.max_refine, p.inner_max_it = inner_tol, max_refine, inner_max_it
     hist = (ctypes.c_double * max(hist_cap, 1))()
@@ -391,9 +392,10 @@ def _params(tol, tol_kind, max_it, inner_tol, max_refine, inner_max_it, hist_cap
     return p, hist
 
 
-def bicgstab2(ctx, csr, b, x, tol=1e-8, tol_kind=0, max_it=100, history=0):
-    """FP64 BiCGStab(2) (BiCGStab2.inl:167-582). x holds the initial guess on entry."""
-    p, hist = _params(tol, tol_kind, max_it, 0.0, 0, 0, history)
+def bicgstab2(ctx, csr, b, x, tol=1e-8, tol_kind=0, max_it=100, history=0, precond=0):
+    """FP64 BiCGStab(2) (BiCGStab2.inl:167-582). x holds the initial guess on entry.
+    precond=1: Jacobi right preconditioning."""
+    p, hist = _params(tol, tol_kind, max_it, 0.0, 0, 0, history, precond)
     r = SolverResult()
     check(lib().dare_bicgstab2_solve(ctx.h, csr.h, _ptr(b), _ptr(x), ctypes.byref(p), ctypes.byref(r)),
           "dare_bicgstab2_solve")
@@ -403,9 +405,10 @@ def bicgstab2(ctx, csr, b, x, tol=1e-8, tol_kind=0, max_it=100, history=0):
 
 
 def refine_mixed(ctx, csr, b, x, tol=1e-8, tol_kind=1, max_it=1000, inner_tol=1e-4, max_refine=30,
-                 inner_max_it=200, history=0):
-    """FP32 inner BiCGStab(2) inside FP64 iterative refinement."""
-    p, hist = _params(tol, tol_kind, max_it, inner_tol, max_refine, inner_max_it, history)
+                 inner_max_it=200, history=0, precond=0):
+    """FP32 inner BiCGStab(2) inside FP64 iterative refinement (precond=1: Jacobi on the inner
+    solves)."""
+    p, hist = _params(tol, tol_kind, max_it, inner_tol, max_refine, inner_max_it, history, precond)
     r = SolverResult()
     check(lib().dare_refine_solve_mixed(ctx.h, csr.h, _ptr(b), _ptr(x), ctypes.byref(p), ctypes.byref(r)),
           "dare_refine_solve_mixed")

@@ -107,6 +107,10 @@ struct dare_csr {
     double* dia = nullptr;
     float* dia32 = nullptr;
     int64_t cap_dia = 0, cap_dia32 = 0;
+    // Jacobi right preconditioning: 1 / diag(A) and the column-scaled FP64 view A diag(A)^-1
+    double* dinv = nullptr;
+    double* dia_pre = nullptr;
+    int64_t cap_dinv = 0, cap_dia_pre = 0;
     // structural description kept for lazy CSR value materialization from the diagonal view
     dare_grid grid_desc{};
     int bc_type[6] = {0, 0, 0, 0, 0, 0};

@@ -180,6 +180,95 @@ static int dot_dev(dare_ctx* ctx, long long n, const double* a, const double* b,
     return DARE_OK;
 }
 
+// ---- Jacobi right preconditioning (M = diag(A)). The method runs on A M^-1 with y = M x:
+// the diagonal view is column-scaled once per solve, x0 -> M x0 before and y -> M^-1 y after,
+// so every pass costs what an unpreconditioned pass costs and the residual b - A M^-1 y is the
+// true residual b - A x (the stopping test keeps its meaning).
+__global__ __launch_bounds__(kThreads) void k_dinv(long long n, const double* __restrict__ diag,
+                                                   double* __restrict__ dinv) {
+    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
+        dinv[i] = 1.0 / diag[i];
+}
+
+// out[s][r] = D[s][r] * dinv[column of slot s in row r]; a removed entry holds 0 and stays 0
+template <typename OT>
+__global__ __launch_bounds__(kThreads) void k_dia_colscale(DiaView V, const double* __restrict__ D,
+                                                           const double* __restrict__ dinv, OT* __restrict__ out) {
+    const long long nr = V.nrows;
+    const int N = V.N;
+    const long long sI = static_cast<long long>(V.n[1]) * V.n[2], sJ = V.n[2];
+    for (long long r = blockIdx.x * (long long)kThreads + threadIdx.x; r < nr; r += (long long)gridDim.x * kThreads) {
+        const long long node = r / N;
+        const int c = static_cast<int>(r % N);
+        const int k = static_cast<int>(node % V.n[2]);
+        const int j = static_cast<int>((node / V.n[2]) % V.n[1]);
+        const int i = static_cast<int>(node / sI);
+        const int pos[3] = {i, j, k};
+        const long long stride[3] = {sI, sJ, 1};
+        long long nb[7];
+        // slots 0 W, 1 S, 2 B, 3 C, 4 T, 5 N, 6 E: low neighbour of axis d is slot d, high is 6 - d
+        nb[3] = node;
+        for (int d = 0; d < 3; d++) {
+            const int n = V.n[d];
+            long long lo = node - stride[d], hi = node + stride[d];
+            if (pos[d] == 0) lo = V.per[d] ? node + static_cast<long long>(n - 1) * stride[d] : node;
+            if (pos[d] == n - 1) hi = V.per[d] ? node - static_cast<long long>(n - 1) * stride[d] : node;
+            nb[d] = lo;
+            nb[6 - d] = hi;
+        }
+        double dv[8], sc[8];
+#pragma unroll
+        for (int sl = 0; sl < 7; sl++) {
+            dv[sl] = D[sl * nr + r];
+            sc[sl] = dinv[nb[sl] * N + c];
+        }
+        dv[7] = (N == 2 && V.cross) ? D[7 * nr + r] : 0.;
+        sc[7] = (N == 2 && V.cross) ? dinv[node * N + (1 - c)] : 0.;
+#pragma unroll
+        for (int sl = 0; sl < 7; sl++) out[sl * nr + r] = static_cast<OT>(dv[sl] * sc[sl]);
+        if (N == 2 && V.cross) out[7 * nr + r] = static_cast<OT>(dv[7] * sc[7]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scale_by(long long n, double* __restrict__ x, const double* __restrict__ s,
+                                                       int invert) {
+    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
+        x[i] = invert ? x[i] * s[i] : x[i] / s[i];
+}
+
+__global__ __launch_bounds__(kThreads) void k_add_correction_pre(long long n, double* __restrict__ x,
+                                                                 const float* __restrict__ d,
+                                                                 const double* __restrict__ dinv) {
+    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
+        x[i] = x[i] + dinv[i] * static_cast<double>(d[i]);
+}
+
+template <typename T>
+int grow_buffer(T** p, int64_t* cap, int64_t need, const char* what) {
+    if (*cap >= need && *p) return DARE_OK;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, sizeof(T) * (need > 0 ? need : 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(std::string(what) + " allocation failed");
+        return DARE_ERR_OOM;
+    }
+    *cap = need;
+    return DARE_OK;
+}
+
+// 1 / diag(A) into A->dinv (slot C of the diagonal view)
+int jacobi_setup(dare_ctx* ctx, dare_csr* A) {
+    DARE_CHECK_ARG(A->has_dia, "Jacobi preconditioning needs a matrix with a diagonal view (dare_csr_build_structure)");
+    const long long n = A->nrows;
+    int rc = grow_buffer(&A->dinv, &A->cap_dinv, n, "Jacobi diagonal");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, A->dia + 3 * n, A->dinv);
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
 static SolveOp<double> op64(const dare_csr* A, bool allow_dia) {
     SolveOp<double> op;
     op.n = A->nrows;
@@ -277,7 +366,23 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     if (rc) return rc;
     const double bnorm = std::sqrt(bnorm2);
     const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
-    const SolveOp<double> op = op64(A, true);
+    SolveOp<double> op = op64(A, true);
+    const bool pre = p->precond == 1;
+    DARE_CHECK_ARG(p->precond == 0 || p->precond == 1, "precond must be 0 (none) or 1 (Jacobi)");
+    if (pre) {
+        DARE_CHECK_ARG(op.dia, "Jacobi preconditioning needs a matrix with a diagonal view");
+        dare_csr* Am = const_cast<dare_csr*>(A);
+        rc = jacobi_setup(ctx, Am);
+        if (rc) return rc;
+        const int slots = A->dia_view.cross ? 8 : 7;
+        rc = grow_buffer(&Am->dia_pre, &Am->cap_dia_pre, static_cast<int64_t>(slots) * n, "preconditioned view");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_dia_colscale<double>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, A->dia_view,
+                           A->dia, A->dinv, Am->dia_pre);
+        hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 0);  // y0 = M x0
+        DARE_LAUNCH_CHECK();
+        op.d = make_dia<double>(A->dia_view, Am->dia_pre);
+    }
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
         if (rc) return rc;
@@ -288,6 +393,10 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     rc = bicgstab2_run<double, double>(ctx, op, b, x, tol_abs, p->max_it, p->history, p->history_cap, &hl, &iters,
                                        &rn, &status, &conv, static_cast<double*>(W.buf), 2);
     if (rc) return rc;
+    if (pre) {
+        hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 1);  // x = M^-1 y
+        DARE_LAUNCH_CHECK();
+    }
     res->converged = conv ? 1 : 0;
     res->iterations = iters;
     res->status = status ? DARE_ERR_BREAKDOWN : 0;
@@ -307,6 +416,9 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     // FP32 copy of the operator (values change every step: converted on every call)
     SolveOp<float> op32;
     op32.n = n;
+    DARE_CHECK_ARG(p->precond == 0 || p->precond == 1, "precond must be 0 (none) or 1 (Jacobi)");
+    const bool pre = p->precond == 1;
+    DARE_CHECK_ARG(!pre || A->has_dia, "Jacobi preconditioning needs a matrix with a diagonal view");
     if (A->has_dia && kXcd * make_dia<double>(A->dia_view, A->dia).per_xcd <= kReduceBlocks) {
         const int slots = A->dia_view.cross ? 8 : 7;
         const int64_t need = static_cast<int64_t>(slots) * n;
@@ -321,11 +433,22 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
             }
             A->cap_dia32 = need;
         }
-        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(need)), dim3(kThreads), 0, ctx->stream, (long long)need, A->dia,
-                           A->dia32);
+        if (pre) {       // FP32 copy of A M^-1 (the inner solves run right-preconditioned)
+            int prc = jacobi_setup(ctx, A);
+            if (prc) return prc;
+            hipLaunchKernelGGL((k_dia_colscale<float>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream,
+                               A->dia_view, A->dia, A->dinv, A->dia32);
+        } else {
+            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(need)), dim3(kThreads), 0, ctx->stream, (long long)need,
+                               A->dia, A->dia32);
+        }
         op32.d = make_dia<float>(A->dia_view, A->dia32);
         op32.dia = kXcd * op32.d.per_xcd <= kReduceBlocks;
     } else {
+        if (pre) {
+            set_error("Jacobi preconditioning needs the diagonal-view operator (grid too large for it)");
+            return DARE_ERR_UNSUPPORTED;
+        }
         int erc = csr_ensure_values(A, ctx->stream);
         if (erc) return erc;
         if (A->cap_val32 < A->nnz) {
@@ -390,7 +513,10 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
                                          &irn, &ist, &iconv, work, 2);
         if (rc) return rc;
         total_it += it;
-        hipLaunchKernelGGL(k_add_correction, dim3(gr), dim3(kThreads), 0, ctx->stream, n, x, d32);
+        if (pre)    // the inner solution is y = M d
+            hipLaunchKernelGGL(k_add_correction_pre, dim3(gr), dim3(kThreads), 0, ctx->stream, n, x, d32, A->dinv);
+        else
+            hipLaunchKernelGGL(k_add_correction, dim3(gr), dim3(kThreads), 0, ctx->stream, n, x, d32);
         DARE_LAUNCH_CHECK();
         refine++;
         if (ist && it <= 1) {

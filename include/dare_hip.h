@@ -93,6 +93,9 @@ typedef struct {
     int32_t inner_max_it;    /* mixed precision: passes per inner solve */
     double* history;         /* optional host array, residual norm after every pass */
     int32_t history_cap;
+    int32_t precond;         /* 0: none (the reference default, M == null); 1: Jacobi, the right
+                              * preconditioner M = diag(A) (BiCGStab2.inl:353-365 applies M^-1 on
+                              * the right); matrices with a diagonal view only */
 } dare_solver_params;
 
 typedef struct {
