@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the target / batch / learned legs")
     ap.add_argument("--target-steps", type=int, default=4)
     ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--only", choices=["target", "precision", "batch", "learned"], default=None,
+                    help="run one extra leg alone and print its JSON (profiling)")
     return ap.parse_args()
 
 
@@ -59,6 +61,12 @@ def main():
 
     torch.cuda.set_device(0)
     ctx = da.Context(0)
+    if args.only:
+        leg = {"target": lambda: target_leg(ctx, args.n, args.target_steps), "precision": lambda: precision_leg(ctx),
+               "batch": lambda: batch_leg(ctx, args.batch_systems), "learned": lambda: learned_leg(ctx)}[args.only]
+        print(json.dumps({args.only: leg()}), flush=True)
+        ctx.close()
+        return
     W = C2Rotation(args.n)
     g = W.grid
     log(f"[bench] grid {args.n}^3, dt {W.dt:.3e}, building structure")
@@ -178,6 +186,8 @@ def main():
         torch.cuda.empty_cache()
         extra["target"] = target_leg(ctx, args.n, args.target_steps)
         torch.cuda.empty_cache()
+        extra["precision"] = precision_leg(ctx)
+        torch.cuda.empty_cache()
         extra["batch"] = batch_leg(ctx, args.batch_systems)
         torch.cuda.empty_cache()
         extra["learned"] = learned_leg(ctx)
@@ -201,7 +211,7 @@ def main():
                         f"BiCGStab2 to rel residual 1e-8",
             "grid": [args.n] * 3, "rows": nrows, "nnz": nnz, "form": best,
         },
-        "forms": {f: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        "forms": {f: {k: (round(v, 4) if isinstance(v, float) and k.endswith("ms") else v) for k, v in r.items()}
                   for f, r in results.items()},
         "speedup_vs_fp64": round(results["fp64"]["solve_ms"] / results[best]["solve_ms"], 3),
         "roofline": {
@@ -310,6 +320,52 @@ def target_leg(ctx, n, steps):
     if net:
         net.close()
     W.A.close()
+    return out
+
+
+def precision_leg(ctx, n=384):
+    """Precision form at 384^3: one implicit variable-coefficient diffusion step (ln D ~
+    N(ln 1e-3, 1), harmonic face values, source U[0,1), Dirichlet 0, mean D dt/dx^2 = 1, from
+    phi = 0). FP32 inner BiCGStab2 (inner tolerance 1e-4) inside FP64 refinement to true
+    relative residual 1e-10, against the plain FP64 solve to the same tolerance; an FP64 solve
+    to 1e-13 is the accuracy reference for both."""
+    import torch
+    import dare_amd as da
+    from dare_amd.workloads import c3_lognormal
+    log(f"[bench] precision: {n}^3 variable-coefficient diffusion")
+    D, src, dt = c3_lognormal(n, seed=3, mean_diff_number=1.0)
+    g = da.Grid.make([n] * 3, size=[1.0, 1.0, 1.0], ghost=2)
+    t = da.Terms.make(dt=dt, diff_cell=D, source=src, bc_type=[da.BC_DIRICHLET] * 6, bc_value=[0.0] * 6)
+    A = da.Csr(ctx)
+    da.build_structure(ctx, A, g, t)
+    phi = torch.zeros((n + 4) ** 3, dtype=torch.float64, device="cuda")
+    rhs = torch.zeros(n ** 3, dtype=torch.float64, device="cuda")
+    x0 = torch.zeros_like(rhs)
+    da.fill_values(ctx, A, g, t, phi, rhs, x0)
+    ctx.synchronize()
+    xr = x0.clone()
+    da.bicgstab2(ctx, A, rhs, xr, tol=1e-13, tol_kind=1, max_it=5000, precond=1)
+    out = {"workload": f"{n}^3 variable-coefficient diffusion step, to true rel residual 1e-10"}
+    for name, mixed, pre in (("fp64", False, 0), ("mixed", True, 0), ("mixed+jacobi", True, 1)):
+        ts = []
+        for rep in range(2):
+            x = x0.clone()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            if mixed:
+                r = da.refine_mixed(ctx, A, rhs, x, tol=1e-10, tol_kind=1, inner_tol=1e-4, max_refine=30,
+                                    inner_max_it=5000, max_it=10000, precond=pre)
+            else:
+                r = da.bicgstab2(ctx, A, rhs, x, tol=1e-10, tol_kind=1, max_it=5000, precond=pre)
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        err = float(torch.linalg.norm(x - xr) / torch.linalg.norm(xr))
+        out[name] = {"ms": round(1e3 * min(ts), 2), "passes": r["iterations"], "refinements": r["refinements"],
+                     "converged": r["converged"], "true_rel_residual": _true_rel(ctx, A, rhs, x),
+                     "rel_error_vs_fp64_reference": err}
+    for name in ("mixed", "mixed+jacobi"):
+        out[name]["speedup_vs_fp64"] = round(out["fp64"]["ms"] / out[name]["ms"], 3)
+    A.close()
     return out
 
 
