@@ -399,6 +399,7 @@ __global__ __launch_bounds__(256) void k_fill_values(Geo G, Terms T, long long n
 // Diagonal-view fill (3-D): one lane per cell, lanes along k, no column sort. Writes the slot
 // values (coalesced, slot-major), rhs and x0. The CSR values are materialized from the slots
 // only when a CSR consumer asks for them (k_materialize_csr, same sort and merge).
+template <bool NT>
 __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __restrict__ dia, int slots,
                                                   double* __restrict__ rhs, double* __restrict__ x0) {
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -413,9 +414,9 @@ __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __rest
         double slotv[kDiaSlots];
         double r = 0., x = 0.;
         row_entries<kRowValues, false>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
-        for (int sl = 0; sl < slots; sl++) dia[sl * nrows + row] = slotv[sl];
-        rhs[row] = r;
-        if (x0) x0[row] = x;
+        for (int sl = 0; sl < slots; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
+        put<NT>(rhs + row, r);
+        if (x0) put<NT>(x0 + row, x);
     }
 }
 
@@ -665,7 +666,11 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         DARE_CHECK_ARG(T.bc_type[f] == A->bc_type[f], "boundary edits differ from the structure's");
     if (A->has_dia) {
         const dim3 grid(ceil_div(G.n[2], 64), ceil_div(G.n[1], 4), G.n[0]);
-        hipLaunchKernelGGL(k_fill_dia, grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->cross ? 8 : 7, rhs, x0);
+        if (A->nrows >= kStreamRows)
+            hipLaunchKernelGGL(k_fill_dia<true>, grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->cross ? 8 : 7, rhs, x0);
+        else
+            hipLaunchKernelGGL(k_fill_dia<false>, grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->cross ? 8 : 7, rhs,
+                               x0);
         DARE_LAUNCH_CHECK();
         A->vals_stale = 1;
         return DARE_OK;

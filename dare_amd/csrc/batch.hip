@@ -129,6 +129,7 @@ __global__ __launch_bounds__(kThreads) void kb_apply(const BSys* __restrict__ sy
 
 struct EpiTrueRes {     // ||b - A x||^2, nothing stored
     static constexpr int NDOT = 1;
+    static constexpr bool kNT = false;
     const double* b;
     __device__ double load(long long i) const { return b[i]; }
     __device__ void operator()(long long, double y, double, double bi, double* acc) const {
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void kb_vec(const BSys* __restrict__ sys,
 #pragma unroll
             for (int q = 0; q < kU; q++) {
                 const long long i = base + q * kThreads + threadIdx.x;
-                if (i < end) dst[i] = static_cast<XT>(sv[q] - beta * dv[q]);          // (5) / (12)
+                if (i < end) put<false>(dst + i, static_cast<XT>(sv[q] - beta * dv[q]));   // (5) / (12)
             }
         } else if (OP == V_RX) {
             double rv[kU], vv[kU], xv[kU], uv[kU];
@@ -212,8 +213,8 @@ __global__ __launch_bounds__(kThreads) void kb_vec(const BSys* __restrict__ sys,
             for (int q = 0; q < kU; q++) {
                 const long long i = base + q * kThreads + threadIdx.x;
                 if (i < end) {
-                    a.r[i] = static_cast<XT>(rv[q] - alpha * vv[q]);                  // (8)
-                    a.x[i] = static_cast<XT>(1. * xv[q] + alpha * uv[q]);             // (10)
+                    put<false>(a.r + i, static_cast<XT>(rv[q] - alpha * vv[q]));       // (8)
+                    put<false>(a.x + i, static_cast<XT>(1. * xv[q] + alpha * uv[q]));  // (10)
                 }
             }
         } else if (OP == V_URS) {
@@ -232,9 +233,9 @@ __global__ __launch_bounds__(kThreads) void kb_vec(const BSys* __restrict__ sys,
             for (int q = 0; q < kU; q++) {
                 const long long i = base + q * kThreads + threadIdx.x;
                 if (i < end) {
-                    a.u[i] = static_cast<XT>(rv[q] - beta * uv[q]);                   // (15)
-                    a.r[i] = static_cast<XT>(rv[q] - alpha * vv[q]);                  // (16)
-                    a.s[i] = static_cast<XT>(sv[q] - alpha * wv[q]);                  // (17)
+                    put<false>(a.u + i, static_cast<XT>(rv[q] - beta * uv[q]));        // (15)
+                    put<false>(a.r + i, static_cast<XT>(rv[q] - alpha * vv[q]));       // (16)
+                    put<false>(a.s + i, static_cast<XT>(sv[q] - alpha * wv[q]));       // (17)
                 }
             }
         } else if (OP == V_END) {
@@ -258,10 +259,10 @@ __global__ __launch_bounds__(kThreads) void kb_vec(const BSys* __restrict__ sys,
                 if (i < end) {
                     double xi = 1. * xv[q] + o1 * rv[q] + o2 * sv[q];                 // (24)
                     xi = 1. * xi + alpha * uv[q];
-                    a.x[i] = static_cast<XT>(xi);
+                    put<false>(a.x + i, static_cast<XT>(xi));
                     const XT rn = static_cast<XT>(1. * rv[q] + (-o1) * sv[q] + (-o2) * tv[q]);   // (25)
-                    a.r[i] = rn;
-                    a.u[i] = static_cast<XT>(1. * uv[q] + (-o1) * vv[q] + (-o2) * wv[q]);
+                    put<false>(a.r + i, rn);
+                    put<false>(a.u + i, static_cast<XT>(1. * uv[q] + (-o1) * vv[q] + (-o2) * wv[q]));
                     const double rd = static_cast<double>(rn);
                     acc[0] += rd * rd;
                     acc[kDots - 1] += hv[q] * rd;
@@ -756,7 +757,7 @@ int solve_impl(dare_batch* B, const BParams& P, int passes, dare_solver_result* 
         launch_vec<float, V_F32>(B, a);                   // FP32 copy of every system's values
         const int max_rounds_inner = (P.inner_max_it + 2) / std::max(passes, 1) + 2;
         for (int ref = 0; ref <= P.max_refine; ref++) {
-            launch_op<NC, double, double>(B, B->x, EpiResid{B->b, r32}, 1, 0);     // r = b - A x (FP64)
+            launch_op<NC, double, double>(B, B->x, EpiResid<>{B->b, r32}, 1, 0);     // r = b - A x (FP64)
             launch_fin<1>(B, 0, BT_OUTER, 1, 0, P);
             DARE_LAUNCH_CHECK();
             int outer = 0;

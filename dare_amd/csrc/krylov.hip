@@ -88,11 +88,12 @@ static int read_state(dare_ctx* ctx, KState* dev, KState* host) {
 
 static_assert(sizeof(KState) <= 64 * sizeof(double), "pinned readback block too small");
 
-// Host-driven BiCGStab2 over device-resident scalars; XT is the type of every vector.
-template <typename VT, typename XT>
-int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
-                  double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                  bool* converged, XT* work, int min_passes) {
+// Host-driven BiCGStab2 over device-resident scalars; XT is the type of every vector, NT the
+// choice of non-temporal stores (vectors larger than the caches).
+template <typename VT, typename XT, bool NT>
+int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
+                     double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
+                     bool* converged, XT* work, int min_passes) {
     const long long n = A.n;
     XT* r = work;
     XT* rh = r + n;
@@ -107,7 +108,7 @@ int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, doubl
     ReduceWs ws = ctx->ws;
     *hist_len = 0;
     *status = 0;
-    A.launch(sm, x, EpiInit<XT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);     // (0)-(2)
+    A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);     // (0)-(2)
     DARE_LAUNCH_CHECK();
     KState h;
     int rc = read_state(ctx, st, &h);
@@ -127,15 +128,15 @@ int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, doubl
             *status = h.next_status;
             break;
         }
-        hipLaunchKernelGGL((k_upd_u<XT>), dim3(gr), dim3(kThreads), 0, sm, n, r, u, st);
-        A.launch(sm, u, EpiDot<XT>{v, rh}, ws, st, TAIL_GAMMA, 1);
-        hipLaunchKernelGGL((k_upd_rx<XT>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, x, u, st);
-        A.launch(sm, r, EpiDot<XT>{s, rh}, ws, st, TAIL_RHO, 1);
-        hipLaunchKernelGGL((k_upd_v<XT>), dim3(gr), dim3(kThreads), 0, sm, n, s, v, st);
-        A.launch(sm, v, EpiDot<XT>{w, rh}, ws, st, TAIL_GAMMA, 1);
-        hipLaunchKernelGGL((k_upd_urs<XT>), dim3(gr), dim3(kThreads), 0, sm, n, u, r, v, s, w, st);
-        A.launch(sm, s, EpiGcr<XT>{t, r}, ws, st, TAIL_GCR, 1);
-        hipLaunchKernelGGL((k_end<XT>), dim3(gr), dim3(kThreads), 0, sm, n, x, r, s, t, u, v, w, rh, ws, st);
+        hipLaunchKernelGGL((k_upd_u<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, u, st);
+        A.launch(sm, u, EpiDot<XT, NT>{v, rh}, ws, st, TAIL_GAMMA, 1);
+        hipLaunchKernelGGL((k_upd_rx<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, x, u, st);
+        A.launch(sm, r, EpiDot<XT, NT>{s, rh}, ws, st, TAIL_RHO, 1);
+        hipLaunchKernelGGL((k_upd_v<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, s, v, st);
+        A.launch(sm, v, EpiDot<XT, NT>{w, rh}, ws, st, TAIL_GAMMA, 1);
+        hipLaunchKernelGGL((k_upd_urs<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, u, r, v, s, w, st);
+        A.launch(sm, s, EpiGcr<XT, NT>{t, r}, ws, st, TAIL_GCR, 1);
+        hipLaunchKernelGGL((k_end<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, x, r, s, t, u, v, w, rh, ws, st);
         launch_finish<2>(sm, ws, gr, st, TAIL_END, 1);
         DARE_LAUNCH_CHECK();
         rc = read_state(ctx, st, &h);
@@ -144,7 +145,7 @@ int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, doubl
             *status = h.status;
             if (h.status == 21) {
                 // tau == 0: true residual check (BiCGStab2.inl:514-536)
-                A.launch(sm, x, EpiInit<XT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);
+                A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);
                 rc = read_state(ctx, st, &h);
                 if (rc) return rc;
                 conv = h.rnorm;
@@ -164,6 +165,17 @@ int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, doubl
     *rnorm = conv;
     *converged = done_ok || conv <= tol_abs;
     return DARE_OK;
+}
+
+template <typename VT, typename XT>
+int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
+                  double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
+                  bool* converged, XT* work, int min_passes) {
+    if (A.n >= kStreamRows)
+        return bicgstab2_run_nt<VT, XT, true>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
+                                              status, converged, work, min_passes);
+    return bicgstab2_run_nt<VT, XT, false>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
+                                           status, converged, work, min_passes);
 }
 
 static int dot_dev(dare_ctx* ctx, long long n, const double* a, const double* b, double* out) {
@@ -294,7 +306,8 @@ int dare_spmv(dare_ctx* ctx, const dare_csr* A, const double* x, double* y) {
     if (rc) return rc;
     // the Tpetra apply replacement: CSR, per-row sum in sorted column order
     SolveOp<double> op = op64(A, false);
-    op.launch(ctx->stream, x, EpiStore<double>{y}, ctx->ws, nullptr, -1, 0);
+    if (A->nrows >= kStreamRows) op.launch(ctx->stream, x, EpiStore<double, true>{y}, ctx->ws, nullptr, -1, 0);
+    else op.launch(ctx->stream, x, EpiStore<double, false>{y}, ctx->ws, nullptr, -1, 0);
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }
@@ -310,7 +323,8 @@ int dare_residual(dare_ctx* ctx, const dare_csr* A, const double* b, const doubl
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(64), 0, ctx->stream, W.st);
-    op.launch(ctx->stream, x, EpiResid64{b, r}, ctx->ws, W.st, TAIL_STORE, 0);
+    if (A->nrows >= kStreamRows) op.launch(ctx->stream, x, EpiResid64<true>{b, r}, ctx->ws, W.st, TAIL_STORE, 0);
+    else op.launch(ctx->stream, x, EpiResid64<false>{b, r}, ctx->ws, W.st, TAIL_STORE, 0);
     DARE_LAUNCH_CHECK();
     if (norm_host) {
         KState h;
@@ -330,7 +344,8 @@ int dare_spmv_dot(dare_ctx* ctx, const dare_csr* A, const double* x, double* y, 
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(64), 0, ctx->stream, W.st);
     // the operator the solver uses (diagonal view when the matrix has one)
     SolveOp<double> op = op64(A, true);
-    op.launch(ctx->stream, x, EpiDot<double>{y, w}, ctx->ws, W.st, TAIL_STORE, 1);
+    if (A->nrows >= kStreamRows) op.launch(ctx->stream, x, EpiDot<double, true>{y, w}, ctx->ws, W.st, TAIL_STORE, 1);
+    else op.launch(ctx->stream, x, EpiDot<double, false>{y, w}, ctx->ws, W.st, TAIL_STORE, 1);
     DARE_LAUNCH_CHECK();
     if (out_host) {
         KState h;
@@ -493,7 +508,8 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     const int inner_max = p->inner_max_it > 0 ? p->inner_max_it : p->max_it;
     const double inner_tol = p->inner_tol > 0. ? p->inner_tol : 1e-4;
     while (true) {
-        op.launch(ctx->stream, x, EpiResid{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);      // r = b - A x (FP64)
+        if (n >= kStreamRows) op.launch(ctx->stream, x, EpiResid<true>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);
+        else op.launch(ctx->stream, x, EpiResid<false>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);   // r = b - A x
         DARE_LAUNCH_CHECK();
         KState h;
         rc = read_state(ctx, W.st, &h);

@@ -233,47 +233,50 @@ DiaOp<VT> make_dia(const DiaView& V, const VT* D) {
 // load(i) fetches the row's vector operands (issued together with the matrix loads);
 // operator() gets y = (A x)_i, xc = x_i and the loaded operands, stores results and adds to
 // the per-thread dot accumulators.
-template <typename XT>
+template <typename XT, bool NT = false>
 struct EpiInit {        // r = b - A x; rh = r; u = v = w = 0; <r,r>, <rh,r>   (BiCGStab2.inl:265-273)
     static constexpr int NDOT = 2;
+    static constexpr bool kNT = NT;
     const XT* b;
     XT *r, *rh, *u, *v, *w;
     __device__ double load(long long i) const { return static_cast<double>(b[i]); }
     __device__ void operator()(long long i, double y, double, double bi, double* acc) const {
         const XT ri = static_cast<XT>(bi - y);
-        r[i] = ri;
-        rh[i] = ri;
-        u[i] = XT(0);
-        v[i] = XT(0);
-        w[i] = XT(0);
+        put<NT>(r + i, ri);
+        put<NT>(rh + i, ri);
+        put<NT>(u + i, XT(0));
+        put<NT>(v + i, XT(0));
+        put<NT>(w + i, XT(0));
         const double rd = static_cast<double>(ri);
         acc[0] += rd * rd;
         acc[1] += rd * rd;
     }
 };
 
-template <typename XT>
+template <typename XT, bool NT = false>
 struct EpiDot {         // out = A p; <out, w>                   (6)-(7), (9)+(11), (13)-(14)
     static constexpr int NDOT = 1;
+    static constexpr bool kNT = NT;
     XT* out;
     const XT* w;
     __device__ double load(long long i) const { return static_cast<double>(w[i]); }
     __device__ void operator()(long long i, double y, double, double wi, double* acc) const {
         const XT yi = static_cast<XT>(y);
-        out[i] = yi;
+        put<NT>(out + i, yi);
         acc[0] += static_cast<double>(yi) * wi;
     }
 };
 
-template <typename XT>
+template <typename XT, bool NT = false>
 struct EpiGcr {         // t = A s; <r,s>, <s,s>, <s,t>, <t,t>, <r,t>   (18)-(20)
     static constexpr int NDOT = 5;
+    static constexpr bool kNT = NT;
     XT* t;
     const XT* r;
     __device__ double load(long long i) const { return static_cast<double>(r[i]); }
     __device__ void operator()(long long i, double y, double sc, double rd, double* acc) const {
         const XT ti = static_cast<XT>(y);
-        t[i] = ti;
+        put<NT>(t + i, ti);
         const double td = static_cast<double>(ti);
         acc[0] += rd * sc;
         acc[1] += sc * sc;
@@ -283,36 +286,43 @@ struct EpiGcr {         // t = A s; <r,s>, <s,s>, <s,t>, <t,t>, <r,t>   (18)-(20
     }
 };
 
+template <bool NT = false>
 struct EpiResid {       // refinement: r = b - A x in FP64, r32 = float(r), <r,r>
     static constexpr int NDOT = 1;
+    static constexpr bool kNT = NT;
     const double* b;
     float* r32;
     __device__ double load(long long i) const { return b[i]; }
     __device__ void operator()(long long i, double y, double, double bi, double* acc) const {
         const double ri = bi - y;
-        r32[i] = static_cast<float>(ri);
+        put<NT>(r32 + i, static_cast<float>(ri));
         acc[0] += ri * ri;
     }
 };
 
+template <bool NT = false>
 struct EpiResid64 {     // r = b - A x in FP64, <r,r>
     static constexpr int NDOT = 1;
+    static constexpr bool kNT = NT;
     const double* b;
     double* r;
     __device__ double load(long long i) const { return b[i]; }
     __device__ void operator()(long long i, double y, double, double bi, double* acc) const {
         const double ri = bi - y;
-        r[i] = ri;
+        put<NT>(r + i, ri);
         acc[0] += ri * ri;
     }
 };
 
-template <typename XT>
+template <typename XT, bool NT = false>
 struct EpiStore {
     static constexpr int NDOT = 1;
+    static constexpr bool kNT = NT;
     XT* out;
     __device__ double load(long long) const { return 0.; }
-    __device__ void operator()(long long i, double y, double, double, double*) const { out[i] = static_cast<XT>(y); }
+    __device__ void operator()(long long i, double y, double, double, double*) const {
+        put<NT>(out + i, static_cast<XT>(y));
+    }
 };
 
 // ------------------------------------------------------------------------------ traversals
@@ -398,8 +408,12 @@ __device__ inline void dia_column(const DiaView& V, const VT* __restrict__ D, co
             xb[c] = static_cast<double>(x[(node + dB) * NC + c]);
             xt[c] = static_cast<double>(x[(node + dT) * NC + c]);
 #pragma unroll
-            for (int sl = 0; sl < 7; sl++) dv[c][sl] = static_cast<double>(D[sl * nr + row]);
-            dv[c][7] = (NC == 2 && V.cross) ? static_cast<double>(D[7 * nr + row]) : 0.;
+            for (int sl = 0; sl < 7; sl++) {
+                if constexpr (Epi::kNT) dv[c][sl] = static_cast<double>(ld_nt(D + sl * nr + row));
+                else dv[c][sl] = static_cast<double>(D[sl * nr + row]);
+            }
+            if constexpr (Epi::kNT) dv[c][7] = (NC == 2 && V.cross) ? static_cast<double>(ld_nt(D + 7 * nr + row)) : 0.;
+            else dv[c][7] = (NC == 2 && V.cross) ? static_cast<double>(D[7 * nr + row]) : 0.;
             pre[c] = epi.load(row0 + row);
         }
 #pragma unroll
@@ -461,7 +475,7 @@ constexpr int kU = 4;
          base_ += (long long)gridDim.x * (kThreads * kU))
 #define DARE_IDX(u) (base_ + (long long)(u) * kThreads + threadIdx.x)
 
-template <typename XT>
+template <typename XT, bool NT>
 __global__ __launch_bounds__(kThreads) void k_upd_u(long long n, const XT* __restrict__ r, XT* __restrict__ u,
                                                     const KState* __restrict__ st) {
     if (st->status) return;
@@ -476,11 +490,11 @@ __global__ __launch_bounds__(kThreads) void k_upd_u(long long n, const XT* __res
         }
 #pragma unroll
         for (int q = 0; q < kU; q++)
-            if (DARE_IDX(q) < n) u[DARE_IDX(q)] = static_cast<XT>(rv[q] - beta * uv[q]);          // (5)
+            if (DARE_IDX(q) < n) put<NT>(u + DARE_IDX(q), static_cast<XT>(rv[q] - beta * uv[q]));   // (5)
     }
 }
 
-template <typename XT>
+template <typename XT, bool NT>
 __global__ __launch_bounds__(kThreads) void k_upd_rx(long long n, XT* __restrict__ r, const XT* __restrict__ v,
                                                      XT* __restrict__ x, const XT* __restrict__ u,
                                                      const KState* __restrict__ st) {
@@ -500,14 +514,14 @@ __global__ __launch_bounds__(kThreads) void k_upd_rx(long long n, XT* __restrict
         for (int q = 0; q < kU; q++) {
             const long long i = DARE_IDX(q);
             if (i < n) {
-                r[i] = static_cast<XT>(rv[q] - a * vv[q]);                                         // (8)
-                x[i] = static_cast<XT>(1. * xv[q] + a * uv[q]);                                    // (10)
+                put<NT>(r + i, static_cast<XT>(rv[q] - a * vv[q]));                                // (8)
+                put<NT>(x + i, static_cast<XT>(1. * xv[q] + a * uv[q]));                           // (10)
             }
         }
     }
 }
 
-template <typename XT>
+template <typename XT, bool NT>
 __global__ __launch_bounds__(kThreads) void k_upd_v(long long n, const XT* __restrict__ s, XT* __restrict__ v,
                                                     const KState* __restrict__ st) {
     if (st->status) return;
@@ -522,11 +536,11 @@ __global__ __launch_bounds__(kThreads) void k_upd_v(long long n, const XT* __res
         }
 #pragma unroll
         for (int q = 0; q < kU; q++)
-            if (DARE_IDX(q) < n) v[DARE_IDX(q)] = static_cast<XT>(sv[q] - beta * vv[q]);          // (12)
+            if (DARE_IDX(q) < n) put<NT>(v + DARE_IDX(q), static_cast<XT>(sv[q] - beta * vv[q]));   // (12)
     }
 }
 
-template <typename XT>
+template <typename XT, bool NT>
 __global__ __launch_bounds__(kThreads) void k_upd_urs(long long n, XT* __restrict__ u, XT* __restrict__ r,
                                                       const XT* __restrict__ v, XT* __restrict__ s,
                                                       const XT* __restrict__ w, const KState* __restrict__ st) {
@@ -547,16 +561,16 @@ __global__ __launch_bounds__(kThreads) void k_upd_urs(long long n, XT* __restric
         for (int q = 0; q < kU; q++) {
             const long long i = DARE_IDX(q);
             if (i < n) {
-                u[i] = static_cast<XT>(rv[q] - beta * uv[q]);                                      // (15)
-                r[i] = static_cast<XT>(rv[q] - a * vv[q]);                                         // (16)
-                s[i] = static_cast<XT>(sv[q] - a * wv[q]);                                         // (17)
+                put<NT>(u + i, static_cast<XT>(rv[q] - beta * uv[q]));                             // (15)
+                put<NT>(r + i, static_cast<XT>(rv[q] - a * vv[q]));                                // (16)
+                put<NT>(s + i, static_cast<XT>(sv[q] - a * wv[q]));                                // (17)
             }
         }
     }
 }
 
 // x += om1 r + om2 s; x += alpha u; r -= om1 s + om2 t; u -= om1 v + om2 w; <r,r>, <rh,r>
-template <typename XT>
+template <typename XT, bool NT>
 __global__ __launch_bounds__(kThreads) void k_end(long long n, XT* __restrict__ x, XT* __restrict__ r,
                                                   const XT* __restrict__ s, const XT* __restrict__ t,
                                                   XT* __restrict__ u, const XT* __restrict__ v,
@@ -586,10 +600,10 @@ __global__ __launch_bounds__(kThreads) void k_end(long long n, XT* __restrict__ 
                 if (i < n) {
                     double xi = 1. * xv[q] + o1 * rv[q] + o2 * sv[q];                           // (24)
                     xi = 1. * xi + a * uv[q];
-                    x[i] = static_cast<XT>(xi);
+                    put<NT>(x + i, static_cast<XT>(xi));
                     const XT rn = static_cast<XT>(1. * rv[q] + (-o1) * sv[q] + (-o2) * tv[q]);  // (25)
-                    r[i] = rn;
-                    u[i] = static_cast<XT>(1. * uv[q] + (-o1) * vv[q] + (-o2) * wv[q]);
+                    put<NT>(r + i, rn);
+                    put<NT>(u + i, static_cast<XT>(1. * uv[q] + (-o1) * vv[q] + (-o2) * wv[q]));
                     const double rd = static_cast<double>(rn);
                     acc[0] += rd * rd;
                     acc[1] += hv[q] * rd;
