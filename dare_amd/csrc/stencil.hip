@@ -86,10 +86,14 @@ __device__ inline double limiter(int id, double r) {
         return (0. < m) ? m : 0.;                 // std::max(0, m)
     }
     case DARE_LIMITER_VANLEER: {
-        if (isnan(r)) return 0.;
-        if (isinf(r)) return r > 0. ? 2. : 0.;
+        // branch-free: the quotient is formed for every r (inf/inf for infinite r) and the
+        // special cases are selected over it, so a wave never splits and the loads of the
+        // stencil can all be in flight before the first division
         const double a = fabs(r);
-        return (r + a) / (1. + a);
+        double q = (r + a) / (1. + a);
+        asm volatile("" : "+v"(q));   // keeps the division unconditional (not sunk into a branch)
+        const double psi = isinf(r) ? (r > 0. ? 2. : 0.) : q;
+        return isnan(r) ? 0. : psi;
     }
     default: return 0.;
     }
@@ -301,24 +305,17 @@ __device__ inline double tvd_flux4(double a, double b, double c, double d, int l
     return (pU + (psi * 0.5) * (pD - pU)) * u;
 }
 
-constexpr int kStepTileK = 64, kStepTileJ = 4, kStepTileI = 16, kStepXcd = 8;
+constexpr int kStepTileK = 64, kStepTileJ = 4, kStepXcd = 8, kStepMinPlanes = 8;
 
-// 3-D, one component: a 64 (k) x 4 (j) tile marches along i holding the planes i-2..i+2 of phi
-// in registers; the x-face flux at (i, i+1) is carried to the next plane as its low face.
-__global__ __launch_bounds__(256) void k_tvd_step3(Geo G, int lim, const double* __restrict__ phi,
-                                                   const double* __restrict__ ux, const double* __restrict__ uy,
-                                                   const double* __restrict__ uz, double fac,
-                                                   double* __restrict__ out, int tiles_k, int tiles_j,
-                                                   int tiles_i, int per_xcd) {
-    const int tile = static_cast<int>(blockIdx.x % kStepXcd) * per_xcd + static_cast<int>(blockIdx.x / kStepXcd);
-    if (tile >= tiles_k * tiles_j * tiles_i) return;
-    const int tk = tile % tiles_k, t2 = tile / tiles_k, tj = t2 % tiles_j, ti = t2 / tiles_j;
-    const int k = tk * kStepTileK + (threadIdx.x & 63);
-    const int j = tj * kStepTileJ + (threadIdx.x >> 6);
-    const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
-    if (k >= n2 || j >= n1) return;
-    const int i0 = ti * kStepTileI;
-    const int i1 = (i0 + kStepTileI < n0) ? i0 + kStepTileI : n0;
+// One x-march of a 64 (k) x 4 (j) column over the planes [i0, i1): the planes i-2..i+2 of phi
+// stay in registers and the x-face flux at (i, i+1) is carried to the next plane as its low
+// face. NT streams the x-face velocities in and the result out with the non-temporal hint.
+template <bool NT>
+__device__ __forceinline__ void tvd_march3(const Geo& G, int lim, const double* __restrict__ phi,
+                                           const double* __restrict__ ux, const double* __restrict__ uy,
+                                           const double* __restrict__ uz, double fac, double* __restrict__ out,
+                                           int k, int j, int i0, int i1) {
+    const int n1 = G.n[1], n2 = G.n[2];
     const long long sI = G.sl[0], sJ = G.sl[1];
     long long lp = (static_cast<long long>(i0 + G.g) * G.np[1] + (j + G.g)) * G.np[2] + (k + G.g);
     // face arrays: x (n0+1, n1, n2), y (n0, n1+1, n2), z (n0, n1, n2+1)
@@ -329,21 +326,23 @@ __global__ __launch_bounds__(256) void k_tvd_step3(Geo G, int lim, const double*
     long long fz = (static_cast<long long>(i0) * n1 + j) * (n2 + 1) + k;
     const long long fzs = static_cast<long long>(n1) * (n2 + 1);
     double pm2 = phi[lp - 2 * sI], pm1 = phi[lp - sI], p0 = phi[lp], pp1 = phi[lp + sI];
-    double u_lo = ux[fx];
-    double f_lo = tvd_flux4(pm2, pm1, p0, pp1, lim, u_lo);
+    double f_lo = tvd_flux4(pm2, pm1, p0, pp1, lim, ux[fx]);
     const double A0 = G.A[0], A1 = G.A[1], A2 = G.A[2];
     for (int i = i0; i < i1; i++) {
+        // every load of the plane first: x (i+2 and the face velocity), y (cells j-2..j+2),
+        // z (cells k-2..k+2); the scheduling barrier keeps them ahead of the divisions
         const double pp2 = phi[lp + 2 * sI];
-        const double u_hi = ux[fx + fxs];
-        const double f_hi = tvd_flux4(pm1, p0, pp1, pp2, lim, u_hi);
-        // y faces: cells j-2..j+2 of this plane
+        double u_hi;
+        if constexpr (NT) u_hi = ld_nt(ux + fx + fxs);
+        else u_hi = ux[fx + fxs];
         const double ym2 = phi[lp - 2 * sJ], ym1 = phi[lp - sJ], yp1 = phi[lp + sJ], yp2 = phi[lp + 2 * sJ];
         const double vy_lo = uy[fy], vy_hi = uy[fy + n2];
-        const double gy_lo = tvd_flux4(ym2, ym1, p0, yp1, lim, vy_lo);
-        const double gy_hi = tvd_flux4(ym1, p0, yp1, yp2, lim, vy_hi);
-        // z faces: cells k-2..k+2
         const double zm2 = phi[lp - 2], zm1 = phi[lp - 1], zp1 = phi[lp + 1], zp2 = phi[lp + 2];
         const double wz_lo = uz[fz], wz_hi = uz[fz + 1];
+        __builtin_amdgcn_sched_barrier(0);
+        const double f_hi = tvd_flux4(pm1, p0, pp1, pp2, lim, u_hi);
+        const double gy_lo = tvd_flux4(ym2, ym1, p0, yp1, lim, vy_lo);
+        const double gy_hi = tvd_flux4(ym1, p0, yp1, yp2, lim, vy_hi);
         const double hz_lo = tvd_flux4(zm2, zm1, p0, zp1, lim, wz_lo);
         const double hz_hi = tvd_flux4(zm1, p0, zp1, zp2, lim, wz_hi);
         double v = A0 * f_hi;
@@ -352,7 +351,7 @@ __global__ __launch_bounds__(256) void k_tvd_step3(Geo G, int lim, const double*
         v -= A1 * gy_lo;
         v += A2 * hz_hi;
         v -= A2 * hz_lo;
-        out[lp] = p0 - fac * v;
+        put<NT>(out + lp, p0 - fac * v);
         // roll
         pm2 = pm1;
         pm1 = p0;
@@ -363,6 +362,33 @@ __global__ __launch_bounds__(256) void k_tvd_step3(Geo G, int lim, const double*
         fx += fxs;
         fy += fys;
         fz += fzs;
+    }
+}
+
+// 3-D, one component, persistent and balanced: the work is the list of column-planes
+// (column = a 64 x 4 tile of (k, j), ordered k-tile fastest; plane = one i), and logical block
+// L takes the contiguous range [L*chunk, (L+1)*chunk), marching each column piece it covers
+// along i. Logical blocks are dealt so that each XCD (blocks b = x mod 8) owns a contiguous
+// run of columns, whose j-halo rows its neighbours read at the same planes from the same L2.
+// The limiter is a template parameter so that each instantiation carries only its own branch.
+template <bool NT, int LIM>
+__global__ __launch_bounds__(256) void k_tvd_step3(Geo G, const double* __restrict__ phi,
+                                                   const double* __restrict__ ux, const double* __restrict__ uy,
+                                                   const double* __restrict__ uz, double fac,
+                                                   double* __restrict__ out, int tiles_k, long long total,
+                                                   long long chunk, int per_xcd) {
+    const long long L = static_cast<long long>(blockIdx.x % kStepXcd) * per_xcd + blockIdx.x / kStepXcd;
+    const int n0 = G.n[0];
+    long long c = L * chunk;
+    const long long cend = (c + chunk < total) ? c + chunk : total;
+    while (c < cend) {
+        const long long col = c / n0;
+        const int i0 = static_cast<int>(c - col * n0);
+        const int i1 = (cend - c < n0 - i0) ? i0 + static_cast<int>(cend - c) : n0;
+        const int k = static_cast<int>(col % tiles_k) * kStepTileK + (threadIdx.x & 63);
+        const int j = static_cast<int>(col / tiles_k) * kStepTileJ + (threadIdx.x >> 6);
+        if (k < G.n[2] && j < G.n[1]) tvd_march3<NT>(G, LIM, phi, ux, uy, uz, fac, out, k, j, i0, i1);
+        c += i1 - i0;
     }
 }
 
@@ -380,6 +406,40 @@ __global__ __launch_bounds__(256) void k_copy_to_field(Geo G, const double* __re
     for (int d = 0; d < DIM; d++) nd += G.si[d] * q[d];
     const long long lp = pad_ord<DIM>(G, q);
     for (int c = 0; c < G.N; c++) phi[lp * G.N + c] = x[nd * G.N + c];
+}
+
+// Launch of k_tvd_step3<NT, LIM> with as many blocks as stay resident at once (no tail
+// round, occupancy queried once per instantiation), each with at least kStepMinPlanes planes.
+template <bool NT, int LIM>
+void launch_step3(dare_ctx* ctx, const Geo& G, const double* phi, const double* ux, const double* uy,
+                  const double* uz, double fac, double* out) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tvd_step3<NT, LIM>, 256, 0);
+        per_cu = (e == hipSuccess && n > 0) ? n : 4;
+    }
+    const int tk = static_cast<int>(ceil_div(G.n[2], kStepTileK));
+    const long long total = static_cast<long long>(tk) * ceil_div(G.n[1], kStepTileJ) * G.n[0];
+    long long nb = static_cast<long long>(ctx->num_cu) * per_cu;
+    const long long by_work = ceil_div(total, kStepMinPlanes);
+    if (nb > by_work) nb = by_work;
+    const int per = static_cast<int>(ceil_div(nb, kStepXcd));
+    const long long chunk = ceil_div(total, static_cast<long long>(per) * kStepXcd);
+    hipLaunchKernelGGL((k_tvd_step3<NT, LIM>), dim3(kStepXcd * per), dim3(256), 0, ctx->stream, G, phi, ux, uy, uz,
+                       fac, out, tk, total, chunk, per);
+}
+
+template <bool NT>
+void launch_step3_lim(dare_ctx* ctx, const Geo& G, int lim, const double* phi, const double* ux,
+                      const double* uy, const double* uz, double fac, double* out) {
+    switch (lim) {
+    case DARE_LIMITER_CDS: launch_step3<NT, DARE_LIMITER_CDS>(ctx, G, phi, ux, uy, uz, fac, out); break;
+    case DARE_LIMITER_UPWIND: launch_step3<NT, DARE_LIMITER_UPWIND>(ctx, G, phi, ux, uy, uz, fac, out); break;
+    case DARE_LIMITER_VANALBADA: launch_step3<NT, DARE_LIMITER_VANALBADA>(ctx, G, phi, ux, uy, uz, fac, out); break;
+    case DARE_LIMITER_MINMOD: launch_step3<NT, DARE_LIMITER_MINMOD>(ctx, G, phi, ux, uy, uz, fac, out); break;
+    default: launch_step3<NT, DARE_LIMITER_VANLEER>(ctx, G, phi, ux, uy, uz, fac, out); break;
+    }
 }
 
 #define DARE_DISPATCH_DIM(G, KERNEL, L, ...)                                                       \
@@ -508,13 +568,8 @@ int dare_tvd_explicit_step(dare_ctx* ctx, const dare_grid* g, int32_t limiter, c
     const Launch L = cell_launch(G);
     const double fac = dt / G.V;
     if (G.dim == 3 && G.N == 1) {
-        const int tk = static_cast<int>(ceil_div(G.n[2], kStepTileK));
-        const int tj = static_cast<int>(ceil_div(G.n[1], kStepTileJ));
-        const int ti = static_cast<int>(ceil_div(G.n[0], kStepTileI));
-        const long long T = static_cast<long long>(tk) * tj * ti;
-        const int per = static_cast<int>((T + kStepXcd - 1) / kStepXcd);
-        hipLaunchKernelGGL(k_tvd_step3, dim3(kStepXcd * per), dim3(256), 0, ctx->stream, G, limiter, phi, ux, uy, uz,
-                           fac, phi_out, tk, tj, ti, per);
+        if (G.ncell_int >= kStreamRows) launch_step3_lim<true>(ctx, G, limiter, phi, ux, uy, uz, fac, phi_out);
+        else launch_step3_lim<false>(ctx, G, limiter, phi, ux, uy, uz, fac, phi_out);
         DARE_LAUNCH_CHECK();
         return DARE_OK;
     }
