@@ -161,18 +161,35 @@ def main():
     copy_gbs = 2 * src.numel() * 8 / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
     del src, dst
 
-    # TVD step kernel, same method
-    for _ in range(2):
-        da.tvd_explicit_step(ctx, g, da.VANLEER, W.phi, W.vel, W.dt, W.phi_work)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
-        da.tvd_explicit_step(ctx, g, da.VANLEER, W.phi, W.vel, W.dt, W.phi_work)
-    e1.record()
-    torch.cuda.synchronize()
-    tvd_s = e0.elapsed_time(e1) / 1e3 / reps
+    # the stencil kernels, same method: TVD step (phi + 3 face velocities in, phi out), face
+    # gradient (phi in, 3 face arrays out), value divergence (3 face arrays in, cells out)
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
     ncell = args.n ** 3
-    tvd_bytes = 40 * ncell
+    nface = 3 * ncell + 3 * args.n ** 2
+    tvd_s = timed(lambda: da.tvd_explicit_step(ctx, g, da.VANLEER, W.phi, W.vel, W.dt, W.phi_work))
+    tvd_bytes = 8 * (2 * ncell + nface)
+    faces = [torch.empty(v.numel(), dtype=torch.float64, device="cuda") for v in W.vel]
+    grad_s = timed(lambda: da.grad_faces(ctx, g, W.phi, faces))
+    div_s = timed(lambda: da.div_faces(ctx, g, faces, W.phi_work))
+    del faces
+    stencils = {}
+    for name, kern, sec, nbytes in (("tvd_step", "k_tvd_step3", tvd_s, tvd_bytes),
+                                    ("gradient", "k_grad_faces", grad_s, 8 * (ncell + nface)),
+                                    ("divergence", "k_div_faces", div_s, 8 * (nface + ncell))):
+        gbs = nbytes / sec / 1e9
+        stencils[name] = {"kernel": kern, "bytes_per_cell": round(nbytes / ncell, 3), "achieved": round(gbs, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                          "avg_launch_us": round(sec * 1e6, 2), "cell_updates_per_s": round(ncell / sec, 1)}
 
     best = min(results, key=lambda f: results[f]["solve_ms"] if results[f]["converged"] else 1e30)
 
@@ -222,11 +239,7 @@ def main():
             "attainable_copy_GBs": round(copy_gbs, 1),
             "frac_of_attainable": round(achieved / copy_gbs, 4),
         },
-        "tvd_step_roofline": {
-            "kernel": "k_tvd_step3", "bytes_per_cell": 40, "achieved": round(tvd_bytes / tvd_s / 1e9, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(tvd_bytes / tvd_s / 1e9 / HBM_PEAK_GBS, 4),
-            "avg_launch_us": round(tvd_s * 1e6, 2), "cell_updates_per_s": round(ncell / tvd_s, 1),
-        },
+        "stencil_rooflines": stencils,
         "cpu_baseline": cpu,
         **extra,
     }

@@ -399,9 +399,9 @@ __global__ __launch_bounds__(256) void k_fill_values(Geo G, Terms T, long long n
 // Diagonal-view fill (3-D): one lane per cell, lanes along k, no column sort. Writes the slot
 // values (coalesced, slot-major), rhs and x0. The CSR values are materialized from the slots
 // only when a CSR consumer asks for them (k_materialize_csr, same sort and merge).
-template <bool NT>
-__global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __restrict__ dia, int slots,
-                                                  double* __restrict__ rhs, double* __restrict__ x0) {
+template <bool NT, bool F32>
+__global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __restrict__ dia, float* __restrict__ dia32,
+                                                  int slots, double* __restrict__ rhs, double* __restrict__ x0) {
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int i = blockIdx.z;
@@ -415,6 +415,8 @@ __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __rest
         double r = 0., x = 0.;
         row_entries<kRowValues, false>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
         for (int sl = 0; sl < slots; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
+        if constexpr (F32)
+            for (int sl = 0; sl < slots; sl++) put<NT>(dia32 + sl * nrows + row, static_cast<float>(slotv[sl]));
         put<NT>(rhs + row, r);
         if (x0) put<NT>(x0 + row, x);
     }
@@ -619,6 +621,7 @@ int dare_csr_build_structure(dare_ctx* ctx, dare_csr* A, const dare_grid* g, con
     for (int f = 0; f < 6; f++) A->bc_type[f] = T.bc_type[f];
     A->cross = T.cross;
     A->vals_stale = 0;
+    A->dia32_fresh = 0;
     // diagonal view for the solver (3-D, 1 or 2 components)
     A->has_dia = 0;
     if (G.dim == 3 && (G.N == 1 || G.N == 2)) {
@@ -666,12 +669,24 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         DARE_CHECK_ARG(T.bc_type[f] == A->bc_type[f], "boundary edits differ from the structure's");
     if (A->has_dia) {
         const dim3 grid(ceil_div(G.n[2], 64), ceil_div(G.n[1], 4), G.n[0]);
-        if (A->nrows >= kStreamRows)
-            hipLaunchKernelGGL(k_fill_dia<true>, grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->cross ? 8 : 7, rhs, x0);
+        const int slots = A->cross ? 8 : 7;
+        // the FP32 copy rides along once a mixed solve has used this matrix
+        const bool f32 = A->dia32_mode && A->dia32 && A->cap_dia32 >= static_cast<int64_t>(slots) * nrows;
+        const bool nt = A->nrows >= kStreamRows;
+        if (nt && f32)
+            hipLaunchKernelGGL((k_fill_dia<true, true>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->dia32, slots,
+                               rhs, x0);
+        else if (nt)
+            hipLaunchKernelGGL((k_fill_dia<true, false>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, nullptr, slots,
+                               rhs, x0);
+        else if (f32)
+            hipLaunchKernelGGL((k_fill_dia<false, true>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->dia32, slots,
+                               rhs, x0);
         else
-            hipLaunchKernelGGL(k_fill_dia<false>, grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->cross ? 8 : 7, rhs,
-                               x0);
+            hipLaunchKernelGGL((k_fill_dia<false, false>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, nullptr, slots,
+                               rhs, x0);
         DARE_LAUNCH_CHECK();
+        A->dia32_fresh = f32 ? 1 : 0;
         A->vals_stale = 1;
         return DARE_OK;
     }
@@ -704,6 +719,7 @@ int dare_csr_upload_host(dare_ctx* ctx, dare_csr* A, int64_t nrows, int64_t nnz,
     A->stencil_structured = 0;
     A->has_dia = 0;
     A->vals_stale = 0;
+    A->dia32_fresh = 0;
     return DARE_OK;
 }
 

@@ -107,6 +107,9 @@ struct dare_csr {
     double* dia = nullptr;
     float* dia32 = nullptr;
     int64_t cap_dia = 0, cap_dia32 = 0;
+    // the FP32 copy for mixed solves: once a mixed solve has asked for it (dia32_mode), every
+    // fill writes it next to the FP64 slots; dia32_fresh says it holds float(dia) of the last fill
+    int dia32_mode = 0, dia32_fresh = 0;
     // Jacobi right preconditioning: 1 / diag(A) and the column-scaled FP64 view A diag(A)^-1
     double* dinv = nullptr;
     double* dia_pre = nullptr;

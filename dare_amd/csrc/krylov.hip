@@ -93,7 +93,7 @@ static_assert(sizeof(KState) <= 64 * sizeof(double), "pinned readback block too 
 template <typename VT, typename XT, bool NT>
 int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
                      double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                     bool* converged, XT* work, int min_passes) {
+                     bool* converged, XT* work, int min_passes, bool x_zero) {
     const long long n = A.n;
     XT* r = work;
     XT* rh = r + n;
@@ -108,7 +108,12 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
     ReduceWs ws = ctx->ws;
     *hist_len = 0;
     *status = 0;
-    A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);     // (0)-(2)
+    if (x_zero) {   // x = 0 written here: no operator sweep needed for r = b - A x
+        hipLaunchKernelGGL((k_init_zero<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, b, x, r, rh, u, v, w, ws, st);
+        launch_finish<2>(sm, ws, gr, st, TAIL_INIT, 0);
+    } else {
+        A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);     // (0)-(2)
+    }
     DARE_LAUNCH_CHECK();
     KState h;
     int rc = read_state(ctx, st, &h);
@@ -170,12 +175,12 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
 template <typename VT, typename XT>
 int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
                   double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                  bool* converged, XT* work, int min_passes) {
+                  bool* converged, XT* work, int min_passes, bool x_zero = false) {
     if (A.n >= kStreamRows)
         return bicgstab2_run_nt<VT, XT, true>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
-                                              status, converged, work, min_passes);
+                                              status, converged, work, min_passes, x_zero);
     return bicgstab2_run_nt<VT, XT, false>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
-                                           status, converged, work, min_passes);
+                                           status, converged, work, min_passes, x_zero);
 }
 
 static int dot_dev(dare_ctx* ctx, long long n, const double* a, const double* b, double* out) {
@@ -441,6 +446,7 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
             if (A->dia32) hipFree(A->dia32);
             A->dia32 = nullptr;
             A->cap_dia32 = 0;
+            A->dia32_fresh = 0;
             if (hipMalloc(&A->dia32, sizeof(float) * need) != hipSuccess) {
                 hipGetLastError();
                 set_error("FP32 diagonal allocation failed");
@@ -453,9 +459,15 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
             if (prc) return prc;
             hipLaunchKernelGGL((k_dia_colscale<float>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream,
                                A->dia_view, A->dia, A->dinv, A->dia32);
+            A->dia32_fresh = 0;
         } else {
-            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(need)), dim3(kThreads), 0, ctx->stream, (long long)need,
-                               A->dia, A->dia32);
+            // written by the last fill when it already ran in FP32 mode, converted here otherwise;
+            // from now on the fills of this matrix write it themselves
+            if (!A->dia32_fresh)
+                hipLaunchKernelGGL(k_to_f32, dim3(grid_for(need)), dim3(kThreads), 0, ctx->stream, (long long)need,
+                                   A->dia, A->dia32);
+            A->dia32_fresh = 1;
+            A->dia32_mode = 1;
         }
         op32.d = make_dia<float>(A->dia_view, A->dia32);
         op32.dia = kXcd * op32.d.per_xcd <= kReduceBlocks;
@@ -521,12 +533,11 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
             break;
         }
         if (refine >= max_refine || total_it >= p->max_it) break;
-        hipLaunchKernelGGL((k_zero<float>), dim3(gr), dim3(kThreads), 0, ctx->stream, n, d32);
         int it = 0, ist = 0, dummy_len = 0;
         double irn = 0.;
         bool iconv = false;
         rc = bicgstab2_run<float, float>(ctx, op32, r32, d32, inner_tol * rn, inner_max, nullptr, 0, &dummy_len, &it,
-                                         &irn, &ist, &iconv, work, 2);
+                                         &irn, &ist, &iconv, work, 2, true);   // d32 = 0 to start
         if (rc) return rc;
         total_it += it;
         if (pre)    // the inner solution is y = M d

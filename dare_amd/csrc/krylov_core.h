@@ -416,8 +416,10 @@ __device__ inline void dia_column(const DiaView& V, const VT* __restrict__ D, co
             else dv[c][7] = (NC == 2 && V.cross) ? static_cast<double>(D[7 * nr + row]) : 0.;
             pre[c] = epi.load(row0 + row);
         }
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
+        // straight-line per component (no loop around the epilogue's stores: LLVM's loop
+        // invariant code motion promotes a store out of a one-trip loop and drops its
+        // non-temporal hint on the way)
+        auto row_out = [&](const int c) {
             double s = 0.;
             s += hasW ? dv[c][0] * xp[c] : 0.;
             s += hasS ? dv[c][1] * xs[c] : 0.;
@@ -429,7 +431,9 @@ __device__ inline void dia_column(const DiaView& V, const VT* __restrict__ D, co
             s += hasN ? dv[c][5] * xnn[c] : 0.;
             s += hasE ? dv[c][6] * xn[c] : 0.;
             epi(row0 + node * NC + c, s, xc[c], pre[c], acc);
-        }
+        };
+        row_out(0);
+        if constexpr (NC == 2) row_out(1);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             xp[c] = xc[c];
@@ -631,6 +635,37 @@ __global__ __launch_bounds__(kThreads) void k_dot(long long n, const XT* __restr
             if (DARE_IDX(q) < n) acc[0] += av[q] * bv[q];
     }
     grid_reduce<1>(acc, ws, st, TAIL_STORE);
+}
+
+// Initialisation (0)-(2) for a zero initial guess: A x0 is +0 in every row, so r = b - A x0 = b
+// bit for bit; x = u = v = w = 0, rh = r, and <r,r>, <rh,r> go to the reduction as EpiInit's do.
+template <typename XT, bool NT>
+__global__ __launch_bounds__(kThreads) void k_init_zero(long long n, const XT* __restrict__ b, XT* __restrict__ x,
+                                                        XT* __restrict__ r, XT* __restrict__ rh, XT* __restrict__ u,
+                                                        XT* __restrict__ v, XT* __restrict__ w, ReduceWs ws,
+                                                        KState* st) {
+    double acc[2] = {0., 0.};
+    DARE_CHUNK_LOOP(n) {
+        XT bv[kU];
+#pragma unroll
+        for (int q = 0; q < kU; q++) bv[q] = b[DARE_IDX(q) < n ? DARE_IDX(q) : 0];
+#pragma unroll
+        for (int q = 0; q < kU; q++) {
+            const long long i = DARE_IDX(q);
+            if (i < n) {
+                put<NT>(x + i, XT(0));
+                put<NT>(r + i, bv[q]);
+                put<NT>(rh + i, bv[q]);
+                put<NT>(u + i, XT(0));
+                put<NT>(v + i, XT(0));
+                put<NT>(w + i, XT(0));
+                const double rd = static_cast<double>(bv[q]);
+                acc[0] += rd * rd;
+                acc[1] += rd * rd;
+            }
+        }
+    }
+    grid_reduce<2>(acc, ws, st, TAIL_INIT);
 }
 
 __global__ __launch_bounds__(kThreads) void k_axpby(long long n, double a, const double* __restrict__ x, double b,
