@@ -36,6 +36,9 @@ struct SolveOp {
             if (d.V.N == 2)
                 hipLaunchKernelGGL((k_apply_dia<2, VT, XT, Epi>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws, st,
                                    tail, check);
+            else if (d.pair == 2)
+                hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 2>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws,
+                                   st, tail, check);
             else
                 hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws, st,
                                    tail, check);
@@ -292,7 +295,7 @@ static SolveOp<double> op64(const dare_csr* A, bool allow_dia) {
     op.csr = CsrOp<double>{A->row_ptr, A->col, A->val, A->nrows};
     op.dia = allow_dia && A->has_dia;
     if (op.dia) {
-        op.d = make_dia<double>(A->dia_view, A->dia);
+        op.d = make_dia<double>(A->dia_view, A->dia, true);
         op.dia = kXcd * op.d.per_xcd <= kReduceBlocks;   // else the CSR traversal
     }
     return op;
@@ -401,7 +404,7 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
                            A->dia, A->dinv, Am->dia_pre);
         hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 0);  // y0 = M x0
         DARE_LAUNCH_CHECK();
-        op.d = make_dia<double>(A->dia_view, Am->dia_pre);
+        op.d = make_dia<double>(A->dia_view, Am->dia_pre, true);
     }
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
@@ -469,7 +472,7 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
             A->dia32_fresh = 1;
             A->dia32_mode = 1;
         }
-        op32.d = make_dia<float>(A->dia_view, A->dia32);
+        op32.d = make_dia<float>(A->dia_view, A->dia32, true);
         op32.dia = kXcd * op32.d.per_xcd <= kReduceBlocks;
     } else {
         if (pre) {

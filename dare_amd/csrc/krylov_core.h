@@ -208,14 +208,16 @@ struct DiaOp {
     int tiles_k, tiles_j, tiles_i;
     int tile_i;         // planes per tile (grown until the grid fits the reduction slots)
     int per_xcd;        // tiles owned by each XCD (grid = kXcd * per_xcd)
+    int pair;           // cells per lane along k (2: tiles of 128 (k) x 4 (j), one component, n2 even)
 };
 
 template <typename VT>
-DiaOp<VT> make_dia(const DiaView& V, const VT* D) {
+DiaOp<VT> make_dia(const DiaView& V, const VT* D, bool allow_pair = false) {
     DiaOp<VT> A;
     A.V = V;
     A.D = D;
-    A.tiles_k = (V.n[2] + kTileK - 1) / kTileK;
+    A.pair = (allow_pair && V.N == 1 && V.n[2] % 2 == 0) ? 2 : 1;
+    A.tiles_k = (V.n[2] + kTileK * A.pair - 1) / (kTileK * A.pair);
     A.tiles_j = (V.n[1] + kTileJ - 1) / kTileJ;
     A.tile_i = kTileI;
     long long T = 0;
@@ -324,6 +326,37 @@ struct EpiStore {
         put<NT>(out + i, static_cast<XT>(y));
     }
 };
+
+// The epilogue of a lane's two adjacent rows i, i+1 (paired traversal): by default two calls,
+// whose single stores land next to each other and merge into one wide store; EpiInit writes
+// five vectors per row, so its pair form orders the stores vector by vector for the same
+// effect. The dot accumulators see row i, then row i+1, as with two calls.
+template <class Epi>
+__device__ __forceinline__ void epi_pair(const Epi& e, long long i, double y0, double y1, double x0, double x1,
+                                         double p0, double p1, double* acc) {
+    e(i, y0, x0, p0, acc);
+    e(i + 1, y1, x1, p1, acc);
+}
+template <typename XT, bool NT>
+__device__ __forceinline__ void epi_pair(const EpiInit<XT, NT>& e, long long i, double y0, double y1, double,
+                                         double, double b0, double b1, double* acc) {
+    const XT r0 = static_cast<XT>(b0 - y0), r1 = static_cast<XT>(b1 - y1);
+    put<NT>(e.r + i, r0);
+    put<NT>(e.r + i + 1, r1);
+    put<NT>(e.rh + i, r0);
+    put<NT>(e.rh + i + 1, r1);
+    put<NT>(e.u + i, XT(0));
+    put<NT>(e.u + i + 1, XT(0));
+    put<NT>(e.v + i, XT(0));
+    put<NT>(e.v + i + 1, XT(0));
+    put<NT>(e.w + i, XT(0));
+    put<NT>(e.w + i + 1, XT(0));
+    const double d0 = static_cast<double>(r0), d1 = static_cast<double>(r1);
+    acc[0] += d0 * d0;
+    acc[1] += d0 * d0;
+    acc[0] += d1 * d1;
+    acc[1] += d1 * d1;
+}
 
 // ------------------------------------------------------------------------------ traversals
 // Generic CSR: one row per lane; the row's entries are fetched four at a time with clamped
@@ -443,7 +476,83 @@ __device__ inline void dia_column(const DiaView& V, const VT* __restrict__ D, co
     }
 }
 
-template <int NC, typename VT, typename XT, class Epi>
+// dia_column_pair: one component, the two cells k (even) and k+1 of a lane. Their slot values,
+// x entries along i and j and epilogue operands are adjacent in memory, so each pair is one
+// access twice as wide (a wave moves twice the bytes per memory instruction). Within the pair
+// the top neighbour of cell k and the bottom neighbour of cell k+1 are the pair itself; only
+// k-1 (cell k) and k+2 (cell k+1) are single loads. Each cell's sum keeps the order of
+// dia_column, so y is bit-identical; the per-lane dot accumulation order differs.
+template <typename VT, typename XT, class Epi>
+__device__ inline void dia_column_pair(const DiaView& V, const VT* __restrict__ D, const XT* __restrict__ x,
+                                       int k, int j, int i0, int i1, const Epi& epi, double* acc) {
+    const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
+    const long long sI = static_cast<long long>(n1) * n2, sJ = n2;
+    const long long nr = V.nrows;
+    const bool hasS = !(j == 0 && V.rm_lo[1]);
+    const bool hasN = !(j == n1 - 1 && V.rm_hi[1] && !(j == 0 && V.rm_lo[1]));
+    const bool hasB = !(k == 0 && V.rm_lo[2]);               // cell k   (its top is cell k+1)
+    const bool hasT = !(k + 1 == n2 - 1 && V.rm_hi[2]);      // cell k+1 (its bottom is cell k; n2 >= 2)
+    const long long dS = !hasS ? 0 : (j == 0) ? (long long)(n1 - 1) * sJ : -sJ;
+    const long long dN = !hasN ? 0 : (j == n1 - 1) ? -(long long)(n1 - 1) * sJ : sJ;
+    const long long dB = !hasB ? 0 : (k == 0) ? (long long)(n2 - 1) : -1;            // from cell k
+    const long long dT = !hasT ? 0 : (k + 1 == n2 - 1) ? -(long long)(n2 - 1) : 1;   // from cell k+1
+    long long node = (static_cast<long long>(i0) * n1 + j) * n2 + k;
+    double xp0, xp1, xc0, xc1;
+    {
+        const bool need = !(i0 == 0 && V.rm_lo[0]);
+        const long long nprev = !need ? node : (i0 == 0) ? node + (long long)(n0 - 1) * sI : node - sI;
+        xc0 = static_cast<double>(x[node]);
+        xc1 = static_cast<double>(x[node + 1]);
+        xp0 = static_cast<double>(x[nprev]);
+        xp1 = static_cast<double>(x[nprev + 1]);
+    }
+    for (int i = i0; i < i1; i++) {
+        const bool hasW = !(i == 0 && V.rm_lo[0]);
+        const bool hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
+        const long long nnext = !hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
+        const double xn0 = static_cast<double>(x[nnext]), xn1 = static_cast<double>(x[nnext + 1]);
+        const double xs0 = static_cast<double>(x[node + dS]), xs1 = static_cast<double>(x[node + dS + 1]);
+        const double xq0 = static_cast<double>(x[node + dN]), xq1 = static_cast<double>(x[node + dN + 1]);
+        const double xb = static_cast<double>(x[node + dB]);
+        const double xt = static_cast<double>(x[node + 1 + dT]);
+        double d0[7], d1[7];
+#pragma unroll
+        for (int sl = 0; sl < 7; sl++) {
+            if constexpr (Epi::kNT) {
+                d0[sl] = static_cast<double>(ld_nt(D + sl * nr + node));
+                d1[sl] = static_cast<double>(ld_nt(D + sl * nr + node + 1));
+            } else {
+                d0[sl] = static_cast<double>(D[sl * nr + node]);
+                d1[sl] = static_cast<double>(D[sl * nr + node + 1]);
+            }
+        }
+        const double pre0 = epi.load(node), pre1 = epi.load(node + 1);
+        double s0 = 0.;
+        s0 += hasW ? d0[0] * xp0 : 0.;
+        s0 += hasS ? d0[1] * xs0 : 0.;
+        s0 += hasB ? d0[2] * xb : 0.;
+        s0 += d0[3] * xc0;
+        s0 += d0[4] * xc1;
+        s0 += hasN ? d0[5] * xq0 : 0.;
+        s0 += hasE ? d0[6] * xn0 : 0.;
+        double s1 = 0.;
+        s1 += hasW ? d1[0] * xp1 : 0.;
+        s1 += hasS ? d1[1] * xs1 : 0.;
+        s1 += d1[2] * xc0;
+        s1 += d1[3] * xc1;
+        s1 += hasT ? d1[4] * xt : 0.;
+        s1 += hasN ? d1[5] * xq1 : 0.;
+        s1 += hasE ? d1[6] * xn1 : 0.;
+        epi_pair(epi, node, s0, s1, xc0, xc1, pre0, pre1, acc);
+        xp0 = xc0;
+        xp1 = xc1;
+        xc0 = xn0;
+        xc1 = xn1;
+        node += sI;
+    }
+}
+
+template <int NC, typename VT, typename XT, class Epi, int PAIR = 1>
 __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
                                                        KState* st, int tail, int check_status) {
     double acc[Epi::NDOT];
@@ -457,12 +566,13 @@ __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* _
         const int t2 = tile / A.tiles_k;
         const int tj = t2 % A.tiles_j;
         const int ti = t2 / A.tiles_j;
-        const int k = tk * kTileK + (threadIdx.x & 63);
+        const int k = (tk * kTileK + (threadIdx.x & 63)) * PAIR;
         const int j = tj * kTileJ + (threadIdx.x >> 6);
         if (k < A.V.n[2] && j < A.V.n[1]) {
             const int i0 = ti * A.tile_i;
             const int i1 = (i0 + A.tile_i < A.V.n[0]) ? i0 + A.tile_i : A.V.n[0];
-            dia_column<NC, VT, XT, Epi>(A.V, A.D, x, 0, k, j, i0, i1, epi, acc);
+            if constexpr (PAIR == 2) dia_column_pair<VT, XT, Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
+            else dia_column<NC, VT, XT, Epi>(A.V, A.D, x, 0, k, j, i0, i1, epi, acc);
         }
     }
     if (tail >= 0) grid_reduce<Epi::NDOT>(acc, ws, st, run ? tail : TAIL_NONE);

@@ -365,18 +365,12 @@ __device__ __forceinline__ void tvd_march3(const Geo& G, int lim, const double* 
     }
 }
 
-// 3-D, one component, persistent and balanced: the work is the list of column-planes
-// (column = a 64 x 4 tile of (k, j), ordered k-tile fastest; plane = one i), and logical block
-// L takes the contiguous range [L*chunk, (L+1)*chunk), marching each column piece it covers
-// along i. Logical blocks are dealt so that each XCD (blocks b = x mod 8) owns a contiguous
-// run of columns, whose j-halo rows its neighbours read at the same planes from the same L2.
-// The limiter is a template parameter so that each instantiation carries only its own branch.
-template <bool NT, int LIM>
-__global__ __launch_bounds__(256) void k_tvd_step3(Geo G, const double* __restrict__ phi,
-                                                   const double* __restrict__ ux, const double* __restrict__ uy,
-                                                   const double* __restrict__ uz, double fac,
-                                                   double* __restrict__ out, int tiles_k, long long total,
-                                                   long long chunk, int per_xcd) {
+// Walks logical block L's column-plane range [L*chunk, (L+1)*chunk) (columns of 64 (k) x 4
+// (j) cells ordered k-tile fastest, planes along i) and calls march(k, j, i0, i1) for each
+// column piece; blocks are dealt XCD by XCD (blocks b = x mod 8 own a contiguous run).
+template <class F>
+__device__ __forceinline__ void march_columns(const Geo& G, int tiles_k, long long total, long long chunk,
+                                              int per_xcd, F&& march) {
     const long long L = static_cast<long long>(blockIdx.x % kStepXcd) * per_xcd + blockIdx.x / kStepXcd;
     const int n0 = G.n[0];
     long long c = L * chunk;
@@ -387,9 +381,100 @@ __global__ __launch_bounds__(256) void k_tvd_step3(Geo G, const double* __restri
         const int i1 = (cend - c < n0 - i0) ? i0 + static_cast<int>(cend - c) : n0;
         const int k = static_cast<int>(col % tiles_k) * kStepTileK + (threadIdx.x & 63);
         const int j = static_cast<int>(col / tiles_k) * kStepTileJ + (threadIdx.x >> 6);
-        if (k < G.n[2] && j < G.n[1]) tvd_march3<NT>(G, LIM, phi, ux, uy, uz, fac, out, k, j, i0, i1);
+        if (k < G.n[2] && j < G.n[1]) march(k, j, i0, i1);
         c += i1 - i0;
     }
+}
+
+// 3-D, one component, persistent and balanced TVD step: march_columns hands out the
+// column-plane ranges; each XCD's contiguous run of columns shares its j-halo rows in its L2.
+// The limiter is a template parameter so that each instantiation carries only its own branch.
+template <bool NT, int LIM>
+__global__ __launch_bounds__(256) void k_tvd_step3(Geo G, const double* __restrict__ phi,
+                                                   const double* __restrict__ ux, const double* __restrict__ uy,
+                                                   const double* __restrict__ uz, double fac,
+                                                   double* __restrict__ out, int tiles_k, long long total,
+                                                   long long chunk, int per_xcd) {
+    march_columns(G, tiles_k, total, chunk, per_xcd, [&](int k, int j, int i0, int i1) {
+        tvd_march3<NT>(G, LIM, phi, ux, uy, uz, fac, out, k, j, i0, i1);
+    });
+}
+
+// 3-D, one component: face gradient (same expressions as k_grad_faces) marching along i with
+// phi(i-1) in a register; face i of an axis is (phi(i) - phi(i-1)) dn_r, the high boundary face
+// (phi(n) - phi(n-1)) dn_r is written by the last cell.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_grad3(Geo G, const double* __restrict__ phi, double* __restrict__ fx,
+                                               double* __restrict__ fy, double* __restrict__ fz, int tiles_k,
+                                               long long total, long long chunk, int per_xcd) {
+    march_columns(G, tiles_k, total, chunk, per_xcd, [&](int k, int j, int i0, int i1) {
+        const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+        const long long sI = G.sl[0], sJ = G.sl[1];
+        const double d0 = G.dnr[0], d1 = G.dnr[1], d2 = G.dnr[2];
+        const bool lastj = j == n1 - 1, lastk = k == n2 - 1;
+        long long lp = (static_cast<long long>(i0 + G.g) * G.np[1] + (j + G.g)) * G.np[2] + (k + G.g);
+        const long long fxs = static_cast<long long>(n1) * n2;
+        long long ox = static_cast<long long>(i0) * fxs + static_cast<long long>(j) * n2 + k;
+        long long oy = (static_cast<long long>(i0) * (n1 + 1) + j) * n2 + k;
+        const long long fys = static_cast<long long>(n1 + 1) * n2;
+        long long oz = (static_cast<long long>(i0) * n1 + j) * (n2 + 1) + k;
+        const long long fzs = static_cast<long long>(n1) * (n2 + 1);
+        double pm = phi[lp - sI];
+        for (int i = i0; i < i1; i++) {
+            const double p0 = phi[lp], pS = phi[lp - sJ], pB = phi[lp - 1];
+            const double pN = phi[lastj ? lp + sJ : lp], pT = phi[lastk ? lp + 1 : lp];
+            put<NT>(fx + ox, (p0 - pm) * d0);
+            put<NT>(fy + oy, (p0 - pS) * d1);
+            if (lastj) put<NT>(fy + oy + n2, (pN - p0) * d1);
+            put<NT>(fz + oz, (p0 - pB) * d2);
+            if (lastk) put<NT>(fz + oz + 1, (pT - p0) * d2);
+            pm = p0;
+            lp += sI;
+            ox += fxs;
+            oy += fys;
+            oz += fzs;
+        }
+        if (i1 == n0) put<NT>(fx + ox, (phi[lp] - pm) * d0);   // lp is now the ghost plane n0
+    });
+}
+
+// 3-D, one component: value divergence A0 E - A0 W + A1 N - A1 S + A2 T - A2 B (the order of
+// k_div_faces) marching along i with the x face shared by consecutive planes in a register.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_div3(Geo G, const double* __restrict__ fx, const double* __restrict__ fy,
+                                              const double* __restrict__ fz, double* __restrict__ out, int tiles_k,
+                                              long long total, long long chunk, int per_xcd) {
+    march_columns(G, tiles_k, total, chunk, per_xcd, [&](int k, int j, int i0, int i1) {
+        const int n1 = G.n[1], n2 = G.n[2];
+        const double A0 = G.A[0], A1 = G.A[1], A2 = G.A[2];
+        long long lp = (static_cast<long long>(i0 + G.g) * G.np[1] + (j + G.g)) * G.np[2] + (k + G.g);
+        const long long sI = G.sl[0];
+        const long long fxs = static_cast<long long>(n1) * n2;
+        long long ox = static_cast<long long>(i0) * fxs + static_cast<long long>(j) * n2 + k;
+        long long oy = (static_cast<long long>(i0) * (n1 + 1) + j) * n2 + k;
+        const long long fys = static_cast<long long>(n1 + 1) * n2;
+        long long oz = (static_cast<long long>(i0) * n1 + j) * (n2 + 1) + k;
+        const long long fzs = static_cast<long long>(n1) * (n2 + 1);
+        double xlo = fx[ox];
+        for (int i = i0; i < i1; i++) {
+            double xhi;
+            if constexpr (NT) xhi = ld_nt(fx + ox + fxs);
+            else xhi = fx[ox + fxs];
+            const double ylo = fy[oy], yhi = fy[oy + n2], zlo = fz[oz], zhi = fz[oz + 1];
+            double v = A0 * xhi;
+            v -= A0 * xlo;
+            v += A1 * yhi;
+            v -= A1 * ylo;
+            v += A2 * zhi;
+            v -= A2 * zlo;
+            put<NT>(out + lp, v);
+            xlo = xhi;
+            lp += sI;
+            ox += fxs;
+            oy += fys;
+            oz += fzs;
+        }
+    });
 }
 
 __global__ void k_fill(double* __restrict__ p, long long n, double v) {
@@ -408,26 +493,57 @@ __global__ __launch_bounds__(256) void k_copy_to_field(Geo G, const double* __re
     for (int c = 0; c < G.N; c++) phi[lp * G.N + c] = x[nd * G.N + c];
 }
 
-// Launch of k_tvd_step3<NT, LIM> with as many blocks as stay resident at once (no tail
-// round, occupancy queried once per instantiation), each with at least kStepMinPlanes planes.
+// Balanced persistent schedule of the 3-D marching kernels: as many 256-thread blocks as stay
+// resident at once (per_cu, queried once per kernel), each with at least kStepMinPlanes planes.
+struct MarchPlan {
+    int tiles_k, per_xcd;
+    long long total, chunk;
+    unsigned grid;
+};
+
+inline MarchPlan march_plan(const dare_ctx* ctx, const Geo& G, int per_cu) {
+    MarchPlan P;
+    P.tiles_k = static_cast<int>(ceil_div(G.n[2], kStepTileK));
+    P.total = static_cast<long long>(P.tiles_k) * ceil_div(G.n[1], kStepTileJ) * G.n[0];
+    long long nb = static_cast<long long>(ctx->num_cu) * per_cu;
+    const long long by_work = ceil_div(P.total, kStepMinPlanes);
+    if (nb > by_work) nb = by_work;
+    P.per_xcd = static_cast<int>(ceil_div(nb, kStepXcd));
+    P.chunk = ceil_div(P.total, static_cast<long long>(P.per_xcd) * kStepXcd);
+    P.grid = static_cast<unsigned>(kStepXcd * P.per_xcd);
+    return P;
+}
+
+template <class K>
+int resident_per_cu(K kernel) {
+    int n = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0);
+    return (e == hipSuccess && n > 0) ? n : 4;
+}
+
 template <bool NT, int LIM>
 void launch_step3(dare_ctx* ctx, const Geo& G, const double* phi, const double* ux, const double* uy,
                   const double* uz, double fac, double* out) {
-    static int per_cu = 0;
-    if (per_cu == 0) {
-        int n = 0;
-        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tvd_step3<NT, LIM>, 256, 0);
-        per_cu = (e == hipSuccess && n > 0) ? n : 4;
-    }
-    const int tk = static_cast<int>(ceil_div(G.n[2], kStepTileK));
-    const long long total = static_cast<long long>(tk) * ceil_div(G.n[1], kStepTileJ) * G.n[0];
-    long long nb = static_cast<long long>(ctx->num_cu) * per_cu;
-    const long long by_work = ceil_div(total, kStepMinPlanes);
-    if (nb > by_work) nb = by_work;
-    const int per = static_cast<int>(ceil_div(nb, kStepXcd));
-    const long long chunk = ceil_div(total, static_cast<long long>(per) * kStepXcd);
-    hipLaunchKernelGGL((k_tvd_step3<NT, LIM>), dim3(kStepXcd * per), dim3(256), 0, ctx->stream, G, phi, ux, uy, uz,
-                       fac, out, tk, total, chunk, per);
+    static const int per_cu = resident_per_cu(k_tvd_step3<NT, LIM>);
+    const MarchPlan P = march_plan(ctx, G, per_cu);
+    hipLaunchKernelGGL((k_tvd_step3<NT, LIM>), dim3(P.grid), dim3(256), 0, ctx->stream, G, phi, ux, uy, uz, fac, out,
+                       P.tiles_k, P.total, P.chunk, P.per_xcd);
+}
+
+template <bool NT>
+void launch_grad3(dare_ctx* ctx, const Geo& G, const double* phi, double* fx, double* fy, double* fz) {
+    static const int per_cu = resident_per_cu(k_grad3<NT>);
+    const MarchPlan P = march_plan(ctx, G, per_cu);
+    hipLaunchKernelGGL((k_grad3<NT>), dim3(P.grid), dim3(256), 0, ctx->stream, G, phi, fx, fy, fz, P.tiles_k, P.total,
+                       P.chunk, P.per_xcd);
+}
+
+template <bool NT>
+void launch_div3(dare_ctx* ctx, const Geo& G, const double* fx, const double* fy, const double* fz, double* out) {
+    static const int per_cu = resident_per_cu(k_div3<NT>);
+    const MarchPlan P = march_plan(ctx, G, per_cu);
+    hipLaunchKernelGGL((k_div3<NT>), dim3(P.grid), dim3(256), 0, ctx->stream, G, fx, fy, fz, out, P.tiles_k, P.total,
+                       P.chunk, P.per_xcd);
 }
 
 template <bool NT>
@@ -513,6 +629,12 @@ int dare_grad_faces(dare_ctx* ctx, const dare_grid* g, const double* phi, double
     DARE_CHECK_ARG(G.g >= 1, "face gradient needs >= 1 ghost layer");
     DARE_CHECK_ARG(G.dim < 2 || fy, "fy is NULL");
     DARE_CHECK_ARG(G.dim < 3 || fz, "fz is NULL");
+    if (G.dim == 3 && G.N == 1) {
+        if (G.ncell_int >= kStreamRows) launch_grad3<true>(ctx, G, phi, fx, fy, fz);
+        else launch_grad3<false>(ctx, G, phi, fx, fy, fz);
+        DARE_LAUNCH_CHECK();
+        return DARE_OK;
+    }
     const FaceGeo F = make_face_geo(G);
     const Launch L = cell_launch(G);
     DARE_DISPATCH_DIM(G, k_grad_faces, L, G, F, phi, fx, fy, fz);
@@ -528,6 +650,12 @@ int dare_div_faces(dare_ctx* ctx, const dare_grid* g, const double* fx, const do
     if (rc) return rc;
     DARE_CHECK_ARG(G.dim < 2 || fy, "fy is NULL");
     DARE_CHECK_ARG(G.dim < 3 || fz, "fz is NULL");
+    if (G.dim == 3 && G.N == 1) {
+        if (G.ncell_int >= kStreamRows) launch_div3<true>(ctx, G, fx, fy, fz, out);
+        else launch_div3<false>(ctx, G, fx, fy, fz, out);
+        DARE_LAUNCH_CHECK();
+        return DARE_OK;
+    }
     const FaceGeo F = make_face_geo(G);
     const Launch L = cell_launch(G);
     DARE_DISPATCH_DIM(G, k_div_faces, L, G, F, fx, fy, fz, out);

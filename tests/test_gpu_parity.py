@@ -40,6 +40,9 @@ CASES = [
     dict(n=[9, 10, 11], ncomp=3, periodic=[1, 0, 1]),
     dict(n=[17, 15], ncomp=2, periodic=[1, 1]),
     dict(n=[23], ncomp=1, periodic=[1]),
+    # two k tiles (one partial), j not a multiple of 4: the marching 3-D kernels' column pieces
+    # then start and end inside columns and cross column boundaries
+    dict(n=[20, 70, 67], ncomp=1, periodic=[0, 1, 0]),
 ]
 
 
@@ -136,6 +139,12 @@ def assembly_case(rng, name):
     elif name == "periodic_conv":
         go = orc.grid([8, 9, 7], ghost=2, periodic=[1, 1, 1])
         kw = dict(dt=5e-3, diff=2e-3, limiter=3)
+    elif name == "periodic_even":     # even k extent: the paired-cell traversal, k wrap
+        go = orc.grid([8, 9, 10], ghost=2, periodic=[1, 1, 1])
+        kw = dict(dt=5e-3, diff=2e-3, limiter=4)
+    elif name == "wide_dirichlet":    # two 128-cell k tiles, the second partial
+        go = orc.grid([6, 5, 140], size=[0.5, 0.4, 1.0], ghost=2)
+        kw = dict(dt=1e-2, diff=1e-3, bc_type=[1, 2, 2, 1, 1, 1], bc_value=[0.5, 0, 0, 1.0, 0, -0.5])
     elif name == "react2":
         go = orc.grid([7, 8, 9], ghost=2, ncomp=2)
         kw = dict(dt=0.1, diff_comp=[1e-3, 5e-3], react=True, bc_type=[1] * 6,
@@ -272,7 +281,7 @@ def test_mixed_refinement_reaches_fp64_residual(ctx, rng, name):
     assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
 
 
-DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "react2"]
+DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "periodic_even", "wide_dirichlet", "react2"]
 
 
 def device_assembly(ctx, rng, name):
@@ -306,7 +315,7 @@ def test_diagonal_view_spmv_matches_oracle(ctx, rng, name):
     d = da.spmv_dot(ctx, A, dev(x), y, dev(w))
     exp = orc.spmv(rp, col, val, x)
     got = host(y)
-    if name == "periodic_conv":
+    if name.startswith("periodic"):
         assert np.max(np.abs(got - exp)) <= 1e-14 * np.max(np.abs(exp))
     else:
         assert bits_equal(got, exp)
