@@ -90,26 +90,31 @@ __device__ inline long long nb_col(const Geo& G, const int q[3], int pos) {
 enum { kRowStructure = 0, kRowValues = 1, kRowFromSlots = 2 };
 constexpr int kPosOfSlot[7] = {1, 3, 5, 0, 6, 4, 2};
 
-template <int MODE, bool SORT>
+// DIM = 0 takes the dimension from G at run time; DIM = 3 (the diagonal-view fill) lets every
+// per-axis loop unroll, so the per-position arrays live in registers.
+template <int MODE, bool SORT, int DIM = 0>
 __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, long long node,
                            int* cols, double* vals, double* rhs_out, double* x0_out,
                            double* slotv = nullptr, const double* slot_in = nullptr) {
     constexpr bool VALUES = MODE == kRowValues;
+    const int dim = DIM ? DIM : G.dim;
     double nbv[7] = {0., 0., 0., 0., 0., 0., 0.};
     double rhs = 0.;
     bool removed[7] = {false, false, false, false, false, false, false};
     double cross = 0.;
     const bool has_cross = T.cross != 0;
     long long lp = 0;
-    for (int d = 0; d < G.dim; d++) lp += G.sl[d] * (q[d] + G.g);
+    for (int d = 0; d < dim; d++) lp += G.sl[d] * (q[d] + G.g);
     const int N = G.N;
 
     // boundary faces hit by an edit (they only depend on the structure)
     int bc_face[3] = {-1, -1, -1};
-    for (int d = 0; d < G.dim; d++) {
+    for (int d = 0; d < dim; d++) {
         if (q[d] == 0 && T.bc_type[2 * d] != 0) bc_face[d] = 2 * d;
         else if (q[d] == G.n[d] - 1 && T.bc_type[2 * d + 1] != 0) bc_face[d] = 2 * d + 1;
-        if (bc_face[d] >= 0) removed[bc_face[d] + 1] = true;
+        // position bc_face + 1, written with constant indices
+        if (bc_face[d] == 2 * d) removed[2 * d + 1] = true;
+        else if (bc_face[d] == 2 * d + 1) removed[2 * d + 2] = true;
     }
 
     if (VALUES) {
@@ -121,7 +126,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
         }
         {   // mb -= div(D_f * grad(mb))
             double o[7] = {0., 0., 0., 0., 0., 0., 0.};
-            for (int d = 0; d < G.dim; d++) {
+            for (int d = 0; d < dim; d++) {
                 double Df[2];
                 for (int h = 0; h < 2; h++) {
                     if (T.diff_cell) {
@@ -143,15 +148,15 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
                 o[2 * d + 1] = -G.A[d] * nbW;
                 o[0] -= G.A[d] * cW;
             }
-            for (int pos = 0; pos < 2 * G.dim + 1; pos++) nbv[pos] += o[pos] * -1.;
+            for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += o[pos] * -1.;
             rhs += 0. * -1.;
         }
         if (T.conv_limiter >= 0) {                         // mb += div(tvd * phi)
             double o[7] = {0., 0., 0., 0., 0., 0., 0.};
             double orhs = 0.;
-            for (int d = 0; d < G.dim; d++) {
+            for (int d = 0; d < dim; d++) {
                 long long fi = 0;
-                for (int e = 0; e < G.dim; e++) fi += T.fs[d][e] * q[e];
+                for (int e = 0; e < dim; e++) fi += T.fs[d][e] * q[e];
                 const double u[2] = {T.vel[d][fi], T.vel[d][fi + T.fs[d][d]]};
                 double nb[2], cc[2], fr[2];
                 const long long s = G.sl[d] * N;
@@ -188,7 +193,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
                 if (d == 0) orhs = G.A[0] * (fr[1] - fr[0]);
                 else orhs += G.A[d] * (fr[1] - fr[0]);
             }
-            for (int pos = 0; pos < 2 * G.dim + 1; pos++) nbv[pos] += o[pos];
+            for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += o[pos];
             rhs += orhs;
         }
         if (T.source) rhs += G.V * T.source[lp * N + c];
@@ -197,15 +202,15 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
             nbv[0] -= G.V * R[c * 2 + c];
             cross = -(G.V * R[c * 2 + (1 - c)]);
         }
-        for (int d = 0; d < G.dim; d++) {                  // boundary edits
+        for (int d = 0; d < dim; d++) {                  // boundary edits
             const int f = bc_face[d];
             if (f < 0) continue;
-            const int pos = f + 1;
+            const double nbf = (f == 2 * d) ? nbv[2 * d + 1] : nbv[2 * d + 2];   // nbv[f + 1]
             if (T.bc_type[f] == DARE_BC_DIRICHLET) {
-                nbv[0] -= nbv[pos];
-                rhs -= (2. * T.bc_value[f]) * nbv[pos];
+                nbv[0] -= nbf;
+                rhs -= (2. * T.bc_value[f]) * nbf;
             } else {
-                nbv[0] += nbv[pos];
+                nbv[0] += nbf;
             }
         }
         *rhs_out = rhs;
@@ -213,7 +218,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
         if (slotv) {      // diagonal view: W,S,B,C,T,N,E,cross; removed entries hold 0
             for (int sl = 0; sl < 7; sl++) {
                 const int pos = kPosOfSlot[sl];
-                slotv[sl] = (pos <= 2 * G.dim && !removed[pos]) ? nbv[pos] : 0.;
+                slotv[sl] = (pos <= 2 * dim && !removed[pos]) ? nbv[pos] : 0.;
             }
             slotv[7] = cross;
         }
@@ -236,7 +241,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
     const int order[7] = {1, 3, 5, 0, 6, 4, 2};
     for (int j = 0; j < 7; j++) {
         const int pos = order[j];
-        if (pos > 2 * G.dim || removed[pos]) continue;
+        if (pos > 2 * dim || removed[pos]) continue;
         kc[k] = nb_col(G, q, pos) * N + c;
         kv[k] = nbv[pos];
         k++;
@@ -413,7 +418,7 @@ __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __rest
         const long long row = node * G.N + c;
         double slotv[kDiaSlots];
         double r = 0., x = 0.;
-        row_entries<kRowValues, false>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
+        row_entries<kRowValues, false, 3>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
         for (int sl = 0; sl < slots; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
         if constexpr (F32)
             for (int sl = 0; sl < slots; sl++) put<NT>(dia32 + sl * nrows + row, static_cast<float>(slotv[sl]));
