@@ -386,18 +386,130 @@ __device__ __forceinline__ void march_columns(const Geo& G, int tiles_k, long lo
     }
 }
 
-// 3-D, one component, persistent and balanced TVD step: march_columns hands out the
-// column-plane ranges; each XCD's contiguous run of columns shares its j-halo rows in its L2.
-// The limiter is a template parameter so that each instantiation carries only its own branch.
+// LDS-staged march of one full 64 (k) x 4 (j) tile (all 256 threads active, so the block can
+// synchronise). Plane i of phi is staged with two ghost rows and columns (rows j0-2..j0+5,
+// columns k0-2..k0+65), each y and z face flux of the plane is computed once, by one lane, and
+// shared through LDS (the face between rows j-1 and j by the lane of cell j; the top face row
+// by wave 3; the right face column by lanes 0..3 of wave 0), and every cell combines its six
+// fluxes in the order of tvd_march3. Each flux has the same four cells and velocity as there,
+// so the result is bit-identical with about two thirds of the flux evaluations and five loads
+// per cell and plane instead of fourteen. The x faces stay in registers as in tvd_march3.
+constexpr int kTvdRow = 68;   // staged row: 2 + 64 + 2 doubles
+template <bool NT>
+__device__ __forceinline__ void tvd_march3_lds(const Geo& G, int lim, const double* __restrict__ phi,
+                                               const double* __restrict__ ux, const double* __restrict__ uy,
+                                               const double* __restrict__ uz, double fac, double* __restrict__ out,
+                                               int k0, int j0, int i0, int i1, double* T, double* Fy, double* Fz) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = k0 + lane, j = j0 + w;
+    const int n1 = G.n[1], n2 = G.n[2];
+    const long long sI = G.sl[0], sJ = G.sl[1];
+    long long lp = (static_cast<long long>(i0 + G.g) * G.np[1] + (j + G.g)) * G.np[2] + (k + G.g);
+    const long long fxs = static_cast<long long>(n1) * n2;
+    long long fx = static_cast<long long>(i0) * fxs + static_cast<long long>(j) * n2 + k;
+    long long fy = (static_cast<long long>(i0) * (n1 + 1) + j) * n2 + k;
+    const long long fys = static_cast<long long>(n1 + 1) * n2;
+    long long fz = (static_cast<long long>(i0) * n1 + j) * (n2 + 1) + k;
+    const long long fzs = static_cast<long long>(n1) * (n2 + 1);
+    // ghost row staged by this wave: j0-2, j0-1 (waves 0, 1: two rows below their own) and
+    // j0+4, j0+5 (waves 2, 3: two rows above)
+    const long long hoff = (w < 2 ? -2 : 2) * sJ;
+    const int hrow = w < 2 ? w : w + 4;
+    // ghost columns (wave 1, lanes 0..15): rows j0..j0+3 x columns k0-2, k0-1, k0+64, k0+65
+    const int gr = (lane >> 2) & 3, gq = lane & 3;
+    const int gdk = gq < 2 ? gq - 2 : 62 + gq;                  // -2, -1, 64, 65
+    const long long goff = (gr - w) * sJ + (gdk - lane);        // from this lane's cell
+    const int gslot = (2 + gr) * kTvdRow + (gdk + 2);
+    const bool gload = w == 1 && lane < 16;
+    // right face column (wave 0, lanes 0..3): row j0+lane, face k0+64
+    const long long zx_off = (static_cast<long long>(lane) - w) * (n2 + 1) + (64 - lane);
+    double pm2 = phi[lp - 2 * sI], pm1 = phi[lp - sI], p0 = phi[lp], pp1 = phi[lp + sI];
+    double f_lo = tvd_flux4(pm2, pm1, p0, pp1, lim, ux[fx]);
+    const double A0 = G.A[0], A1 = G.A[1], A2 = G.A[2];
+    double* Trow = T + (2 + w) * kTvdRow + 2 + lane;
+    for (int i = i0; i < i1; i++) {
+        const double pp2 = phi[lp + 2 * sI];
+        double u_hi;
+        if constexpr (NT) u_hi = ld_nt(ux + fx + fxs);
+        else u_hi = ux[fx + fxs];
+        const double hal = phi[lp + hoff];
+        const double vy = uy[fy], wz = uz[fz];
+        const double vy4 = w == 3 ? uy[fy + n2] : 0.;
+        const double wz4 = (w == 0 && lane < 4) ? uz[fz + zx_off] : 0.;
+        const double gv = gload ? phi[lp + goff] : 0.;
+        Trow[0] = p0;
+        T[hrow * kTvdRow + 2 + lane] = hal;
+        if (gload) T[gslot] = gv;
+        __syncthreads();
+        const double f_hi = tvd_flux4(pm1, p0, pp1, pp2, lim, u_hi);
+        const double* Tc = T + w * kTvdRow + 2 + lane;             // column k, rows j-2 ..
+        const double gy_lo = tvd_flux4(Tc[0], Tc[kTvdRow], Tc[2 * kTvdRow], Tc[3 * kTvdRow], lim, vy);
+        const double* Tr = T + (2 + w) * kTvdRow + lane;           // row j, columns k-2 ..
+        const double hz_lo = tvd_flux4(Tr[0], Tr[1], Tr[2], Tr[3], lim, wz);
+        Fy[w * 64 + lane] = gy_lo;
+        Fz[w * 65 + lane] = hz_lo;
+        if (w == 3) {
+            const double* Tt = T + 4 * kTvdRow + 2 + lane;
+            Fy[4 * 64 + lane] = tvd_flux4(Tt[0], Tt[kTvdRow], Tt[2 * kTvdRow], Tt[3 * kTvdRow], lim, vy4);
+        }
+        if (w == 0 && lane < 4) {
+            const double* Te = T + (2 + lane) * kTvdRow + 64;
+            Fz[lane * 65 + 64] = tvd_flux4(Te[0], Te[1], Te[2], Te[3], lim, wz4);
+        }
+        __syncthreads();
+        const double gy_hi = Fy[(w + 1) * 64 + lane];
+        const double hz_hi = Fz[w * 65 + lane + 1];
+        double v = A0 * f_hi;
+        v -= A0 * f_lo;
+        v += A1 * gy_hi;
+        v -= A1 * gy_lo;
+        v += A2 * hz_hi;
+        v -= A2 * hz_lo;
+        put<NT>(out + lp, p0 - fac * v);
+        pm2 = pm1;
+        pm1 = p0;
+        p0 = pp1;
+        pp1 = pp2;
+        f_lo = f_hi;
+        lp += sI;
+        fx += fxs;
+        fy += fys;
+        fz += fzs;
+    }
+}
+
+// 3-D, one component, persistent and balanced TVD step. Logical block L takes the column-plane
+// range [L*chunk, (L+1)*chunk) (the walk of march_columns); a piece of a full tile marches
+// with the LDS-staged form, a piece of an edge tile (k or j extent cut by the grid) with the
+// register form, each lane on its own. The limiter is a template parameter so that each
+// instantiation carries only its own branch.
 template <bool NT, int LIM>
 __global__ __launch_bounds__(256) void k_tvd_step3(Geo G, const double* __restrict__ phi,
                                                    const double* __restrict__ ux, const double* __restrict__ uy,
                                                    const double* __restrict__ uz, double fac,
                                                    double* __restrict__ out, int tiles_k, long long total,
                                                    long long chunk, int per_xcd) {
-    march_columns(G, tiles_k, total, chunk, per_xcd, [&](int k, int j, int i0, int i1) {
-        tvd_march3<NT>(G, LIM, phi, ux, uy, uz, fac, out, k, j, i0, i1);
-    });
+    __shared__ double T[8 * kTvdRow];
+    __shared__ double Fy[5 * 64];
+    __shared__ double Fz[4 * 65];
+    const long long L = static_cast<long long>(blockIdx.x % kStepXcd) * per_xcd + blockIdx.x / kStepXcd;
+    const int n0 = G.n[0];
+    long long c = L * chunk;
+    const long long cend = (c + chunk < total) ? c + chunk : total;
+    while (c < cend) {
+        const long long col = c / n0;
+        const int i0 = static_cast<int>(c - col * n0);
+        const int i1 = (cend - c < n0 - i0) ? i0 + static_cast<int>(cend - c) : n0;
+        const int k0 = static_cast<int>(col % tiles_k) * kStepTileK;
+        const int j0 = static_cast<int>(col / tiles_k) * kStepTileJ;
+        if (k0 + kStepTileK <= G.n[2] && j0 + kStepTileJ <= G.n[1]) {      // block-uniform
+            tvd_march3_lds<NT>(G, LIM, phi, ux, uy, uz, fac, out, k0, j0, i0, i1, T, Fy, Fz);
+        } else {
+            const int k = k0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
+            if (k < G.n[2] && j < G.n[1]) tvd_march3<NT>(G, LIM, phi, ux, uy, uz, fac, out, k, j, i0, i1);
+        }
+        c += i1 - i0;
+    }
 }
 
 // 3-D, one component: face gradient (same expressions as k_grad_faces) marching along i with
