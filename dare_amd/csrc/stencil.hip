@@ -386,20 +386,22 @@ __device__ __forceinline__ void march_columns(const Geo& G, int tiles_k, long lo
     }
 }
 
-// LDS-staged march of one full 64 (k) x 4 (j) tile (all 256 threads active, so the block can
-// synchronise). Plane i of phi is staged with two ghost rows and columns (rows j0-2..j0+5,
+// LDS-staged march of one full 64 (k) x TJ (j) tile (all 64 TJ threads active, so the block can
+// synchronise). Plane i of phi is staged with two ghost rows and columns (rows j0-2..j0+TJ+1,
 // columns k0-2..k0+65), each y and z face flux of the plane is computed once, by one lane, and
 // shared through LDS (the face between rows j-1 and j by the lane of cell j; the top face row
-// by wave 3; the right face column by lanes 0..3 of wave 0), and every cell combines its six
-// fluxes in the order of tvd_march3. Each flux has the same four cells and velocity as there,
-// so the result is bit-identical with about two thirds of the flux evaluations and five loads
-// per cell and plane instead of fourteen. The x faces stay in registers as in tvd_march3.
+// by the last wave; the right face column by lanes 0..TJ-1 of wave 0), and every cell combines
+// its six fluxes in the order of tvd_march3. Each flux has the same four cells and velocity as
+// there, so the result is bit-identical with about (3 + 2/TJ) / 5 of the flux evaluations and
+// five loads per cell and plane instead of fourteen. The x faces stay in registers as in
+// tvd_march3. Waves 0, 1 stage the two ghost rows below the tile, waves 2, 3 the two above.
 constexpr int kTvdRow = 68;   // staged row: 2 + 64 + 2 doubles
-template <bool NT>
+template <bool NT, int TJ>
 __device__ __forceinline__ void tvd_march3_lds(const Geo& G, int lim, const double* __restrict__ phi,
                                                const double* __restrict__ ux, const double* __restrict__ uy,
                                                const double* __restrict__ uz, double fac, double* __restrict__ out,
                                                int k0, int j0, int i0, int i1, double* T, double* Fy, double* Fz) {
+    static_assert(TJ >= 4 && TJ <= 16, "ghost rows are staged by waves 0..3, ghost columns by wave 1");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int k = k0 + lane, j = j0 + w;
     const int n1 = G.n[1], n2 = G.n[2];
@@ -411,17 +413,18 @@ __device__ __forceinline__ void tvd_march3_lds(const Geo& G, int lim, const doub
     const long long fys = static_cast<long long>(n1 + 1) * n2;
     long long fz = (static_cast<long long>(i0) * n1 + j) * (n2 + 1) + k;
     const long long fzs = static_cast<long long>(n1) * (n2 + 1);
-    // ghost row staged by this wave: j0-2, j0-1 (waves 0, 1: two rows below their own) and
-    // j0+4, j0+5 (waves 2, 3: two rows above)
-    const long long hoff = (w < 2 ? -2 : 2) * sJ;
-    const int hrow = w < 2 ? w : w + 4;
-    // ghost columns (wave 1, lanes 0..15): rows j0..j0+3 x columns k0-2, k0-1, k0+64, k0+65
-    const int gr = (lane >> 2) & 3, gq = lane & 3;
+    // ghost row staged by waves 0..3: j0-2, j0-1, j0+TJ, j0+TJ+1 (staged rows 0, 1, TJ+2, TJ+3)
+    const bool hload = w < 4;
+    const int hj = w < 2 ? w - 2 : TJ + w - 2;                  // tile-relative row
+    const long long hoff = static_cast<long long>(hj - w) * sJ;
+    const int hrow = hj + 2;
+    // ghost columns (wave 1, lanes 0..4 TJ-1): rows j0..j0+TJ-1 x columns k0-2, k0-1, k0+64, k0+65
+    const int gr = lane >> 2, gq = lane & 3;
     const int gdk = gq < 2 ? gq - 2 : 62 + gq;                  // -2, -1, 64, 65
-    const long long goff = (gr - w) * sJ + (gdk - lane);        // from this lane's cell
+    const long long goff = static_cast<long long>(gr - w) * sJ + (gdk - lane);   // from this lane's cell
     const int gslot = (2 + gr) * kTvdRow + (gdk + 2);
-    const bool gload = w == 1 && lane < 16;
-    // right face column (wave 0, lanes 0..3): row j0+lane, face k0+64
+    const bool gload = w == 1 && lane < 4 * TJ;
+    // right face column (wave 0, lanes 0..TJ-1): row j0+lane, face k0+64
     const long long zx_off = (static_cast<long long>(lane) - w) * (n2 + 1) + (64 - lane);
     double pm2 = phi[lp - 2 * sI], pm1 = phi[lp - sI], p0 = phi[lp], pp1 = phi[lp + sI];
     double f_lo = tvd_flux4(pm2, pm1, p0, pp1, lim, ux[fx]);
@@ -432,13 +435,13 @@ __device__ __forceinline__ void tvd_march3_lds(const Geo& G, int lim, const doub
         double u_hi;
         if constexpr (NT) u_hi = ld_nt(ux + fx + fxs);
         else u_hi = ux[fx + fxs];
-        const double hal = phi[lp + hoff];
+        const double hal = hload ? phi[lp + hoff] : 0.;
         const double vy = uy[fy], wz = uz[fz];
-        const double vy4 = w == 3 ? uy[fy + n2] : 0.;
-        const double wz4 = (w == 0 && lane < 4) ? uz[fz + zx_off] : 0.;
+        const double vyt = w == TJ - 1 ? uy[fy + n2] : 0.;
+        const double wzr = (w == 0 && lane < TJ) ? uz[fz + zx_off] : 0.;
         const double gv = gload ? phi[lp + goff] : 0.;
         Trow[0] = p0;
-        T[hrow * kTvdRow + 2 + lane] = hal;
+        if (hload) T[hrow * kTvdRow + 2 + lane] = hal;
         if (gload) T[gslot] = gv;
         __syncthreads();
         const double f_hi = tvd_flux4(pm1, p0, pp1, pp2, lim, u_hi);
@@ -448,13 +451,13 @@ __device__ __forceinline__ void tvd_march3_lds(const Geo& G, int lim, const doub
         const double hz_lo = tvd_flux4(Tr[0], Tr[1], Tr[2], Tr[3], lim, wz);
         Fy[w * 64 + lane] = gy_lo;
         Fz[w * 65 + lane] = hz_lo;
-        if (w == 3) {
-            const double* Tt = T + 4 * kTvdRow + 2 + lane;
-            Fy[4 * 64 + lane] = tvd_flux4(Tt[0], Tt[kTvdRow], Tt[2 * kTvdRow], Tt[3 * kTvdRow], lim, vy4);
+        if (w == TJ - 1) {                                        // top face row j0+TJ
+            const double* Tt = T + TJ * kTvdRow + 2 + lane;
+            Fy[TJ * 64 + lane] = tvd_flux4(Tt[0], Tt[kTvdRow], Tt[2 * kTvdRow], Tt[3 * kTvdRow], lim, vyt);
         }
-        if (w == 0 && lane < 4) {
+        if (w == 0 && lane < TJ) {                                // right face column k0+64
             const double* Te = T + (2 + lane) * kTvdRow + 64;
-            Fz[lane * 65 + 64] = tvd_flux4(Te[0], Te[1], Te[2], Te[3], lim, wz4);
+            Fz[lane * 65 + 64] = tvd_flux4(Te[0], Te[1], Te[2], Te[3], lim, wzr);
         }
         __syncthreads();
         const double gy_hi = Fy[(w + 1) * 64 + lane];
@@ -478,20 +481,22 @@ __device__ __forceinline__ void tvd_march3_lds(const Geo& G, int lim, const doub
     }
 }
 
-// 3-D, one component, persistent and balanced TVD step. Logical block L takes the column-plane
-// range [L*chunk, (L+1)*chunk) (the walk of march_columns); a piece of a full tile marches
-// with the LDS-staged form, a piece of an edge tile (k or j extent cut by the grid) with the
-// register form, each lane on its own. The limiter is a template parameter so that each
-// instantiation carries only its own branch.
+// 3-D, one component, persistent and balanced TVD step over 64 (k) x kStepTvdJ (j) tiles.
+// Logical block L takes the column-plane range [L*chunk, (L+1)*chunk) (the walk of
+// march_columns); a piece of a full tile marches with the LDS-staged form, a piece of an edge
+// tile (k or j extent cut by the grid) with the register form, each lane on its own. The
+// limiter is a template parameter so that each instantiation carries only its own branch.
+constexpr int kStepTvdJ = 8;
 template <bool NT, int LIM>
-__global__ __launch_bounds__(256) void k_tvd_step3(Geo G, const double* __restrict__ phi,
-                                                   const double* __restrict__ ux, const double* __restrict__ uy,
-                                                   const double* __restrict__ uz, double fac,
-                                                   double* __restrict__ out, int tiles_k, long long total,
-                                                   long long chunk, int per_xcd) {
-    __shared__ double T[8 * kTvdRow];
-    __shared__ double Fy[5 * 64];
-    __shared__ double Fz[4 * 65];
+__global__ __launch_bounds__(64 * kStepTvdJ) void k_tvd_step3(Geo G, const double* __restrict__ phi,
+                                                              const double* __restrict__ ux,
+                                                              const double* __restrict__ uy,
+                                                              const double* __restrict__ uz, double fac,
+                                                              double* __restrict__ out, int tiles_k, long long total,
+                                                              long long chunk, int per_xcd) {
+    __shared__ double T[(kStepTvdJ + 4) * kTvdRow];
+    __shared__ double Fy[(kStepTvdJ + 1) * 64];
+    __shared__ double Fz[kStepTvdJ * 65];
     const long long L = static_cast<long long>(blockIdx.x % kStepXcd) * per_xcd + blockIdx.x / kStepXcd;
     const int n0 = G.n[0];
     long long c = L * chunk;
@@ -501,9 +506,9 @@ __global__ __launch_bounds__(256) void k_tvd_step3(Geo G, const double* __restri
         const int i0 = static_cast<int>(c - col * n0);
         const int i1 = (cend - c < n0 - i0) ? i0 + static_cast<int>(cend - c) : n0;
         const int k0 = static_cast<int>(col % tiles_k) * kStepTileK;
-        const int j0 = static_cast<int>(col / tiles_k) * kStepTileJ;
-        if (k0 + kStepTileK <= G.n[2] && j0 + kStepTileJ <= G.n[1]) {      // block-uniform
-            tvd_march3_lds<NT>(G, LIM, phi, ux, uy, uz, fac, out, k0, j0, i0, i1, T, Fy, Fz);
+        const int j0 = static_cast<int>(col / tiles_k) * kStepTvdJ;
+        if (k0 + kStepTileK <= G.n[2] && j0 + kStepTvdJ <= G.n[1]) {      // block-uniform
+            tvd_march3_lds<NT, kStepTvdJ>(G, LIM, phi, ux, uy, uz, fac, out, k0, j0, i0, i1, T, Fy, Fz);
         } else {
             const int k = k0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
             if (k < G.n[2] && j < G.n[1]) tvd_march3<NT>(G, LIM, phi, ux, uy, uz, fac, out, k, j, i0, i1);
@@ -613,10 +618,10 @@ struct MarchPlan {
     unsigned grid;
 };
 
-inline MarchPlan march_plan(const dare_ctx* ctx, const Geo& G, int per_cu) {
+inline MarchPlan march_plan(const dare_ctx* ctx, const Geo& G, int per_cu, int tile_j = kStepTileJ) {
     MarchPlan P;
     P.tiles_k = static_cast<int>(ceil_div(G.n[2], kStepTileK));
-    P.total = static_cast<long long>(P.tiles_k) * ceil_div(G.n[1], kStepTileJ) * G.n[0];
+    P.total = static_cast<long long>(P.tiles_k) * ceil_div(G.n[1], tile_j) * G.n[0];
     long long nb = static_cast<long long>(ctx->num_cu) * per_cu;
     const long long by_work = ceil_div(P.total, kStepMinPlanes);
     if (nb > by_work) nb = by_work;
@@ -627,19 +632,19 @@ inline MarchPlan march_plan(const dare_ctx* ctx, const Geo& G, int per_cu) {
 }
 
 template <class K>
-int resident_per_cu(K kernel) {
+int resident_per_cu(K kernel, int threads = 256) {
     int n = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0);
-    return (e == hipSuccess && n > 0) ? n : 4;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0);
+    return (e == hipSuccess && n > 0) ? n : 2;
 }
 
 template <bool NT, int LIM>
 void launch_step3(dare_ctx* ctx, const Geo& G, const double* phi, const double* ux, const double* uy,
                   const double* uz, double fac, double* out) {
-    static const int per_cu = resident_per_cu(k_tvd_step3<NT, LIM>);
-    const MarchPlan P = march_plan(ctx, G, per_cu);
-    hipLaunchKernelGGL((k_tvd_step3<NT, LIM>), dim3(P.grid), dim3(256), 0, ctx->stream, G, phi, ux, uy, uz, fac, out,
-                       P.tiles_k, P.total, P.chunk, P.per_xcd);
+    static const int per_cu = resident_per_cu(k_tvd_step3<NT, LIM>, 64 * kStepTvdJ);
+    const MarchPlan P = march_plan(ctx, G, per_cu, kStepTvdJ);
+    hipLaunchKernelGGL((k_tvd_step3<NT, LIM>), dim3(P.grid), dim3(64 * kStepTvdJ), 0, ctx->stream, G, phi, ux, uy, uz,
+                       fac, out, P.tiles_k, P.total, P.chunk, P.per_xcd);
 }
 
 template <bool NT>

@@ -5,6 +5,7 @@
 // Kokkos::DualView, a host copy and a device copy are kept with explicit synchronisation.
 #pragma once
 
+#include <array>
 #include <memory>
 #include <string>
 #include <vector>
@@ -38,6 +39,10 @@ public:
     SC operator[](std::size_t i) const { return host_[i]; }
     std::size_t GetSize() const { return host_.size(); }
     std::size_t GetNumEquations() const { return N; }
+    std::size_t GetNumComponents() const { return N; }
+    // component names for output (GridVector.h: SetComponentName / GetComponentName)
+    void SetComponentName(std::size_t c, std::string name) { cnames_[c] = std::move(name); }
+    const std::string& GetComponentName(std::size_t c) const { return cnames_[c]; }
     const GridRepresentation& GetGridRepresentation() const { return grep_; }
     const std::string& GetName() const { return name_; }
     void SetValues(SC v) {
@@ -79,6 +84,7 @@ private:
     DeviceBuffer<SC> dev_;
     bool host_dirty_ = false;
     bool device_dirty_ = false;
+    std::array<std::string, N> cnames_{};
 };
 
 template <typename Grid, typename SC, std::size_t N>

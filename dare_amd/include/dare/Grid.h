@@ -6,6 +6,7 @@
 #include <array>
 #include <cstdint>
 #include <stdexcept>
+#include <string>
 
 #include "dare_hip.h"
 
@@ -44,6 +45,9 @@ public:
     }
 
     Representation GetRepresentation() const { return Representation(this); }
+    // the grid's name (CartesianMesh.h: the first constructor argument); output groups data by it
+    const std::string& GetName() const { return name_; }
+    void SetName(std::string name) { name_ = std::move(name); }
     const VecSC& GetCellWidth() const { return width_; }
     const VecSC& GetFaceArea() const { return area_; }
     double GetCellVolume() const { return volume_; }
@@ -63,6 +67,7 @@ private:
     double volume_ = 1.;
     int32_t num_ghost_;
     VecLO periodic_;
+    std::string name_ = "Cartesian" + std::to_string(Dim) + "D";
 };
 
 template <std::size_t Dim>

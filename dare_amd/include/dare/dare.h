@@ -4,3 +4,4 @@
 #include "Grid.h"
 #include "Data.h"
 #include "MatrixSystem.h"
+#include "IO.h"

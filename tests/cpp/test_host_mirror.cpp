@@ -5,6 +5,9 @@
 //   SolveScalar1D     — test/test_integration_Cartesian_Trilinos.cpp:82-160
 //   HaloExchange      — MPI/test/test_HaloBuffer.cpp:33-115 (one rank, periodic)
 //   DeviceDiffusion3D — counter_diffusion pattern on the device fast path vs the heat kernel
+//   VTKWrite          — IO/test/test_VTKWriter.cpp pattern: a field changed on the device
+//                       (periodic ghost exchange) written through dare::io::VTKWriter into
+//                       argv[1] (checked by tests/test_host_mirror.py)
 // Exit code 0 when every case passes.
 #include <cmath>
 #include <cstdio>
@@ -247,10 +250,33 @@ static void DeviceDiffusion3D() {
     std::printf("  DeviceDiffusion3D: max error vs heat kernel %.2e\n", err);
 }
 
-int main() {
+static std::string g_vtk_dir;
+
+static void VTKWrite() {
+    if (g_vtk_dir.empty()) return;
+    using G = dare::Grid::Cartesian<3>;
+    G grid({6, 5, 4}, {1.2, 1., 0.8}, 2, {1, 1, 1});
+    grid.SetName("mirror3D");
+    auto grep = grid.GetRepresentation();
+    Data::Field<G, double, 2> field("conc", grep, 1);
+    auto& v = field.GetDataVector();
+    v.SetComponentName(0, "a");
+    v.SetComponentName(1, "b");
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 5; j++)
+            for (int k = 0; k < 4; k++)
+                for (int c = 0; c < 2; c++)
+                    v.At(grep.MapInternalToLocal(std::array<int32_t, 3>{i, j, k}), c) = 1e6 * i + 1e3 * j + k + 0.125 * c;
+    field.ExchangeHaloCells();   // periodic ghosts filled on the device only
+    dare::io::VTKWriter<G> writer(0.5, 4);
+    EXPECT(writer.Write(g_vtk_dir, &v), "VTK write to %s failed", g_vtk_dir.c_str());
+}
+
+int main(int argc, char** argv) {
+    if (argc > 1) g_vtk_dir = argv[1];
     const std::vector<std::pair<const char*, std::function<void()>>> cases = {
         {"TrilinosBuild", TrilinosBuild}, {"SolveLaplace", SolveLaplace}, {"SolveScalar1D", SolveScalar1D},
-        {"HaloExchange", HaloExchange}, {"DeviceDiffusion3D", DeviceDiffusion3D}};
+        {"HaloExchange", HaloExchange}, {"DeviceDiffusion3D", DeviceDiffusion3D}, {"VTKWrite", VTKWrite}};
     for (const auto& c : cases) {
         const int before = g_fail;
         std::printf("[ RUN ] %s\n", c.first);

@@ -38,10 +38,29 @@ def test_driver_builds():
 
 
 @pytest.mark.gpu
-def test_host_mirror_driver():
+def test_host_mirror_driver(tmp_path):
     assert os.access(BIN, os.X_OK), "tests/cpp/test_host_mirror is not built (run __graft_entry__.build())"
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([BIN, str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    for case in ("TrilinosBuild", "SolveLaplace", "SolveScalar1D", "HaloExchange", "DeviceDiffusion3D"):
+    for case in ("TrilinosBuild", "SolveLaplace", "SolveScalar1D", "HaloExchange", "DeviceDiffusion3D", "VTKWrite"):
         assert f"[ OK ] {case}" in r.stdout
+    # the VTK file of a field whose periodic ghosts were filled on the device: internal cells
+    # and face ghosts hold the internal cell they wrap to; edge and corner ghosts (two or more
+    # indices outside) are not exchanged (HaloBuffer face ghosts only) and keep their 0
+    from test_vtk_writer import read_vts
+    import numpy as np
+    head, arrays, ext, _ = read_vts(str(tmp_path / "mirror3D_4" / "mirror3D_4_0.vts"))
+    n, g = [6, 5, 4], 2
+    assert ext == [0, 10, 0, 9, 0, 8]
+    # the Field names its time level 0 "conc_0"
+    vals = arrays["conc_0"][1].reshape(n[2] + 2 * g, n[1] + 2 * g, n[0] + 2 * g, 2)
+    for k in range(n[2] + 2 * g):
+        for j in range(n[1] + 2 * g):
+            for i in range(n[0] + 2 * g):
+                outside = sum(not (g <= q < g + m) for q, m in ((i, n[0]), (j, n[1]), (k, n[2])))
+                wi, wj, wk = (i - g) % n[0], (j - g) % n[1], (k - g) % n[2]
+                for c in range(2):
+                    exp = 1e6 * wi + 1e3 * wj + wk + 0.125 * c if outside <= 1 else 0.0
+                    assert vals[k, j, i, c] == exp, (i, j, k, c)
+    assert arrays["TimeValue"][1].tolist() == [0.5]
