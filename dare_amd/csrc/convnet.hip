@@ -64,9 +64,9 @@ __device__ __host__ inline float bf2f(uint16_t h) {
 // (K = 32 per instruction). Operand layout: lane l holds A[l % 16][8 (l / 16) + j] and
 // B[8 (l / 16) + j][l % 16] (j = 0..7, 16 contiguous bytes each: eight channels of one output
 // channel's weights / of one voxel), and D[4 (l / 16) + j][l % 16] (j = 0..3: four consecutive
-// output channels of one voxel, stored as one 8-byte write). K runs over 32-channel slices of
-// one tap, or, for 16-channel inputs, over two taps at once. A wave owns NSEG x 16 voxels of one
-// (z, y) row, so each weight fragment feeds NSEG instructions; a block is 4 such rows.
+// output channels of one voxel, stored as one 8-byte write). K runs over two taps x 16 input
+// channels. A wave owns NSEG x 16 voxels of one (z, y) row, so each weight fragment feeds NSEG
+// instructions; a block is 4 such rows and stages their input in LDS.
 // OUT: 0 BF16 all channels, 1 FP32 channel 0 only.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -76,73 +76,266 @@ __device__ inline bf16x8 load8(const uint16_t* p, bool ok) {
     return __builtin_bit_cast(bf16x8, u);
 }
 
+// Output-channel blocks of 16 per work-group (a group stages its slice of the weights).
+template <int COUT>
+constexpr int conv_nbb() { return COUT / 16 < 2 ? COUT / 16 : 2; }
+
 template <int CIN, int COUT, bool RELU, int OUT, int NSEG>
 __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
                                                      uint16_t* __restrict__ Y, float* __restrict__ Yf, int D, int H,
                                                      int W) {
+    // Block tile: 16 NSEG voxels along x, 4 rows along y (one per wave), one plane z, and
+    // 16 NBB output channels (blockIdx.y = row block * NGRP + channel group). Per 16-channel
+    // slice of the input, the block stages in LDS the input it needs -- (16 NSEG + 2) x 6 x 3
+    // voxels, zero outside the box -- as [half][voxel] (16 bytes = 8 channels per voxel), and
+    // its weights as [half][tap][co] (16 bytes each). Every MFMA then reads both operands from
+    // LDS: a staged input value feeds up to 27 taps x NBB instructions. All global loads of a
+    // stage are issued before it is written, in batches of eight per thread.
+    // K = 32 is two taps x 16 channels (the 28th tap is zero).
+    constexpr int SX = 16 * NSEG + 2, SY = 6, NV = SX * SY * 3;
+    constexpr int NBB = conv_nbb<COUT>(), NGRP = COUT / (16 * NBB);
+    constexpr int NWP = 2 * 27 * 16 * NBB;                     // weight pieces per slice
+    constexpr int NPIECE = 2 * NV + NWP;
+    __shared__ uint4 stage[2 * NV];
+    __shared__ uint4 wl[NWP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int y = blockIdx.y * 4 + wave;
-    const int z = blockIdx.z;
-    const int xw = blockIdx.x * (16 * NSEG);
-    if (y >= H) return;                        // whole wave (no block barrier below)
-    const int i = lane & 15, kq = lane >> 4;
-    constexpr int NB = COUT / 16;
-    constexpr int NCHUNK = (CIN == 16) ? 14 : 27 * (CIN / 32);
-    floatx4 acc[NSEG][NB];
+    const int grp = blockIdx.y % NGRP;
+    const int x0 = blockIdx.x * (16 * NSEG), y0 = (blockIdx.y / NGRP) * 4, z = blockIdx.z;
+    const int y = y0 + wave;
+    const int i = lane & 15, kq = lane >> 4, half = kq & 1;
+    floatx4 acc[NSEG][NBB];
 #pragma unroll
     for (int s = 0; s < NSEG; s++)
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) acc[s][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int c = 0; c < NCHUNK; c++) {
-        int tap, ch;
-        if (CIN == 16) {
-            tap = 2 * c + (kq >> 1);
-            ch = 8 * (kq & 1);
-        } else {
-            tap = c / (CIN / 32);
-            ch = 32 * (c % (CIN / 32)) + 8 * kq;
+        for (int nb = 0; nb < NBB; nb++) acc[s][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int cc = 0; cc < CIN / 16; cc++) {
+        for (int b0 = 0; b0 < NPIECE; b0 += 8 * kNetThreads) {
+            uint4 u[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int t = b0 + r * kNetThreads + threadIdx.x;
+                u[r] = uint4{0u, 0u, 0u, 0u};
+                if (t < 2 * NV) {        // input piece (voxel, half), half fastest
+                    const int v = t >> 1, h = t & 1;
+                    const int xl = v % SX, yl = (v / SX) % SY, zl = v / (SX * SY);
+                    const int gx = x0 - 1 + xl, gy = y0 - 1 + yl, gz = z - 1 + zl;
+                    if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
+                        u[r] = *reinterpret_cast<const uint4*>(
+                            X + ((static_cast<long long>(gz) * H + gy) * W + gx) * CIN + 16 * cc + 8 * h);
+                } else if (t < NPIECE) { // weight piece (half, tap, co), co fastest
+                    const int w = t - 2 * NV;
+                    const int co = w % (16 * NBB), tap = (w / (16 * NBB)) % 27, h = w / (16 * NBB * 27);
+                    u[r] = *reinterpret_cast<const uint4*>(
+                        Wt + (static_cast<long long>(tap) * COUT + grp * 16 * NBB + co) * CIN + 16 * cc + 8 * h);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int t = b0 + r * kNetThreads + threadIdx.x;
+                if (t < 2 * NV) stage[(t & 1) * NV + (t >> 1)] = u[r];
+                else if (t < NPIECE) wl[t - 2 * NV] = u[r];
+            }
         }
-        const bool tap_ok = tap < 27;
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const int zz = z + dz, yy = y + dy;
-        const bool row_ok = tap_ok && zz >= 0 && zz < D && yy >= 0 && yy < H;
-        bf16x8 wa[NB];
+        __syncthreads();
+        if (y < H) {                           // wave-uniform; every wave reaches both barriers
+            // operands of step c + 1 are read from LDS before the MFMAs of step c
+            bf16x8 wa[NBB], xb[NSEG];
+            auto fetch = [&](int c, bf16x8* w, bf16x8* x) {
+                const int tap = 2 * c + (kq >> 1);
+                const bool tap_ok = tap < 27;
+                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++)
-            wa[nb] = load8(Wt + (static_cast<long long>(tap_ok ? tap : 0) * COUT + nb * 16 + i) * CIN + ch, tap_ok);
-        const long long rowv = (static_cast<long long>(row_ok ? zz : 0) * H + (row_ok ? yy : 0)) * W;
+                for (int nb = 0; nb < NBB; nb++)
+                    w[nb] = tap_ok ? __builtin_bit_cast(bf16x8, wl[(half * 27 + tap) * 16 * NBB + nb * 16 + i])
+                                   : bf16x8{};
+                const int base = half * NV + ((dz + 1) * SY + (wave + dy + 1)) * SX + (i + dx + 1);
 #pragma unroll
-        for (int s = 0; s < NSEG; s++) {
-            const int xx = xw + 16 * s + i + dx;
-            const bool ok = row_ok && xx >= 0 && xx < W;
-            const bf16x8 xb = load8(X + (rowv + (ok ? xx : 0)) * CIN + ch, ok);
+                for (int s = 0; s < NSEG; s++)
+                    x[s] = tap_ok ? __builtin_bit_cast(bf16x8, stage[base + 16 * s]) : bf16x8{};
+            };
+            fetch(0, wa, xb);
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++)
-                acc[s][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[nb], xb, acc[s][nb], 0, 0, 0);
+            for (int c = 0; c < 14; c++) {
+                bf16x8 wn[NBB], xn[NSEG];
+                if (c + 1 < 14) fetch(c + 1, wn, xn);
+                __builtin_amdgcn_sched_barrier(0);   // keep the reads of c + 1 ahead of these MFMAs
+#pragma unroll
+                for (int s = 0; s < NSEG; s++)
+#pragma unroll
+                    for (int nb = 0; nb < NBB; nb++)
+                        acc[s][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[nb], xb[s], acc[s][nb], 0, 0, 0);
+                if (c + 1 < 14) {
+#pragma unroll
+                    for (int nb = 0; nb < NBB; nb++) wa[nb] = wn[nb];
+#pragma unroll
+                    for (int s = 0; s < NSEG; s++) xb[s] = xn[s];
+                }
+            }
         }
+        __syncthreads();
     }
+    if (y >= H) return;
     const long long vrow = (static_cast<long long>(z) * H + y) * W;
 #pragma unroll
     for (int s = 0; s < NSEG; s++) {
-        const int v = xw + 16 * s + i;
+        const int v = x0 + 16 * s + i;
         if (v >= W) continue;
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) {
+        for (int nb = 0; nb < NBB; nb++) {
             float o[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 o[j] = acc[s][nb][j];
                 if (RELU) o[j] = o[j] > 0.f ? o[j] : 0.f;
             }
+            const int cob = grp * NBB + nb;
             if (OUT == 0) {
                 uint2 pk;
                 pk.x = static_cast<uint32_t>(f2bf(o[0])) | (static_cast<uint32_t>(f2bf(o[1])) << 16);
                 pk.y = static_cast<uint32_t>(f2bf(o[2])) | (static_cast<uint32_t>(f2bf(o[3])) << 16);
-                *reinterpret_cast<uint2*>(Y + (vrow + v) * COUT + nb * 16 + kq * 4) = pk;
-            } else if (nb == 0 && kq == 0) {
+                *reinterpret_cast<uint2*>(Y + (vrow + v) * COUT + cob * 16 + kq * 4) = pk;
+            } else if (cob == 0 && kq == 0) {
                 Yf[vrow + v] = o[0];
             }
         }
+    }
+}
+
+// z-marching form for 16-channel inputs: a block owns a 16 NSEG (x) x 4 (y) tile, 16 output
+// channels (group grp), and the planes z0 .. z0+ZB-1. Its weights (all taps and input
+// channels of its 16 NBB outputs) stay in LDS for the whole march, and the input planes live
+// in a ring of three LDS slots: plane z+2 is loaded into registers while plane z is computed
+// and written into the slot of plane z-1 afterwards, so each input plane is staged once per
+// block instead of three times and its loads overlap the MFMAs. Same operand layout, K order
+// and accumulation order as k_conv (bit-identical results).
+template <int CIN, int COUT, bool RELU, int OUT, int NSEG>
+__global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
+                                                        uint16_t* __restrict__ Y, float* __restrict__ Yf, int D, int H,
+                                                        int W, int ZB) {
+    static_assert(CIN == 16, "ring of full-channel planes: 16 input channels");
+    constexpr int SX = 16 * NSEG + 2, SY = 6, NPL = SX * SY;    // voxels of one staged plane
+    constexpr int CH8 = CIN / 8;                                 // 16-byte pieces per voxel
+    constexpr int NBB = 1, NGRP = COUT / 16;                     // 16 outputs per block
+    constexpr int NWP = CH8 * 27 * 16 * NBB;                     // weight pieces
+    constexpr int PPL = CH8 * NPL;                               // pieces of one plane
+    constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;   // per thread
+    __shared__ uint4 ring[3 * PPL];                              // [slot][piece8][voxel]
+    __shared__ uint4 wl[NWP];                                    // [piece8][tap][co]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = blockIdx.y % NGRP;
+    const int x0 = blockIdx.x * (16 * NSEG), y0 = (blockIdx.y / NGRP) * 4;
+    const int z0 = blockIdx.z * ZB;
+    const int z1 = (z0 + ZB < D) ? z0 + ZB : D;
+    const int y = y0 + wave;
+    const int i = lane & 15, kq = lane >> 4, half = kq & 1;
+    // one staged plane: piece t = (voxel, p8), p8 fastest
+    auto load_plane = [&](int gz, uint4* u) {
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int t = r * kNetThreads + threadIdx.x;
+            u[r] = uint4{0u, 0u, 0u, 0u};
+            if (t < PPL && gz >= 0 && gz < D) {
+                const int v = t / CH8, p8 = t % CH8;
+                const int gx = x0 - 1 + v % SX, gy = y0 - 1 + v / SX;
+                if (gx >= 0 && gx < W && gy >= 0 && gy < H)
+                    u[r] = *reinterpret_cast<const uint4*>(
+                        X + ((static_cast<long long>(gz) * H + gy) * W + gx) * CIN + 8 * p8);
+            }
+        }
+    };
+    auto store_plane = [&](int slot, const uint4* u) {
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int t = r * kNetThreads + threadIdx.x;
+            if (t < PPL) ring[slot * PPL + (t % CH8) * NPL + t / CH8] = u[r];
+        }
+    };
+    // weights and the first three planes (z0-1, z0, z0+1 -> slots by plane mod 3)
+    for (int t = threadIdx.x; t < NWP; t += kNetThreads) {
+        const int co = t % (16 * NBB), tap = (t / (16 * NBB)) % 27, p8 = t / (16 * NBB * 27);
+        wl[t] = *reinterpret_cast<const uint4*>(Wt + (static_cast<long long>(tap) * COUT + grp * 16 * NBB + co) * CIN +
+                                               8 * p8);
+    }
+    {
+        uint4 u[RPL];
+        for (int q = -1; q <= 1; q++) {
+            load_plane(z0 + q, u);
+            store_plane((z0 + q + 3) % 3, u);
+        }
+    }
+    __syncthreads();
+    for (int z = z0; z < z1; z++) {
+        uint4 nxt[RPL];
+        if (z + 1 < z1) load_plane(z + 2, nxt);               // block-uniform condition
+        floatx4 acc[NSEG][NBB];
+#pragma unroll
+        for (int s = 0; s < NSEG; s++)
+#pragma unroll
+            for (int nb = 0; nb < NBB; nb++) acc[s][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (y < H) {
+            // operands of step c + 1 are read from LDS before the MFMAs of step c (the LDS
+            // latency overlaps the matrix work of the same wave)
+            const int p8 = half;
+            bf16x8 wa[NBB], xb[NSEG];
+            auto fetch = [&](int c, bf16x8* w, bf16x8* x) {
+                const int tap = 2 * c + (kq >> 1);
+                const bool tap_ok = tap < 27;
+                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+                for (int nb = 0; nb < NBB; nb++)
+                    w[nb] = tap_ok ? __builtin_bit_cast(bf16x8, wl[(p8 * 27 + tap) * 16 * NBB + nb * 16 + i]) : bf16x8{};
+                const int slot = (z + dz + 3) % 3;
+                const int base = slot * PPL + p8 * NPL + (wave + dy + 1) * SX + (i + dx + 1);
+#pragma unroll
+                for (int s = 0; s < NSEG; s++)
+                    x[s] = tap_ok ? __builtin_bit_cast(bf16x8, ring[base + 16 * s]) : bf16x8{};
+            };
+            fetch(0, wa, xb);
+#pragma unroll
+            for (int c = 0; c < 14; c++) {
+                bf16x8 wn[NBB], xn[NSEG];
+                if (c + 1 < 14) fetch(c + 1, wn, xn);
+                __builtin_amdgcn_sched_barrier(0);   // keep the reads of c + 1 ahead of these MFMAs
+#pragma unroll
+                for (int s = 0; s < NSEG; s++)
+#pragma unroll
+                    for (int nb = 0; nb < NBB; nb++)
+                        acc[s][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[nb], xb[s], acc[s][nb], 0, 0, 0);
+                if (c + 1 < 14) {
+#pragma unroll
+                    for (int nb = 0; nb < NBB; nb++) wa[nb] = wn[nb];
+#pragma unroll
+                    for (int s = 0; s < NSEG; s++) xb[s] = xn[s];
+                }
+            }
+            const long long vrow = (static_cast<long long>(z) * H + y) * W;
+#pragma unroll
+            for (int s = 0; s < NSEG; s++) {
+                const int v = x0 + 16 * s + i;
+                if (v >= W) continue;
+#pragma unroll
+                for (int nb = 0; nb < NBB; nb++) {
+                    float o[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        o[j] = acc[s][nb][j];
+                        if (RELU) o[j] = o[j] > 0.f ? o[j] : 0.f;
+                    }
+                    const int cob = grp * NBB + nb;
+                    if (OUT == 0) {
+                        uint2 pk;
+                        pk.x = static_cast<uint32_t>(f2bf(o[0])) | (static_cast<uint32_t>(f2bf(o[1])) << 16);
+                        pk.y = static_cast<uint32_t>(f2bf(o[2])) | (static_cast<uint32_t>(f2bf(o[3])) << 16);
+                        *reinterpret_cast<uint2*>(Y + (vrow + v) * COUT + cob * 16 + kq * 4) = pk;
+                    } else if (cob == 0 && kq == 0) {
+                        Yf[vrow + v] = o[0];
+                    }
+                }
+            }
+        }
+        __syncthreads();                                         // plane z-1 no longer read
+        if (z + 1 < z1) store_plane((z + 2) % 3, nxt);
+        __syncthreads();
     }
 }
 
@@ -219,63 +412,90 @@ __global__ __launch_bounds__(kNetThreads) void k_chunk_sum(const float* __restri
 }
 
 // ------------------------------------------------------------------------------ resampling
+// Element-wise kernels over channels-last BF16 tensors, eight channels (16 bytes) per work item
+// (C is 16, 32 or 64); per channel the arithmetic and its order are those of one element at a
+// time. Voxel counts fit 32 bits (a 1024^3 box has 2^30 voxels).
+union Bf8 {
+    uint4 v;
+    uint16_t h[8];
+};
+
+__device__ inline Bf8 ld8(const uint16_t* p, long long i8) { return Bf8{reinterpret_cast<const uint4*>(p)[i8]}; }
+__device__ inline void st8(uint16_t* p, long long i8, const Bf8& b) { reinterpret_cast<uint4*>(p)[i8] = b.v; }
+
+#define DARE_EW8_LOOP(n8)                                                                             \
+    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < (n8);                   \
+         q += (long long)gridDim.x * kNetThreads)
+
+// (z, y, x, channel group) of work item q over a D x H x W box with C/8 groups per voxel
+__device__ inline void ew_coords(long long q, int C8, int H, int W, int* z, int* y, int* x, int* cg) {
+    *cg = static_cast<int>(q % C8);
+    const long long v = q / C8;
+    *x = static_cast<int>(v % W);
+    const long long r = v / W;
+    *y = static_cast<int>(r % H);
+    *z = static_cast<int>(r / H);
+}
+
 // out (coarse, dims D x H x W) = mean of the 8 children of the fine grid (2D x 2H x 2W)
 __global__ __launch_bounds__(kNetThreads) void k_pool(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int C,
                                                      int D, int H, int W) {
-    const long long n = static_cast<long long>(D) * H * W * C;
-    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int c = static_cast<int>(q % C);
-        long long v = q / C;
-        const int x = static_cast<int>(v % W);
-        v /= W;
-        const int y = static_cast<int>(v % H);
-        const int z = static_cast<int>(v / H);
-        float s = 0.f;
+    const int C8 = C / 8;
+    const long long n8 = static_cast<long long>(D) * H * W * C8;
+    DARE_EW8_LOOP(n8) {
+        int z, y, x, cg;
+        ew_coords(q, C8, H, W, &z, &y, &x, &cg);
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++)
                 for (int d = 0; d < 2; d++) {
                     const long long f = ((static_cast<long long>(2 * z + a) * (2 * H) + (2 * y + b)) * (2 * W) + (2 * x + d));
-                    s += bf2f(in[f * C + c]);
+                    const Bf8 c = ld8(in, f * C8 + cg);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) s[e] += bf2f(c.h[e]);
                 }
-        out[q] = f2bf(s * 0.125f);
+        Bf8 o;
+#pragma unroll
+        for (int e = 0; e < 8; e++) o.h[e] = f2bf(s[e] * 0.125f);
+        st8(out, q, o);
     }
 }
 
 // out (fine, D x H x W) = parent value (nearest upsampling)
 __global__ __launch_bounds__(kNetThreads) void k_up(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int C,
                                                    int D, int H, int W) {
-    const long long n = static_cast<long long>(D) * H * W * C;
-    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int c = static_cast<int>(q % C);
-        long long v = q / C;
-        const int x = static_cast<int>(v % W);
-        v /= W;
-        const int y = static_cast<int>(v % H);
-        const int z = static_cast<int>(v / H);
+    const int C8 = C / 8;
+    const long long n8 = static_cast<long long>(D) * H * W * C8;
+    DARE_EW8_LOOP(n8) {
+        int z, y, x, cg;
+        ew_coords(q, C8, H, W, &z, &y, &x, &cg);
         const long long p = (static_cast<long long>(z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2;
-        out[q] = in[p * C + c];
+        st8(out, q, ld8(in, p * C8 + cg));
     }
 }
 
 // out[parent] = (y[parent] > 0) ? sum of the 8 children of du : 0   (up-sampling backward + ReLU)
 __global__ __launch_bounds__(kNetThreads) void k_up_bwd_mask(const uint16_t* __restrict__ du, const uint16_t* __restrict__ y,
                                                             uint16_t* __restrict__ out, int C, int D, int H, int W) {
-    const long long n = static_cast<long long>(D) * H * W * C;
-    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int c = static_cast<int>(q % C);
-        long long v = q / C;
-        const int x = static_cast<int>(v % W);
-        v /= W;
-        const int yy = static_cast<int>(v % H);
-        const int z = static_cast<int>(v / H);
-        float s = 0.f;
+    const int C8 = C / 8;
+    const long long n8 = static_cast<long long>(D) * H * W * C8;
+    DARE_EW8_LOOP(n8) {
+        int z, yy, x, cg;
+        ew_coords(q, C8, H, W, &z, &yy, &x, &cg);
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++)
                 for (int d = 0; d < 2; d++) {
                     const long long f = ((static_cast<long long>(2 * z + a) * (2 * H) + (2 * yy + b)) * (2 * W) + (2 * x + d));
-                    s += bf2f(du[f * C + c]);
+                    const Bf8 c = ld8(du, f * C8 + cg);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) s[e] += bf2f(c.h[e]);
                 }
-        out[q] = bf2f(y[q]) > 0.f ? f2bf(s) : uint16_t(0);
+        const Bf8 yv = ld8(y, q);
+        Bf8 o;
+#pragma unroll
+        for (int e = 0; e < 8; e++) o.h[e] = bf2f(yv.h[e]) > 0.f ? f2bf(s[e]) : uint16_t(0);
+        st8(out, q, o);
     }
 }
 
@@ -284,30 +504,44 @@ __global__ __launch_bounds__(kNetThreads) void k_pool_bwd_add_mask(const uint16_
                                                                   const uint16_t* __restrict__ extra,
                                                                   const uint16_t* __restrict__ y,
                                                                   uint16_t* __restrict__ out, int C, int D, int H, int W) {
-    const long long n = static_cast<long long>(D) * H * W * C;
-    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int c = static_cast<int>(q % C);
-        long long v = q / C;
-        const int x = static_cast<int>(v % W);
-        v /= W;
-        const int yy = static_cast<int>(v % H);
-        const int z = static_cast<int>(v / H);
+    const int C8 = C / 8;
+    const long long n8 = static_cast<long long>(D) * H * W * C8;
+    DARE_EW8_LOOP(n8) {
+        int z, yy, x, cg;
+        ew_coords(q, C8, H, W, &z, &yy, &x, &cg);
         const long long p = (static_cast<long long>(z / 2) * (H / 2) + yy / 2) * (W / 2) + x / 2;
-        const float g = bf2f(extra[q]) + 0.125f * bf2f(dp[p * C + c]);
-        out[q] = bf2f(y[q]) > 0.f ? f2bf(g) : uint16_t(0);
+        const Bf8 ex = ld8(extra, q), pv = ld8(dp, p * C8 + cg), yv = ld8(y, q);
+        Bf8 o;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float g = bf2f(ex.h[e]) + 0.125f * bf2f(pv.h[e]);
+            o.h[e] = bf2f(yv.h[e]) > 0.f ? f2bf(g) : uint16_t(0);
+        }
+        st8(out, q, o);
     }
 }
 
+// n: elements (a multiple of 8)
 __global__ __launch_bounds__(kNetThreads) void k_add(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b,
                                                     uint16_t* __restrict__ out, long long n) {
-    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads)
-        out[q] = f2bf(bf2f(a[q]) + bf2f(b[q]));
+    DARE_EW8_LOOP(n / 8) {
+        const Bf8 x = ld8(a, q), z = ld8(b, q);
+        Bf8 o;
+#pragma unroll
+        for (int e = 0; e < 8; e++) o.h[e] = f2bf(bf2f(x.h[e]) + bf2f(z.h[e]));
+        st8(out, q, o);
+    }
 }
 
 __global__ __launch_bounds__(kNetThreads) void k_mask(const uint16_t* __restrict__ g, const uint16_t* __restrict__ y,
                                                      uint16_t* __restrict__ out, long long n) {
-    for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads)
-        out[q] = bf2f(y[q]) > 0.f ? g[q] : uint16_t(0);
+    DARE_EW8_LOOP(n / 8) {
+        const Bf8 gv = ld8(g, q), yv = ld8(y, q);
+        Bf8 o;
+#pragma unroll
+        for (int e = 0; e < 8; e++) o.h[e] = bf2f(yv.h[e]) > 0.f ? gv.h[e] : uint16_t(0);
+        st8(out, q, o);
+    }
 }
 
 // ------------------------------------------------------------------------------ solver side
@@ -364,13 +598,15 @@ __global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double
     }
 }
 
-__global__ __launch_bounds__(64) void k_scales_finish(const double* __restrict__ part, int nblocks, double* __restrict__ s) {
-    if (threadIdx.x >= 3) return;
+// one wave per scale: lanes stride over the block maxima, then a wave maximum (max is exact in
+// any order)
+__global__ __launch_bounds__(192) void k_scales_finish(const double* __restrict__ part, int nblocks, double* __restrict__ s) {
+    const int which = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double m = 0.;
-    for (int b = 0; b < nblocks; b++) m = fmax(m, part[threadIdx.x * 1024 + b]);
+    for (int b = lane; b < nblocks; b += 64) m = fmax(m, part[which * 1024 + b]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off));
     // floors keep the channels finite when a field is flat (first step of a trajectory)
-    if (threadIdx.x == 0) s[0] = fmax(m, 1e-300);
-    else s[threadIdx.x] = fmax(m, 1e-300);
+    if (lane == 0) s[which] = fmax(m, 1e-300);
 }
 
 // x0[v][c]: c0 = phi^n / s0, c1 = (phi^n - phi^{n-1}) / s1, c2 = resid * resid_scale / s2,
@@ -551,15 +787,33 @@ template <int CIN, int COUT, bool RELU, int OUT>
 void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, float* Yf, const int* d, int level) {
     const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
     const dim3 blk(kNetThreads);
+    const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / (16 * conv_nbb<COUT>()));
+    if constexpr (CIN == 16) {
+        // z-marching form: planes per block chosen for about three blocks per CU over the grid
+        const int nseg = W >= 64 ? 4 : (W >= 32 ? 2 : 1);
+        const unsigned gzm = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
+        const long long cols = static_cast<long long>((W + 16 * nseg - 1) / (16 * nseg)) * gzm;
+        int zb = static_cast<int>(std::max<long long>(2, std::min<long long>(D, cols * D / (3LL * 256))));
+        zb = std::min(zb, 32);
+        const dim3 grid(static_cast<unsigned>((W + 16 * nseg - 1) / (16 * nseg)), gzm,
+                        static_cast<unsigned>((D + zb - 1) / zb));
+        if (nseg == 4)
+            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 4>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb);
+        else if (nseg == 2)
+            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 2>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb);
+        else
+            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 1>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb);
+        return;
+    }
     if (W >= 64) {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 4>), dim3((W + 63) / 64, (H + 3) / 4, D), blk, 0, sm, X, Wt,
-                           Y, Yf, D, H, W);
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 4>), dim3((W + 63) / 64, gy, D), blk, 0, sm, X, Wt, Y, Yf, D,
+                           H, W);
     } else if (W >= 32) {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 2>), dim3((W + 31) / 32, (H + 3) / 4, D), blk, 0, sm, X, Wt,
-                           Y, Yf, D, H, W);
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 2>), dim3((W + 31) / 32, gy, D), blk, 0, sm, X, Wt, Y, Yf, D,
+                           H, W);
     } else {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1>), dim3((W + 15) / 16, (H + 3) / 4, D), blk, 0, sm, X, Wt,
-                           Y, Yf, D, H, W);
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1>), dim3((W + 15) / 16, gy, D), blk, 0, sm, X, Wt, Y, Yf, D,
+                           H, W);
     }
 }
 
@@ -795,7 +1049,7 @@ int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
     const unsigned gs = std::min<unsigned>(ew_grid(n), 1024);
     hipLaunchKernelGGL(k_scales, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid, resid_scale,
                        net->part);
-    hipLaunchKernelGGL(k_scales_finish, dim3(1), dim3(64), 0, ctx->stream, net->part, static_cast<int>(gs), net->scales);
+    hipLaunchKernelGGL(k_scales_finish, dim3(1), dim3(192), 0, ctx->stream, net->part, static_cast<int>(gs), net->scales);
     hipLaunchKernelGGL(k_pack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid,
                        resid_scale, ex, net->scales, net->x0);
     DARE_LAUNCH_CHECK();
