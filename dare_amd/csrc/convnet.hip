@@ -80,10 +80,15 @@ __device__ inline bf16x8 load8(const uint16_t* p, bool ok) {
 template <int COUT>
 constexpr int conv_nbb() { return COUT / 16 < 2 ? COUT / 16 : 2; }
 
-template <int CIN, int COUT, bool RELU, int OUT, int NSEG>
+// Fusions of k_conv (MODE bits): kConvUp -- the input is the half-resolution tensor and the
+// conv reads its nearest-neighbour upsampling (staging fetches each voxel's parent); kConvAdd --
+// the epilogue also writes Y2 = bf16(Y + E) (the decoder's skip connection, as k_add).
+constexpr int kConvUp = 1, kConvAdd = 2;
+
+template <int CIN, int COUT, bool RELU, int OUT, int NSEG, int MODE = 0>
 __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
                                                      uint16_t* __restrict__ Y, float* __restrict__ Yf, int D, int H,
-                                                     int W) {
+                                                     int W, const uint16_t* __restrict__ E, uint16_t* __restrict__ Y2) {
     // Block tile: 16 NSEG voxels along x, 4 rows along y (one per wave), one plane z, and
     // 16 NBB output channels (blockIdx.y = row block * NGRP + channel group). Per 16-channel
     // slice of the input, the block stages in LDS the input it needs -- (16 NSEG + 2) x 6 x 3
@@ -119,9 +124,12 @@ __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict
                     const int v = t >> 1, h = t & 1;
                     const int xl = v % SX, yl = (v / SX) % SY, zl = v / (SX * SY);
                     const int gx = x0 - 1 + xl, gy = y0 - 1 + yl, gz = z - 1 + zl;
-                    if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
-                        u[r] = *reinterpret_cast<const uint4*>(
-                            X + ((static_cast<long long>(gz) * H + gy) * W + gx) * CIN + 16 * cc + 8 * h);
+                    if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D) {
+                        const long long src = (MODE & kConvUp)
+                            ? (static_cast<long long>(gz >> 1) * (H >> 1) + (gy >> 1)) * (W >> 1) + (gx >> 1)
+                            : (static_cast<long long>(gz) * H + gy) * W + gx;
+                        u[r] = *reinterpret_cast<const uint4*>(X + src * CIN + 16 * cc + 8 * h);
+                    }
                 } else if (t < NPIECE) { // weight piece (half, tap, co), co fastest
                     const int w = t - 2 * NV;
                     const int co = w % (16 * NBB), tap = (w / (16 * NBB)) % 27, h = w / (16 * NBB * 27);
@@ -190,10 +198,26 @@ __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict
             }
             const int cob = grp * NBB + nb;
             if (OUT == 0) {
+                uint16_t b[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) b[j] = f2bf(o[j]);
+                const long long at = (vrow + v) * COUT + cob * 16 + kq * 4;
                 uint2 pk;
-                pk.x = static_cast<uint32_t>(f2bf(o[0])) | (static_cast<uint32_t>(f2bf(o[1])) << 16);
-                pk.y = static_cast<uint32_t>(f2bf(o[2])) | (static_cast<uint32_t>(f2bf(o[3])) << 16);
-                *reinterpret_cast<uint2*>(Y + (vrow + v) * COUT + cob * 16 + kq * 4) = pk;
+                pk.x = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 16);
+                pk.y = static_cast<uint32_t>(b[2]) | (static_cast<uint32_t>(b[3]) << 16);
+                *reinterpret_cast<uint2*>(Y + at) = pk;
+                if constexpr ((MODE & kConvAdd) != 0) {
+                    const uint2 ev = *reinterpret_cast<const uint2*>(E + at);
+                    const uint16_t e[4] = {static_cast<uint16_t>(ev.x & 0xffffu), static_cast<uint16_t>(ev.x >> 16),
+                                           static_cast<uint16_t>(ev.y & 0xffffu), static_cast<uint16_t>(ev.y >> 16)};
+                    uint16_t t[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) t[j] = f2bf(bf2f(b[j]) + bf2f(e[j]));
+                    uint2 sk;
+                    sk.x = static_cast<uint32_t>(t[0]) | (static_cast<uint32_t>(t[1]) << 16);
+                    sk.y = static_cast<uint32_t>(t[2]) | (static_cast<uint32_t>(t[3]) << 16);
+                    *reinterpret_cast<uint2*>(Y2 + at) = sk;
+                }
             } else if (cob == 0 && kq == 0) {
                 Yf[vrow + v] = o[0];
             }
@@ -783,20 +807,51 @@ long long level_voxels(const int* d, int level) {
     return static_cast<long long>(d[0] >> level) * (d[1] >> level) * (d[2] >> level);
 }
 
-template <int CIN, int COUT, bool RELU, int OUT>
-void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, float* Yf, const int* d, int level) {
+// compute units of the current device and resident 256-thread blocks of a kernel (queried
+// once per instantiation)
+inline int device_cus() {
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            n = v;
+        else
+            n = 256;
+    }
+    return n;
+}
+
+template <auto K>   // one cached value per kernel (the kernels share one function-pointer type)
+int resident_blocks() {
+    static int n = 0;
+    if (n == 0) {
+        int v = 0;
+        n = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, K, kNetThreads, 0) == hipSuccess && v > 0) ? v : 1;
+    }
+    return n;
+}
+
+template <int CIN, int COUT, bool RELU, int OUT, int MODE = 0>
+void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, float* Yf, const int* d, int level,
+          const uint16_t* E = nullptr, uint16_t* Y2 = nullptr) {
     const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
     const dim3 blk(kNetThreads);
     const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / (16 * conv_nbb<COUT>()));
-    if constexpr (CIN == 16) {
-        // z-marching form: planes per block chosen for about three blocks per CU over the grid
+    if constexpr (CIN == 16 && MODE == 0) {
+        // z-marching form: the planes per block are chosen so that the whole grid is resident at
+        // once (no tail round): z-blocks = resident blocks / (x, y, channel-group) columns
         const int nseg = W >= 64 ? 4 : (W >= 32 ? 2 : 1);
+        const unsigned gx = static_cast<unsigned>((W + 16 * nseg - 1) / (16 * nseg));
         const unsigned gzm = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
-        const long long cols = static_cast<long long>((W + 16 * nseg - 1) / (16 * nseg)) * gzm;
-        int zb = static_cast<int>(std::max<long long>(2, std::min<long long>(D, cols * D / (3LL * 256))));
-        zb = std::min(zb, 32);
-        const dim3 grid(static_cast<unsigned>((W + 16 * nseg - 1) / (16 * nseg)), gzm,
-                        static_cast<unsigned>((D + zb - 1) / zb));
+        int per_cu = 1;
+        if (nseg == 4) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 4>>();
+        else if (nseg == 2) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 2>>();
+        else per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 1>>();
+        const long long cols = static_cast<long long>(gx) * gzm;
+        const long long nzb = std::max<long long>(1, static_cast<long long>(device_cus()) * per_cu / cols);
+        const int zb = static_cast<int>((D + nzb - 1) / nzb);
+        const dim3 grid(gx, gzm, static_cast<unsigned>((D + zb - 1) / zb));
         if (nseg == 4)
             hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 4>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb);
         else if (nseg == 2)
@@ -806,14 +861,14 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
         return;
     }
     if (W >= 64) {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 4>), dim3((W + 63) / 64, gy, D), blk, 0, sm, X, Wt, Y, Yf, D,
-                           H, W);
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 4, MODE>), dim3((W + 63) / 64, gy, D), blk, 0, sm, X, Wt, Y,
+                           Yf, D, H, W, E, Y2);
     } else if (W >= 32) {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 2>), dim3((W + 31) / 32, gy, D), blk, 0, sm, X, Wt, Y, Yf, D,
-                           H, W);
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 2, MODE>), dim3((W + 31) / 32, gy, D), blk, 0, sm, X, Wt, Y,
+                           Yf, D, H, W, E, Y2);
     } else {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1>), dim3((W + 15) / 16, gy, D), blk, 0, sm, X, Wt, Y, Yf, D,
-                           H, W);
+        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1, MODE>), dim3((W + 15) / 16, gy, D), blk, 0, sm, X, Wt, Y,
+                           Yf, D, H, W, E, Y2);
     }
 }
 
@@ -1074,14 +1129,11 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
                        d[0] >> 2, d[1] >> 2, d[2] >> 2);
     conv<32, 64, true, 0>(sm, net->p2, W + o[4], net->a5, nullptr, d, 2);
     conv<64, 64, true, 0>(sm, net->a5, W + o[5], net->e3, nullptr, d, 2);
-    hipLaunchKernelGGL(k_up, dim3(ew_grid(V1 * 64)), dim3(kNetThreads), 0, sm, net->e3, net->u2, 64, d[0] >> 1,
-                       d[1] >> 1, d[2] >> 1);
-    conv<64, 32, true, 0>(sm, net->u2, W + o[6], net->a7, nullptr, d, 1);
-    hipLaunchKernelGGL(k_add, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->a7, net->e2, net->s2, V1 * 32);
+    // decoder: the upsampling is read inside the conv (u2, u1 are written by the backward pass,
+    // which needs them) and the skip sums s2 = a7 + e2, s1 = a9 + e1 come out of its epilogue
+    conv<64, 32, true, 0, kConvUp | kConvAdd>(sm, net->e3, W + o[6], net->a7, nullptr, d, 1, net->e2, net->s2);
     conv<32, 32, true, 0>(sm, net->s2, W + o[7], net->d2, nullptr, d, 1);
-    hipLaunchKernelGGL(k_up, dim3(ew_grid(V0 * 32)), dim3(kNetThreads), 0, sm, net->d2, net->u1, 32, d[0], d[1], d[2]);
-    conv<32, 16, true, 0>(sm, net->u1, W + o[8], net->a9, nullptr, d, 0);
-    hipLaunchKernelGGL(k_add, dim3(ew_grid(V0 * 16)), dim3(kNetThreads), 0, sm, net->a9, net->e1, net->s1, V0 * 16);
+    conv<32, 16, true, 0, kConvUp | kConvAdd>(sm, net->d2, W + o[8], net->a9, nullptr, d, 0, net->e1, net->s1);
     conv<16, 16, true, 0>(sm, net->s1, W + o[9], net->d1, nullptr, d, 0);
     conv<16, 16, false, 1>(sm, net->d1, W + o[10], nullptr, net->out, d, 0);
     DARE_LAUNCH_CHECK();
@@ -1167,7 +1219,8 @@ int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net) {
     wgrad_any(net, sm, 9, net->s1, net->t0b);
     conv_any(sm, 16, 16, false, net->t0b, WB + o[9], net->t0c, d, 0);              // ds1 (kept)
     hipLaunchKernelGGL(k_mask, dim3(ew_grid(V0 * 16)), dim3(kNetThreads), 0, sm, net->t0c, net->a9, net->t0a, V0 * 16);
-    // L9: g9 = t0a
+    // L9: g9 = t0a (u1 = up(d2), read fused by the forward pass, materialized here)
+    hipLaunchKernelGGL(k_up, dim3(ew_grid(V0 * 32)), dim3(kNetThreads), 0, sm, net->d2, net->u1, 32, d[0], d[1], d[2]);
     wgrad_any(net, sm, 8, net->u1, net->t0a);
     conv_any(sm, 16, 32, false, net->t0a, WB + o[8], net->t0w, d, 0);              // du1 (32 ch)
     hipLaunchKernelGGL(k_up_bwd_mask, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->t0w, net->d2, net->h1a,
@@ -1176,7 +1229,9 @@ int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net) {
     wgrad_any(net, sm, 7, net->s2, net->h1a);
     conv_any(sm, 32, 32, false, net->h1a, WB + o[7], net->h1b, d, 1);              // ds2 (kept)
     hipLaunchKernelGGL(k_mask, dim3(ew_grid(V1 * 32)), dim3(kNetThreads), 0, sm, net->h1b, net->a7, net->h1c, V1 * 32);
-    // L7: g7 = h1c
+    // L7: g7 = h1c (u2 = up(e3), as u1)
+    hipLaunchKernelGGL(k_up, dim3(ew_grid(V1 * 64)), dim3(kNetThreads), 0, sm, net->e3, net->u2, 64, d[0] >> 1,
+                       d[1] >> 1, d[2] >> 1);
     wgrad_any(net, sm, 6, net->u2, net->h1c);
     conv_any(sm, 32, 64, false, net->h1c, WB + o[6], net->h1w, d, 1);              // du2 (64 ch)
     hipLaunchKernelGGL(k_up_bwd_mask, dim3(ew_grid(V2 * 64)), dim3(kNetThreads), 0, sm, net->h1w, net->e3, net->q2a,
