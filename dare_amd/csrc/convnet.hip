@@ -225,24 +225,27 @@ __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict
     }
 }
 
-// z-marching form for 16-channel inputs: a block owns a 16 NSEG (x) x 4 (y) tile, 16 output
-// channels (group grp), and the planes z0 .. z0+ZB-1. Its weights (all taps and input
-// channels of its 16 NBB outputs) stay in LDS for the whole march, and the input planes live
-// in a ring of three LDS slots: plane z+2 is loaded into registers while plane z is computed
-// and written into the slot of plane z-1 afterwards, so each input plane is staged once per
-// block instead of three times and its loads overlap the MFMAs. Same operand layout, K order
-// and accumulation order as k_conv (bit-identical results).
-template <int CIN, int COUT, bool RELU, int OUT, int NSEG>
+// z-marching form for 16- and 32-channel inputs: a block owns a 16 NSEG (x) x 4 (y) tile, 16
+// output channels (group grp), and the planes z0 .. z0+ZB-1. Its weights (all taps and input
+// channels of its 16 outputs) stay in LDS for the whole march, and the input planes live in a
+// ring of three LDS slots: plane z+2 is loaded into registers while plane z is computed and
+// written into the slot of plane z-1 afterwards, so each input plane is staged once per block
+// instead of three times and its loads overlap the MFMAs. MODE as k_conv (upsampled input,
+// skip-sum epilogue). Same operand layout, K order and accumulation order as k_conv
+// (bit-identical results).
+template <int CIN, int COUT, bool RELU, int OUT, int NSEG, int MODE = 0>
 __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
                                                         uint16_t* __restrict__ Y, float* __restrict__ Yf, int D, int H,
-                                                        int W, int ZB) {
-    static_assert(CIN == 16, "ring of full-channel planes: 16 input channels");
+                                                        int W, int ZB, const uint16_t* __restrict__ E,
+                                                        uint16_t* __restrict__ Y2) {
+    static_assert(CIN == 16 || CIN == 32, "ring of full-channel planes: 16 or 32 input channels");
     constexpr int SX = 16 * NSEG + 2, SY = 6, NPL = SX * SY;    // voxels of one staged plane
     constexpr int CH8 = CIN / 8;                                 // 16-byte pieces per voxel
-    constexpr int NBB = 1, NGRP = COUT / 16;                     // 16 outputs per block
-    constexpr int NWP = CH8 * 27 * 16 * NBB;                     // weight pieces
+    constexpr int NGRP = COUT / 16;                              // 16 outputs per block
+    constexpr int NWP = CH8 * 27 * 16;                           // weight pieces
     constexpr int PPL = CH8 * NPL;                               // pieces of one plane
     constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;   // per thread
+    constexpr int NSTEP = (CIN / 16) * 14;                       // K = 32 steps
     __shared__ uint4 ring[3 * PPL];                              // [slot][piece8][voxel]
     __shared__ uint4 wl[NWP];                                    // [piece8][tap][co]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -261,9 +264,12 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
             if (t < PPL && gz >= 0 && gz < D) {
                 const int v = t / CH8, p8 = t % CH8;
                 const int gx = x0 - 1 + v % SX, gy = y0 - 1 + v / SX;
-                if (gx >= 0 && gx < W && gy >= 0 && gy < H)
-                    u[r] = *reinterpret_cast<const uint4*>(
-                        X + ((static_cast<long long>(gz) * H + gy) * W + gx) * CIN + 8 * p8);
+                if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                    const long long src = (MODE & kConvUp)
+                        ? (static_cast<long long>(gz >> 1) * (H >> 1) + (gy >> 1)) * (W >> 1) + (gx >> 1)
+                        : (static_cast<long long>(gz) * H + gy) * W + gx;
+                    u[r] = *reinterpret_cast<const uint4*>(X + src * CIN + 8 * p8);
+                }
             }
         }
     };
@@ -276,9 +282,8 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
     };
     // weights and the first three planes (z0-1, z0, z0+1 -> slots by plane mod 3)
     for (int t = threadIdx.x; t < NWP; t += kNetThreads) {
-        const int co = t % (16 * NBB), tap = (t / (16 * NBB)) % 27, p8 = t / (16 * NBB * 27);
-        wl[t] = *reinterpret_cast<const uint4*>(Wt + (static_cast<long long>(tap) * COUT + grp * 16 * NBB + co) * CIN +
-                                               8 * p8);
+        const int co = t % 16, tap = (t / 16) % 27, p8 = t / (16 * 27);
+        wl[t] = *reinterpret_cast<const uint4*>(Wt + (static_cast<long long>(tap) * COUT + grp * 16 + co) * CIN + 8 * p8);
     }
     {
         uint4 u[RPL];
@@ -291,23 +296,20 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
     for (int z = z0; z < z1; z++) {
         uint4 nxt[RPL];
         if (z + 1 < z1) load_plane(z + 2, nxt);               // block-uniform condition
-        floatx4 acc[NSEG][NBB];
+        floatx4 acc[NSEG];
 #pragma unroll
-        for (int s = 0; s < NSEG; s++)
-#pragma unroll
-            for (int nb = 0; nb < NBB; nb++) acc[s][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < NSEG; s++) acc[s] = floatx4{0.f, 0.f, 0.f, 0.f};
         if (y < H) {
-            // operands of step c + 1 are read from LDS before the MFMAs of step c (the LDS
-            // latency overlaps the matrix work of the same wave)
-            const int p8 = half;
-            bf16x8 wa[NBB], xb[NSEG];
-            auto fetch = [&](int c, bf16x8* w, bf16x8* x) {
-                const int tap = 2 * c + (kq >> 1);
+            // operands of step q + 1 are read from LDS before the MFMAs of step q (the LDS
+            // latency overlaps the matrix work of the same wave); step q = 16-channel slice
+            // q / 14, taps 2 (q % 14) and 2 (q % 14) + 1
+            bf16x8 wa, xb[NSEG];
+            auto fetch = [&](int q, bf16x8& w, bf16x8* x) {
+                const int p8 = 2 * (q / 14) + half;
+                const int tap = 2 * (q % 14) + (kq >> 1);
                 const bool tap_ok = tap < 27;
                 const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-#pragma unroll
-                for (int nb = 0; nb < NBB; nb++)
-                    w[nb] = tap_ok ? __builtin_bit_cast(bf16x8, wl[(p8 * 27 + tap) * 16 * NBB + nb * 16 + i]) : bf16x8{};
+                w = tap_ok ? __builtin_bit_cast(bf16x8, wl[(p8 * 27 + tap) * 16 + i]) : bf16x8{};
                 const int slot = (z + dz + 3) % 3;
                 const int base = slot * PPL + p8 * NPL + (wave + dy + 1) * SX + (i + dx + 1);
 #pragma unroll
@@ -316,18 +318,14 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
             };
             fetch(0, wa, xb);
 #pragma unroll
-            for (int c = 0; c < 14; c++) {
-                bf16x8 wn[NBB], xn[NSEG];
-                if (c + 1 < 14) fetch(c + 1, wn, xn);
-                __builtin_amdgcn_sched_barrier(0);   // keep the reads of c + 1 ahead of these MFMAs
+            for (int q = 0; q < NSTEP; q++) {
+                bf16x8 wn, xn[NSEG];
+                if (q + 1 < NSTEP) fetch(q + 1, wn, xn);
+                __builtin_amdgcn_sched_barrier(0);   // keep the reads of q + 1 ahead of these MFMAs
 #pragma unroll
-                for (int s = 0; s < NSEG; s++)
-#pragma unroll
-                    for (int nb = 0; nb < NBB; nb++)
-                        acc[s][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[nb], xb[s], acc[s][nb], 0, 0, 0);
-                if (c + 1 < 14) {
-#pragma unroll
-                    for (int nb = 0; nb < NBB; nb++) wa[nb] = wn[nb];
+                for (int s = 0; s < NSEG; s++) acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb[s], acc[s], 0, 0, 0);
+                if (q + 1 < NSTEP) {
+                    wa = wn;
 #pragma unroll
                     for (int s = 0; s < NSEG; s++) xb[s] = xn[s];
                 }
@@ -337,23 +335,35 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
             for (int s = 0; s < NSEG; s++) {
                 const int v = x0 + 16 * s + i;
                 if (v >= W) continue;
+                float o[4];
 #pragma unroll
-                for (int nb = 0; nb < NBB; nb++) {
-                    float o[4];
+                for (int j = 0; j < 4; j++) {
+                    o[j] = acc[s][j];
+                    if (RELU) o[j] = o[j] > 0.f ? o[j] : 0.f;
+                }
+                if (OUT == 0) {
+                    uint16_t b[4];
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        o[j] = acc[s][nb][j];
-                        if (RELU) o[j] = o[j] > 0.f ? o[j] : 0.f;
+                    for (int j = 0; j < 4; j++) b[j] = f2bf(o[j]);
+                    const long long at = (vrow + v) * COUT + grp * 16 + kq * 4;
+                    uint2 pk;
+                    pk.x = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 16);
+                    pk.y = static_cast<uint32_t>(b[2]) | (static_cast<uint32_t>(b[3]) << 16);
+                    *reinterpret_cast<uint2*>(Y + at) = pk;
+                    if constexpr ((MODE & kConvAdd) != 0) {
+                        const uint2 ev = *reinterpret_cast<const uint2*>(E + at);
+                        const uint16_t e[4] = {static_cast<uint16_t>(ev.x & 0xffffu), static_cast<uint16_t>(ev.x >> 16),
+                                               static_cast<uint16_t>(ev.y & 0xffffu), static_cast<uint16_t>(ev.y >> 16)};
+                        uint16_t t[4];
+#pragma unroll
+                        for (int j = 0; j < 4; j++) t[j] = f2bf(bf2f(b[j]) + bf2f(e[j]));
+                        uint2 sk;
+                        sk.x = static_cast<uint32_t>(t[0]) | (static_cast<uint32_t>(t[1]) << 16);
+                        sk.y = static_cast<uint32_t>(t[2]) | (static_cast<uint32_t>(t[3]) << 16);
+                        *reinterpret_cast<uint2*>(Y2 + at) = sk;
                     }
-                    const int cob = grp * NBB + nb;
-                    if (OUT == 0) {
-                        uint2 pk;
-                        pk.x = static_cast<uint32_t>(f2bf(o[0])) | (static_cast<uint32_t>(f2bf(o[1])) << 16);
-                        pk.y = static_cast<uint32_t>(f2bf(o[2])) | (static_cast<uint32_t>(f2bf(o[3])) << 16);
-                        *reinterpret_cast<uint2*>(Y + (vrow + v) * COUT + cob * 16 + kq * 4) = pk;
-                    } else if (cob == 0 && kq == 0) {
-                        Yf[vrow + v] = o[0];
-                    }
+                } else if (grp == 0 && kq == 0) {
+                    Yf[vrow + v] = o[0];
                 }
             }
         }
@@ -838,26 +848,30 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
     const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
     const dim3 blk(kNetThreads);
     const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / (16 * conv_nbb<COUT>()));
-    if constexpr (CIN == 16 && MODE == 0) {
+    if constexpr (CIN == 16 || (CIN == 32 && COUT == 16)) {
         // z-marching form: the planes per block are chosen so that the whole grid is resident at
-        // once (no tail round): z-blocks = resident blocks / (x, y, channel-group) columns
-        const int nseg = W >= 64 ? 4 : (W >= 32 ? 2 : 1);
+        // once (no tail round): z-blocks = resident blocks / (x, y, channel-group) columns. 32-
+        // channel rings use tiles of at most 32 voxels along x (LDS).
+        const int nseg = (W >= 64 && CIN == 16) ? 4 : (W >= 32 ? 2 : 1);
         const unsigned gx = static_cast<unsigned>((W + 16 * nseg - 1) / (16 * nseg));
         const unsigned gzm = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
         int per_cu = 1;
-        if (nseg == 4) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 4>>();
-        else if (nseg == 2) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 2>>();
-        else per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 1>>();
+        if (nseg == 4) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, (CIN == 16 ? 4 : 2), MODE>>();
+        else if (nseg == 2) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 2, MODE>>();
+        else per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 1, MODE>>();
         const long long cols = static_cast<long long>(gx) * gzm;
         const long long nzb = std::max<long long>(1, static_cast<long long>(device_cus()) * per_cu / cols);
         const int zb = static_cast<int>((D + nzb - 1) / nzb);
         const dim3 grid(gx, gzm, static_cast<unsigned>((D + zb - 1) / zb));
         if (nseg == 4)
-            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 4>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb);
+            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, (CIN == 16 ? 4 : 2), MODE>), grid, blk, 0, sm, X, Wt, Y,
+                               Yf, D, H, W, zb, E, Y2);
         else if (nseg == 2)
-            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 2>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb);
+            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 2, MODE>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb,
+                               E, Y2);
         else
-            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 1>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb);
+            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 1, MODE>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb,
+                               E, Y2);
         return;
     }
     if (W >= 64) {
