@@ -106,6 +106,32 @@ def test_forward_matches_oracle(ctx, rng):
     net.close()
 
 
+def test_forward_wide_box_matches_oracle(ctx, rng):
+    """A box wide enough for every tiling of the conv kernels: 64- and 32-voxel x tiles, and
+    z-marching blocks that carry their plane ring over several planes (16 x 64 x 256: 64
+    tile columns at full resolution, so the resident grid splits z into blocks of two planes)."""
+    import torch
+    nb = [16, 64, 256]
+    net = WarmStartNet(ctx, nb, in_channels=3, seed=5)
+    w = he_init(3, seed=5, out_scale=1.0)
+    net.set_params(w)
+    g = da.Grid.make(nb, size=[1, 1, 1], ghost=2)
+    dev, hostf = fields(rng, nb)
+    net.pack(g, dev[0], dev[1], dev[2], 0.7)
+    net.forward()
+    ctx.synchronize()
+    ptr, _ = net.tensors()
+    x0 = d2h_bf16(ctx, ptr["x0"], nb + [16])
+    out_ref, A = cn.forward(x0, cn.split(cn.bf16(w)), exact=False)
+    shapes = {"e1": (0, 16), "e2": (1, 32), "e3": (2, 64), "s2": (1, 32), "s1": (0, 16), "d1": (0, 16)}
+    for name, (lv, c) in shapes.items():
+        got = d2h_bf16(ctx, ptr[name], [m >> lv for m in nb] + [c])
+        assert rel(got, A[name]) <= 1e-2, (name, rel(got, A[name]))
+    out = d2h_f32(ctx, ptr["out"], nb)
+    assert rel(out, out_ref) <= 1e-2, rel(out, out_ref)
+    net.close()
+
+
 def test_backward_matches_oracle(ctx, rng):
     import torch
     net, g, dev, (phi, phim, rhs, ext), w, _ = setup(ctx, rng)
