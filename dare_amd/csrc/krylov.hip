@@ -36,7 +36,11 @@ struct SolveOp {
             if (d.V.N == 2)
                 hipLaunchKernelGGL((k_apply_dia<2, VT, XT, Epi>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws, st,
                                    tail, check);
-            else if (d.pair == 2)
+            else if (d.pair == 4) {   // FP32 operators only (make_dia)
+                if constexpr (std::is_same_v<VT, float> && std::is_same_v<XT, float>)
+                    hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 4>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi,
+                                       ws, st, tail, check);
+            } else if (d.pair == 2)
                 hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 2>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws,
                                    st, tail, check);
             else

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace dare_hip {
@@ -217,6 +219,7 @@ DiaOp<VT> make_dia(const DiaView& V, const VT* D, bool allow_pair = false) {
     A.V = V;
     A.D = D;
     A.pair = (allow_pair && V.N == 1 && V.n[2] % 2 == 0) ? 2 : 1;
+    if (allow_pair && std::is_same_v<VT, float> && V.N == 1 && V.n[2] % 4 == 0) A.pair = 4;
     A.tiles_k = (V.n[2] + kTileK * A.pair - 1) / (kTileK * A.pair);
     A.tiles_j = (V.n[1] + kTileJ - 1) / kTileJ;
     A.tile_i = kTileI;
@@ -356,6 +359,38 @@ __device__ __forceinline__ void epi_pair(const EpiInit<XT, NT>& e, long long i, 
     acc[1] += d0 * d0;
     acc[0] += d1 * d1;
     acc[1] += d1 * d1;
+}
+
+// The epilogue of four adjacent rows (the FP32 quad traversal): four calls, or for EpiInit the
+// stores ordered vector by vector, as epi_pair.
+template <class Epi>
+__device__ __forceinline__ void epi_quad(const Epi& e, long long i, const double* y, const double* xc,
+                                         const double* pre, double* acc) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) e(i + m, y[m], xc[m], pre[m], acc);
+}
+template <typename XT, bool NT>
+__device__ __forceinline__ void epi_quad(const EpiInit<XT, NT>& e, long long i, const double* y, const double*,
+                                         const double* b, double* acc) {
+    XT r[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) r[m] = static_cast<XT>(b[m] - y[m]);
+#pragma unroll
+    for (int m = 0; m < 4; m++) put<NT>(e.r + i + m, r[m]);
+#pragma unroll
+    for (int m = 0; m < 4; m++) put<NT>(e.rh + i + m, r[m]);
+#pragma unroll
+    for (int m = 0; m < 4; m++) put<NT>(e.u + i + m, XT(0));
+#pragma unroll
+    for (int m = 0; m < 4; m++) put<NT>(e.v + i + m, XT(0));
+#pragma unroll
+    for (int m = 0; m < 4; m++) put<NT>(e.w + i + m, XT(0));
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double dd = static_cast<double>(r[m]);
+        acc[0] += dd * dd;
+        acc[1] += dd * dd;
+    }
 }
 
 // ------------------------------------------------------------------------------ traversals
@@ -552,6 +587,76 @@ __device__ inline void dia_column_pair(const DiaView& V, const VT* __restrict__ 
     }
 }
 
+// dia_column_quad: the FP32 operator of the inner solves (one component, FP32 slots and
+// vectors), four k-adjacent rows per lane, so every slot, x and vector access is one 16-byte
+// float4 and a wave moves 1 KB per memory instruction. The products and sums are FP32 (the
+// precision of the inner iterations; the slot order of each row is dia_column's), the dot
+// accumulators stay FP64. Rows k..k+3 with k a multiple of 4 and n2 % 4 == 0.
+template <class Epi>
+__device__ inline void dia_column_quad(const DiaView& V, const float* __restrict__ D, const float* __restrict__ x,
+                                       int k, int j, int i0, int i1, const Epi& epi, double* acc) {
+    const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
+    const long long sI = static_cast<long long>(n1) * n2, sJ = n2;
+    const long long nr = V.nrows;
+    const bool hasS = !(j == 0 && V.rm_lo[1]);
+    const bool hasN = !(j == n1 - 1 && V.rm_hi[1] && !(j == 0 && V.rm_lo[1]));
+    const bool hasB = !(k == 0 && V.rm_lo[2]);               // cell k   (its top is cell k+1)
+    const bool hasT = !(k + 3 == n2 - 1 && V.rm_hi[2]);      // cell k+3 (n2 >= 4)
+    const long long dS = !hasS ? 0 : (j == 0) ? (long long)(n1 - 1) * sJ : -sJ;
+    const long long dN = !hasN ? 0 : (j == n1 - 1) ? -(long long)(n1 - 1) * sJ : sJ;
+    const long long dB = !hasB ? 0 : (k == 0) ? (long long)(n2 - 1) : -1;            // from cell k
+    const long long dT = !hasT ? 0 : (k + 3 == n2 - 1) ? -(long long)(n2 - 1) : 1;   // from cell k+3
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    auto ld4 = [](const float* p) { return *reinterpret_cast<const f4*>(p); };
+    auto ld4nt = [](const float* p) {
+        if constexpr (Epi::kNT) return __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+        else return *reinterpret_cast<const f4*>(p);
+    };
+    long long node = (static_cast<long long>(i0) * n1 + j) * n2 + k;
+    f4 xp, xc;
+    {
+        const bool need = !(i0 == 0 && V.rm_lo[0]);
+        const long long nprev = !need ? node : (i0 == 0) ? node + (long long)(n0 - 1) * sI : node - sI;
+        xc = ld4(x + node);
+        xp = ld4(x + nprev);
+    }
+    for (int i = i0; i < i1; i++) {
+        const bool hasW = !(i == 0 && V.rm_lo[0]);
+        const bool hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
+        const long long nnext = !hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
+        const f4 xn = ld4(x + nnext), xs = ld4(x + node + dS), xq = ld4(x + node + dN);
+        const float xb = x[node + dB], xt = x[node + 3 + dT];
+        f4 d[7];
+#pragma unroll
+        for (int sl = 0; sl < 7; sl++) d[sl] = ld4nt(D + sl * nr + node);
+        const double p0 = epi.load(node), p1 = epi.load(node + 1), p2 = epi.load(node + 2), p3 = epi.load(node + 3);
+        const float xcv[4] = {xc.x, xc.y, xc.z, xc.w}, xpv[4] = {xp.x, xp.y, xp.z, xp.w};
+        const float xsv[4] = {xs.x, xs.y, xs.z, xs.w}, xqv[4] = {xq.x, xq.y, xq.z, xq.w};
+        const float xnv[4] = {xn.x, xn.y, xn.z, xn.w};
+        double y[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            auto dv = [&](int sl) { return m == 0 ? d[sl].x : m == 1 ? d[sl].y : m == 2 ? d[sl].z : d[sl].w; };
+            float s = 0.f;
+            s += hasW ? dv(0) * xpv[m] : 0.f;
+            s += hasS ? dv(1) * xsv[m] : 0.f;
+            if (m == 0) s += hasB ? dv(2) * xb : 0.f;
+            else s += dv(2) * xcv[m - 1];
+            s += dv(3) * xcv[m];
+            if (m == 3) s += hasT ? dv(4) * xt : 0.f;
+            else s += dv(4) * xcv[m + 1];
+            s += hasN ? dv(5) * xqv[m] : 0.f;
+            s += hasE ? dv(6) * xnv[m] : 0.f;
+            y[m] = static_cast<double>(s);
+        }
+        const double xcd[4] = {xcv[0], xcv[1], xcv[2], xcv[3]}, pre[4] = {p0, p1, p2, p3};
+        epi_quad(epi, node, y, xcd, pre, acc);
+        xp = xc;
+        xc = xn;
+        node += sI;
+    }
+}
+
 template <int NC, typename VT, typename XT, class Epi, int PAIR = 1>
 __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
                                                        KState* st, int tail, int check_status) {
@@ -571,8 +676,14 @@ __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* _
         if (k < A.V.n[2] && j < A.V.n[1]) {
             const int i0 = ti * A.tile_i;
             const int i1 = (i0 + A.tile_i < A.V.n[0]) ? i0 + A.tile_i : A.V.n[0];
-            if constexpr (PAIR == 2) dia_column_pair<VT, XT, Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
-            else dia_column<NC, VT, XT, Epi>(A.V, A.D, x, 0, k, j, i0, i1, epi, acc);
+            if constexpr (PAIR == 4) {
+                static_assert(std::is_same_v<VT, float> && std::is_same_v<XT, float>, "quad traversal is FP32");
+                dia_column_quad<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
+            } else if constexpr (PAIR == 2) {
+                dia_column_pair<VT, XT, Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
+            } else {
+                dia_column<NC, VT, XT, Epi>(A.V, A.D, x, 0, k, j, i0, i1, epi, acc);
+            }
         }
     }
     if (tail >= 0) grid_reduce<Epi::NDOT>(acc, ws, st, run ? tail : TAIL_NONE);
