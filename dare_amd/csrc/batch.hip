@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kThreads) void kb_finish(const BSys* __restrict__ s
             c->outer_active = 0;
         } else {                  // next inner FP32 solve on the residual
             c->active = 1;
-            c->tol_abs = P.inner_tol * rn;
+            c->tol_abs = inner_threshold(P.inner_tol, rn, c->outer_tol);
             c->max_it = P.inner_max_it;
             c->pass = 1;
             c->status = 0;

@@ -152,6 +152,15 @@ __device__ __forceinline__ void put(T* p, T v) {
 // rows from which a solve streams (an FP64 vector of this length is larger than the eight
 // XCD L2 caches together); below it, outputs stay cached for the next kernel of the pass
 constexpr long long kStreamRows = 1LL << 22;
+// Mixed-precision refinement: an inner FP32 solve stops at max(inner_tol * ||r||,
+// kLastInnerFrac * outer threshold). Once inner_tol * ||r|| is below the outer threshold the
+// last inner solve only has to reach a fraction of that threshold, so a closer initial guess
+// ends the last refinement early instead of paying a full inner_tol reduction.
+constexpr double kLastInnerFrac = 0.25;
+__host__ __device__ inline double inner_threshold(double inner_tol, double rn, double outer_abs) {
+    const double t = inner_tol * rn, floor_t = kLastInnerFrac * outer_abs;
+    return t > floor_t ? t : floor_t;
+}
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll

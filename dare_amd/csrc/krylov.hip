@@ -543,8 +543,9 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         int it = 0, ist = 0, dummy_len = 0;
         double irn = 0.;
         bool iconv = false;
-        rc = bicgstab2_run<float, float>(ctx, op32, r32, d32, inner_tol * rn, inner_max, nullptr, 0, &dummy_len, &it,
-                                         &irn, &ist, &iconv, work, 2, true);   // d32 = 0 to start
+        rc = bicgstab2_run<float, float>(ctx, op32, r32, d32, inner_threshold(inner_tol, rn, tol_abs), inner_max,
+                                         nullptr, 0, &dummy_len, &it, &irn, &ist, &iconv, work, 2,
+                                         true);   // d32 = 0 to start
         if (rc) return rc;
         total_it += it;
         if (pre)    // the inner solution is y = M d

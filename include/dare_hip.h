@@ -88,7 +88,9 @@ typedef struct {
     double tol;              /* convergence tolerance */
     int32_t tol_kind;        /* 0: ||r|| <= tol (reference RNORM), 1: ||r|| <= tol * ||b|| */
     int32_t max_it;          /* maximum BiCGStab2 passes */
-    double inner_tol;        /* mixed precision: inner FP32 relative reduction per refinement */
+    double inner_tol;        /* mixed precision: inner FP32 relative reduction per refinement;
+                              * an inner solve stops at max(inner_tol * ||r||, tol_abs / 4), so the
+                              * last one reduces only as far as the outer threshold needs */
     int32_t max_refine;      /* mixed precision: maximum refinement steps */
     int32_t inner_max_it;    /* mixed precision: passes per inner solve */
     double* history;         /* optional host array, residual norm after every pass */
