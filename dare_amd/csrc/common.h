@@ -110,7 +110,8 @@ struct dare_csr {
     // the FP32 copy for mixed solves: once a mixed solve has asked for it (dia32_mode), every
     // fill writes it next to the FP64 slots; dia32_fresh says it holds float(dia) of the last fill
     int dia32_mode = 0, dia32_fresh = 0;
-    // Jacobi right preconditioning: 1 / diag(A) and the column-scaled FP64 view A diag(A)^-1
+    // Jacobi right preconditioning: 1 / diag(A) and the column-scaled FP64 operator A diag(A)^-1
+    // (diagonal-view slots, or CSR values for a matrix without a diagonal view)
     double* dinv = nullptr;
     double* dia_pre = nullptr;
     int64_t cap_dinv = 0, cap_dia_pre = 0;

@@ -254,6 +254,28 @@ __global__ __launch_bounds__(kThreads) void k_dia_colscale(DiaView V, const doub
     }
 }
 
+// 1 / diag(A) from the CSR rows (columns ascending, duplicates merged); a row without a
+// diagonal entry gets 1 / 0 = inf, as 1 / diag does on the diagonal view
+__global__ __launch_bounds__(kThreads) void k_csr_dinv(long long n, const int32_t* __restrict__ rp,
+                                                       const int32_t* __restrict__ col,
+                                                       const double* __restrict__ val, double* __restrict__ dinv) {
+    for (long long r = blockIdx.x * (long long)kThreads + threadIdx.x; r < n; r += (long long)gridDim.x * kThreads) {
+        double d = 0.;
+        for (int j = rp[r]; j < rp[r + 1]; j++)
+            if (col[j] == r) d = val[j];
+        dinv[r] = 1.0 / d;
+    }
+}
+
+// out[j] = val[j] * dinv[col[j]]: the CSR values of A diag(A)^-1
+template <typename OT>
+__global__ __launch_bounds__(kThreads) void k_csr_colscale(long long nnz, const int32_t* __restrict__ col,
+                                                           const double* __restrict__ val,
+                                                           const double* __restrict__ dinv, OT* __restrict__ out) {
+    for (long long j = blockIdx.x * (long long)kThreads + threadIdx.x; j < nnz; j += (long long)gridDim.x * kThreads)
+        out[j] = static_cast<OT>(val[j] * dinv[col[j]]);
+}
+
 __global__ __launch_bounds__(kThreads) void k_scale_by(long long n, double* __restrict__ x, const double* __restrict__ s,
                                                        int invert) {
     for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
@@ -282,13 +304,26 @@ int grow_buffer(T** p, int64_t* cap, int64_t need, const char* what) {
     return DARE_OK;
 }
 
-// 1 / diag(A) into A->dinv (slot C of the diagonal view)
+// 1 / diag(A) into A->dinv: slot C of the diagonal view, or the CSR diagonal of a matrix
+// without one (its values must be present)
 int jacobi_setup(dare_ctx* ctx, dare_csr* A) {
-    DARE_CHECK_ARG(A->has_dia, "Jacobi preconditioning needs a matrix with a diagonal view (dare_csr_build_structure)");
     const long long n = A->nrows;
     int rc = grow_buffer(&A->dinv, &A->cap_dinv, n, "Jacobi diagonal");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, A->dia + 3 * n, A->dinv);
+    if (A->has_dia)
+        hipLaunchKernelGGL(k_dinv, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, A->dia + 3 * n, A->dinv);
+    else
+        hipLaunchKernelGGL(k_csr_dinv, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, A->row_ptr, A->col,
+                           A->val, A->dinv);
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+// A diag(A)^-1 as CSR values (FP64 into A->dia_pre, or FP32 into A->val32); dinv must be set
+template <typename OT>
+int jacobi_csr_values(dare_ctx* ctx, dare_csr* A, OT* out) {
+    hipLaunchKernelGGL((k_csr_colscale<OT>), dim3(grid_for(A->nnz)), dim3(kThreads), 0, ctx->stream,
+                       (long long)A->nnz, A->col, A->val, A->dinv, out);
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }
@@ -396,23 +431,30 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     SolveOp<double> op = op64(A, true);
     const bool pre = p->precond == 1;
     DARE_CHECK_ARG(p->precond == 0 || p->precond == 1, "precond must be 0 (none) or 1 (Jacobi)");
-    if (pre) {
-        DARE_CHECK_ARG(op.dia, "Jacobi preconditioning needs a matrix with a diagonal view");
-        dare_csr* Am = const_cast<dare_csr*>(A);
-        rc = jacobi_setup(ctx, Am);
-        if (rc) return rc;
-        const int slots = A->dia_view.cross ? 8 : 7;
-        rc = grow_buffer(&Am->dia_pre, &Am->cap_dia_pre, static_cast<int64_t>(slots) * n, "preconditioned view");
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_dia_colscale<double>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, A->dia_view,
-                           A->dia, A->dinv, Am->dia_pre);
-        hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 0);  // y0 = M x0
-        DARE_LAUNCH_CHECK();
-        op.d = make_dia<double>(A->dia_view, Am->dia_pre, true);
-    }
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
         if (rc) return rc;
+    }
+    if (pre) {
+        dare_csr* Am = const_cast<dare_csr*>(A);
+        rc = jacobi_setup(ctx, Am);
+        if (rc) return rc;
+        if (op.dia) {
+            const int slots = A->dia_view.cross ? 8 : 7;
+            rc = grow_buffer(&Am->dia_pre, &Am->cap_dia_pre, static_cast<int64_t>(slots) * n, "preconditioned view");
+            if (rc) return rc;
+            hipLaunchKernelGGL((k_dia_colscale<double>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream,
+                               A->dia_view, A->dia, A->dinv, Am->dia_pre);
+            op.d = make_dia<double>(A->dia_view, Am->dia_pre, true);
+        } else {                  // generic CSR: scaled copy of the values
+            rc = grow_buffer(&Am->dia_pre, &Am->cap_dia_pre, A->nnz, "preconditioned values");
+            if (rc) return rc;
+            rc = jacobi_csr_values<double>(ctx, Am, Am->dia_pre);
+            if (rc) return rc;
+            op.csr.val = Am->dia_pre;
+        }
+        hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 0);  // y0 = M x0
+        DARE_LAUNCH_CHECK();
     }
     int iters = 0, status = 0, hl = 0;
     double rn = 0.;
@@ -445,7 +487,6 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     op32.n = n;
     DARE_CHECK_ARG(p->precond == 0 || p->precond == 1, "precond must be 0 (none) or 1 (Jacobi)");
     const bool pre = p->precond == 1;
-    DARE_CHECK_ARG(!pre || A->has_dia, "Jacobi preconditioning needs a matrix with a diagonal view");
     if (A->has_dia && kXcd * make_dia<double>(A->dia_view, A->dia).per_xcd <= kReduceBlocks) {
         const int slots = A->dia_view.cross ? 8 : 7;
         const int64_t need = static_cast<int64_t>(slots) * n;
@@ -479,10 +520,6 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         op32.d = make_dia<float>(A->dia_view, A->dia32, true);
         op32.dia = kXcd * op32.d.per_xcd <= kReduceBlocks;
     } else {
-        if (pre) {
-            set_error("Jacobi preconditioning needs the diagonal-view operator (grid too large for it)");
-            return DARE_ERR_UNSUPPORTED;
-        }
         int erc = csr_ensure_values(A, ctx->stream);
         if (erc) return erc;
         if (A->cap_val32 < A->nnz) {
@@ -495,8 +532,15 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
             }
             A->cap_val32 = A->nnz;
         }
-        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->nnz)), dim3(kThreads), 0, ctx->stream, (long long)A->nnz, A->val,
-                           A->val32);
+        if (pre) {       // FP32 CSR values of A M^-1
+            erc = jacobi_setup(ctx, A);
+            if (erc) return erc;
+            erc = jacobi_csr_values<float>(ctx, A, A->val32);
+            if (erc) return erc;
+        } else {
+            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A->nnz)), dim3(kThreads), 0, ctx->stream, (long long)A->nnz,
+                               A->val, A->val32);
+        }
         op32.dia = false;
     }
     op32.csr = CsrOp<float>{A->row_ptr, A->col, A->val32, n};

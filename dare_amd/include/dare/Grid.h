@@ -1,6 +1,9 @@
 // dare::Grid::Cartesian<Dim> and its representation for one rank (no MPI decomposition).
 // Mirrors Grid/Cartesian/CartesianMesh.inl:35-112 and CartesianRepresentation.inl:32-534:
 // cell width L/n, volume, face areas; ghost-padded local indices with the last index fastest.
+// A representation may be staggered (Options, CartesianMesh.h:209,283): along every axis with
+// opt[d] != 0 its cells sit on the faces of the grid's cells, so a non-periodic axis gains one
+// cell at its upper end and its centres move by -dx/2 (CartesianRepresentation.inl:50-77).
 #pragma once
 
 #include <array>
@@ -31,6 +34,7 @@ public:
     using VecLO = std::array<int32_t, Dim>;
     using Representation = CartesianRepresentation<Dim>;
     using NeighborID = CartesianNeighbor;
+    using Options = VecLO;    // per axis: 0 cell centred, != 0 staggered
 
     Cartesian(const IndexGlobal& res, const VecSC& size, int32_t num_ghost, const VecLO& periodic = VecLO{})
         : res_(res), size_(size), num_ghost_(num_ghost), periodic_(periodic) {
@@ -44,7 +48,7 @@ public:
         for (std::size_t d = 0; d < Dim; d++) area_[d] = volume_ / width_[d];
     }
 
-    Representation GetRepresentation() const { return Representation(this); }
+    Representation GetRepresentation(const Options& opt = Options{}) const { return Representation(this, opt); }
     // the grid's name (CartesianMesh.h: the first constructor argument); output groups data by it
     const std::string& GetName() const { return name_; }
     void SetName(std::string name) { name_ = std::move(name); }
@@ -78,12 +82,22 @@ public:
     using GO = int64_t;
     using Index = typename GridType::Index;
     using VecSC = typename GridType::VecSC;
+    using Options = typename GridType::Options;
 
     CartesianRepresentation() = default;
-    explicit CartesianRepresentation(const GridType* g) : grid_(g) {
+    explicit CartesianRepresentation(const GridType* g, const Options& opt = Options{}) : grid_(g), opt_(opt) {
+        static const char axis[3] = {'X', 'Y', 'Z'};
+        name_ = g->GetName();
+        for (std::size_t d = 0; d < Dim; d++)
+            if (opt[d] != 0) name_ += std::string("_") + axis[d];
         const int32_t ng = g->GetNumGhost();
         for (std::size_t d = 0; d < Dim; d++) {
             res_int_[d] = static_cast<LO>(g->GetGlobalResolution()[d]);
+            offset_[d] = 0.;
+            if (opt[d] != 0 && g->GetPeriodicity()[d] == 0) {   // one rank owns the upper boundary
+                res_int_[d] += 1;
+                offset_[d] = -0.5 * g->GetCellWidth()[d];
+            }
             res_[d] = res_int_[d] + 2 * ng;
         }
         stride_[Dim - 1] = stride_int_[Dim - 1] = 1;          // hierarchic sums, :88-112
@@ -94,6 +108,17 @@ public:
     }
 
     const GridType* GetGrid() const { return grid_; }
+    // the grid's name with "_X"/"_Y"/"_Z" for each staggered axis; output groups data by it
+    const std::string& GetName() const { return name_; }
+    const Options& GetOptions() const { return opt_; }
+    bool IsStaggered() const {
+        for (auto o : opt_)
+            if (o) return true;
+        return false;
+    }
+    // position of the first internal cell's lower face (0 on one rank, -dx/2 on a staggered axis)
+    const VecSC& GetOffsetSize() const { return offset_; }
+    int32_t GetNumberGhostCells() const { return grid_->GetNumGhost(); }
     LO GetNumberLocalCells() const { return product(res_); }
     LO GetNumberLocalCellsInternal() const { return product(res_int_); }
     GO GetNumberGlobalCellsInternal() const { return GetNumberLocalCellsInternal(); }
@@ -134,6 +159,7 @@ public:
         return ind;
     }
     GO MapLocalToGlobalInternal(LO n) const { return n; }          // one rank: global == local
+    Index MapOrdinalToIndexGlobalInternal(GO n) const { return MapOrdinalToIndexLocalInternal(static_cast<LO>(n)); }
     LO MapGlobalToLocalInternal(GO n) const { return static_cast<LO>(n); }
     // MapIndexToOrdinalGlobalInternal with the periodic wrap of :418-450
     GO MapIndexToOrdinalGlobalInternal(Index ind, bool map_periodic) const {
@@ -155,11 +181,13 @@ public:
     VecSC GetCoordinatesCenter(const Index& ind) const {         // :230-238
         VecSC pos;
         for (std::size_t d = 0; d < Dim; d++)
-            pos[d] = 0. + (ind[d] - grid_->GetNumGhost() + 0.5) * grid_->GetCellWidth()[d];
+            pos[d] = offset_[d] + (ind[d] - grid_->GetNumGhost() + 0.5) * grid_->GetCellWidth()[d];
         return pos;
     }
 
-    // the C-ABI grid descriptor of this representation for an N-component field
+    // the C-ABI grid descriptor of this representation for an N-component field. The device
+    // derives the cell width as size / n, so a staggered axis (n + 1 cells) reports the
+    // extent (n + 1) dx that keeps dx, volume and face areas those of the grid.
     dare_grid Descriptor(int ncomp) const {
         dare_grid g{};
         g.dim = static_cast<int32_t>(Dim);
@@ -169,7 +197,10 @@ public:
             const bool act = d < static_cast<int>(Dim);
             g.n[d] = act ? res_int_[d] : 1;
             g.periodic[d] = act ? grid_->GetPeriodicity()[d] : 0;
-            g.size[d] = act ? grid_->GetGlobalSize()[d] : 1.;
+            g.size[d] = act ? (res_int_[d] == grid_->GetGlobalResolution()[d]
+                                   ? grid_->GetGlobalSize()[d]
+                                   : grid_->GetCellWidth()[d] * res_int_[d])
+                            : 1.;
         }
         return g;
     }
@@ -181,6 +212,9 @@ private:
         return p;
     }
     const GridType* grid_ = nullptr;
+    Options opt_{};
+    std::string name_;
+    VecSC offset_{};
     Index res_{}, res_int_{};
     std::array<LO, Dim> stride_{}, stride_int_{};
 };

@@ -1,9 +1,10 @@
 // dare::io::VTKWriter for Cartesian grid vectors, without the VTK library.
-// Mirrors IO/VTKWriter.h:73-157 / VTKWriter.inl:38-95 (one writer per output step, data grouped
-// by grid, one cell-data array per grid vector named after it, component names, a TimeValue
-// field-data entry when time >= 0) and the Cartesian options of
-// Grid/Cartesian/VTKOptions_Cartesian.h:144-204,264-322: points at the cell vertices of the
-// ghost-padded local grid, offset by -g*dx; VTK's x-fastest cell order (the Mapper of :78-118);
+// Mirrors IO/VTKWriter.h:73-157 / VTKWriter.inl:38-95,124-137 (one writer per output step, data
+// grouped by representation name, so a staggered field gets its own grid; one cell-data array
+// per grid vector named after it, component names, a TimeValue field-data entry when
+// time >= 0) and the Cartesian options of Grid/Cartesian/VTKOptions_Cartesian.h:144-204,264-322:
+// points at the cell vertices of the ghost-padded local grid, offset by the representation's
+// offset (-dx/2 on a staggered axis) minus g*dx; VTK's x-fastest cell order (the Mapper of :78-118);
 // 1-D grids written as a pseudo 2-D grid one cell thick with dy = dx; whole extent = the
 // internal cells 0..n, piece extent = the local cells including the ghost layers counted from
 // the rank's cell offset (0 on one rank), i.e. 0..n+2g as GetPExtentLocal computes it, ghost
@@ -96,7 +97,7 @@ public:
         bool ok = true;
         auto collect = [&](auto* v) {
             const Representation& grep = v->GetGridRepresentation();
-            const std::string gname = grep.GetGrid()->GetName();
+            const std::string gname = grep.GetName();   // staggered representations apart
             reps.emplace(gname, &grep);
             grouped[gname].push_back(Gather(*v));
         };
@@ -145,7 +146,7 @@ private:
         for (std::size_t d = 0; d < Dim; d++) {
             npt[d] = res[d] + 1;
             dx[d] = dn[d];
-            off[d] = -static_cast<double>(g) * dn[d];
+            off[d] = grep.GetOffsetSize()[d] - static_cast<double>(g) * dn[d];
             whole[2 * d + 1] = res_int[d];
             piece[2 * d + 1] = res[d];
         }

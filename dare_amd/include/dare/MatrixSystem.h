@@ -297,6 +297,12 @@ struct Terms {
     Terms& Neumann(int face) { t.bc_type[face] = DARE_BC_NEUMANN; return *this; }
 };
 
+// A built preconditioner (the RCP<OperatorType> M of the reference): which right
+// preconditioner the device solver applies (dare_solver_params.precond)
+struct Preconditioner {
+    int32_t precond;
+};
+
 // ------------------------------------------------------------------------------ Trilinos
 template <typename SC>
 class Trilinos {
@@ -370,7 +376,7 @@ public:
     std::shared_ptr<MatrixType>& GetA() { return A_; }
     std::shared_ptr<VecType>& GetX() { return x_; }
     std::shared_ptr<VecType>& GetB() { return B_; }
-    std::shared_ptr<void>& GetM() { return M_; }   // no preconditioner in this build
+    std::shared_ptr<Preconditioner>& GetM() { return M_; }   // null, as Trilinos::GetM before one is set
 
     // Trilinos.inl:168-177: x[node*N + n] -> field.At(MapInternalToLocal(node), n), on the device
     template <typename Grid, std::size_t N>
@@ -385,7 +391,7 @@ private:
     std::shared_ptr<Device> dev_;
     std::shared_ptr<MatrixType> A_;
     std::shared_ptr<VecType> x_, B_;
-    std::shared_ptr<void> M_;
+    std::shared_ptr<Preconditioner> M_;
     bool structured_ = false;
 };
 
@@ -398,13 +404,13 @@ template <typename SC>
 class TrilinosSolver {
 public:
     // TrilinosSolver.inl:71-93 + CreateSolver(BumbleBee, "BICGSTAB2") (:150-152)
-    ReturnType Solve(SolverPackage pack, const std::string& type, const std::shared_ptr<void>& M,
+    ReturnType Solve(SolverPackage pack, const std::string& type, const std::shared_ptr<Preconditioner>& M,
                      const std::shared_ptr<DeviceCsr>& A, const std::shared_ptr<DeviceVector>& x,
                      const std::shared_ptr<DeviceVector>& B, const std::shared_ptr<ParameterList>& params) {
         if (pack != SolverPackage::BumbleBee || type != "BICGSTAB2")
             throw std::runtime_error("this build provides BumbleBee/BICGSTAB2 only (got '" + type + "')");
-        if (M) throw std::runtime_error("preconditioners are not provided by this build");
         dare_solver_params p{};
+        p.precond = M ? M->precond : 0;      // right preconditioner (BiCGStab2.inl:353-365)
         p.tol = params ? params->get<double>("Convergence Tolerance", 1e-8) : 1e-8;   // BiCGStab2.inl:29-30
         p.max_it = params ? params->get<int>("Maximum Iterations", 100) : 100;
         p.tol_kind = 0;                                                               // RNORM (:34)
@@ -420,11 +426,21 @@ public:
                      const std::shared_ptr<ParameterList>& params) {
         return Solve(pack, type, nullptr, A, x, B, params);
     }
-    std::shared_ptr<void> BuildPreconditioner(PreCondPackage pack, const std::string&,
-                                              const std::shared_ptr<ParameterList>&,
-                                              const std::shared_ptr<DeviceCsr>&) {
-        if (pack != PreCondPackage::None) throw std::runtime_error("preconditioners are not provided by this build");
-        return nullptr;
+    // TrilinosSolver.inl:97-127 + CreatePreconditionerIfPack2 (:164-175). This build provides
+    // Ifpack2 "RELAXATION" of type "Jacobi" with one sweep and damping 1, i.e. M^-1 = diag(A)^-1,
+    // which the device solver applies on the right; None returns null as the reference does.
+    // Other types throw (the reference prints an error and terminates).
+    std::shared_ptr<Preconditioner> BuildPreconditioner(PreCondPackage pack, const std::string& type,
+                                                        const std::shared_ptr<ParameterList>& param,
+                                                        const std::shared_ptr<DeviceCsr>&) {
+        if (pack == PreCondPackage::None) return nullptr;
+        const std::string relax = param ? param->get<std::string>("relaxation: type", "Jacobi") : "Jacobi";
+        const int sweeps = param ? param->get<int>("relaxation: sweeps", 1) : 1;
+        const double omega = param ? param->get<double>("relaxation: damping factor", 1.) : 1.;
+        if (pack == PreCondPackage::Ifpack2 && type == "RELAXATION" && relax == "Jacobi" && sweeps == 1 && omega == 1.)
+            return std::make_shared<Preconditioner>(Preconditioner{DARE_PRECOND_JACOBI});
+        throw std::runtime_error("Preconditioner of type " + type +
+                                 " is unknown to this build (Ifpack2 RELAXATION / Jacobi, one sweep, damping 1)");
     }
     int GetNumIterations() const { return num_iter_; }      // TrilinosSolver.inl:263-265
     double GetResidual() const { return residual_; }

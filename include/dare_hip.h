@@ -48,6 +48,10 @@ extern "C" {
 #define DARE_BC_DIRICHLET 1
 #define DARE_BC_NEUMANN 2
 
+/* Right preconditioners of the solver (dare_solver_params.precond). */
+#define DARE_PRECOND_NONE 0
+#define DARE_PRECOND_JACOBI 1   /* M = diag(A): Ifpack2 "RELAXATION", Jacobi, one sweep, damping 1 */
+
 typedef struct dare_ctx dare_ctx;
 typedef struct dare_csr dare_csr;
 
@@ -97,7 +101,8 @@ typedef struct {
     int32_t history_cap;
     int32_t precond;         /* 0: none (the reference default, M == null); 1: Jacobi, the right
                               * preconditioner M = diag(A) (BiCGStab2.inl:353-365 applies M^-1 on
-                              * the right); matrices with a diagonal view only */
+                              * the right); the diagonal-view operator is column-scaled, a
+                              * generic CSR matrix gets a scaled copy of its values */
 } dare_solver_params;
 
 typedef struct {

@@ -8,6 +8,15 @@
 //   VTKWrite          — IO/test/test_VTKWriter.cpp pattern: a field changed on the device
 //                       (periodic ghost exchange) written through dare::io::VTKWriter into
 //                       argv[1] (checked by tests/test_host_mirror.py)
+//   StaggeredStorage  — test/test_integration_Cartesian_GridVector.cpp:104-158 and
+//                       test_integration_Cartesian_Field.cpp:84-142 (sizes and names of
+//                       staggered representations), plus their cell positions
+//   SolveStaggered1D  — test/test_integration_Cartesian_Trilinos.cpp:155-220
+//   SolveStaggered2D  — test/test_integration_Cartesian_Trilinos.cpp:410-585 (X and Y)
+//   DeviceStaggered   — the face gradient of a y-staggered linear field on the device: the
+//                       descriptor of a staggered representation keeps the grid's dx
+//   SolveJacobi       — SolveLaplace with Ifpack2 RELAXATION/Jacobi as M
+//                       (TrilinosSolver.inl:97-175), and the device fast path with and without it
 // Exit code 0 when every case passes.
 #include <cmath>
 #include <cstdio>
@@ -75,7 +84,7 @@ static void TrilinosBuild() {
     }
 }
 
-static void SolveLaplace() {
+static void SolveLaplaceWith(Matrix::PreCondPackage pack, const std::string& ptype) {
     constexpr std::size_t N = 4;
     using G = dare::Grid::Cartesian<1>;
     G grid({23}, {1.}, 1);
@@ -124,7 +133,10 @@ static void SolveLaplace() {
     params->set("Convergence Tolerance", 1e-12);
     params->set("Maximum Iterations", 5000);
     Matrix::TrilinosSolver<double> solver;
-    auto M = solver.BuildPreconditioner(Matrix::PreCondPackage::None, "none", nullptr, trilinos.GetA());
+    auto pparams = std::make_shared<ParameterList>();
+    pparams->set("relaxation: type", std::string("Jacobi"));
+    auto M = solver.BuildPreconditioner(pack, ptype, pparams, trilinos.GetA());
+    EXPECT((pack == Matrix::PreCondPackage::None) == !M, "preconditioner %s", ptype.c_str());
     auto ret = solver.Solve(Matrix::SolverPackage::BumbleBee, "BICGSTAB2", M, trilinos.GetA(), trilinos.GetX(),
                             trilinos.GetB(), params);
     EXPECT(ret == Matrix::ReturnType::Converged, "SolveLaplace did not converge");
@@ -134,7 +146,60 @@ static void SolveLaplace() {
             const double phi = (node + 0.5) / 23.;
             EXPECT(std::fabs(phi - x[node * N + n]) < 1e-8, "node %d comp %zu: %g vs %g", node, n, x[node * N + n], phi);
         }
-    std::printf("  SolveLaplace: %d passes\n", solver.GetNumIterations());
+    std::printf("  SolveLaplace (%s): %d passes\n", ptype.c_str(), solver.GetNumIterations());
+}
+
+static void SolveLaplace() { SolveLaplaceWith(Matrix::PreCondPackage::None, "none"); }
+
+// Ifpack2 RELAXATION / Jacobi as the right preconditioner M of TrilinosSolver::Solve
+// (TrilinosSolver.inl:71-127): on the generic host-functor matrix (scaled CSR values) and on
+// the device fast path (scaled diagonal view); same solutions as without it
+static void SolveJacobi() {
+    SolveLaplaceWith(Matrix::PreCondPackage::Ifpack2, "RELAXATION");
+    Matrix::TrilinosSolver<double> solver;
+    bool threw = false;
+    try {
+        solver.BuildPreconditioner(Matrix::PreCondPackage::Ifpack2, "ILUT", nullptr, nullptr);
+    } catch (const std::exception&) {
+        threw = true;
+    }
+    EXPECT(threw, "an unknown preconditioner type must be refused");
+    using G = dare::Grid::Cartesian<3>;
+    const int n = 20;
+    G grid({n, n + 2, n + 4}, {1., 1.1, 1.2}, 2);
+    auto grep = grid.GetRepresentation();
+    Data::GridVector<G, double, 1> phi("phi", grep);
+    const auto res = grep.GetLocalResolution();
+    for (int i = 0; i < res[0]; i++)
+        for (int j = 0; j < res[1]; j++)
+            for (int k = 0; k < res[2]; k++) phi.At({i, j, k}, 0) = std::sin(0.3 * i) + std::cos(0.2 * j * k);
+    Matrix::Terms terms;
+    terms.DDT(0.05).Diffusion(1e-2);
+    for (int f = 0; f < 6; f++) terms.Dirichlet(f, 0.25 * f);
+    auto params = std::make_shared<ParameterList>();
+    params->set("Maximum Iterations", 500);
+    std::vector<double> sol[2];
+    int passes[2];
+    for (int use = 0; use < 2; use++) {
+        Matrix::Trilinos<double> msystem;
+        msystem.Build(grep, phi, terms, false);
+        params->set("Convergence Tolerance", 1e-13 * msystem.GetB()->norm2());
+        auto M = use ? solver.BuildPreconditioner(Matrix::PreCondPackage::Ifpack2, "RELAXATION", nullptr, msystem.GetA())
+                     : nullptr;
+        auto ret = solver.Solve(Matrix::SolverPackage::BumbleBee, "BICGSTAB2", M, msystem.GetA(), msystem.GetX(),
+                                msystem.GetB(), params);
+        EXPECT(ret == Matrix::ReturnType::Converged, "device solve %d did not converge", use);
+        sol[use] = msystem.GetX()->getData();
+        passes[use] = solver.GetNumIterations();
+    }
+    double err = 0., mx = 0.;
+    for (std::size_t r = 0; r < sol[0].size(); r++) {
+        err = std::fmax(err, std::fabs(sol[0][r] - sol[1][r]));
+        mx = std::fmax(mx, std::fabs(sol[0][r]));
+    }
+    EXPECT(err <= 1e-10 * mx, "Jacobi solution differs by %g (max %g)", err, mx);
+    std::printf("  SolveJacobi device path: %d passes plain, %d with Jacobi, difference %.1e\n", passes[0], passes[1],
+                err / mx);
 }
 
 static void SolveScalar1D() {
@@ -250,6 +315,206 @@ static void DeviceDiffusion3D() {
     std::printf("  DeviceDiffusion3D: max error vs heat kernel %.2e\n", err);
 }
 
+static void StaggeredStorage() {
+    constexpr std::size_t N = 3;
+    static const char axis[3] = {'X', 'Y', 'Z'};
+    using G = dare::Grid::Cartesian<3>;
+    G grid({100, 101, 102}, {1., 2., 3.}, 2);
+    for (int d = 0; d < 3; d++) {
+        G::Options opt{0, 0, 0};
+        opt[d] = 1;
+        auto rep = grid.GetRepresentation(opt);
+        Data::GridVector<G, double, N> data("test", rep);
+        int64_t cells = 1;
+        for (int e = 0; e < 3; e++) cells *= 100 + e + (e == d) + 4;
+        EXPECT(rep.GetNumberLocalCells() == cells, "axis %d: %d local cells, expected %lld", d,
+               rep.GetNumberLocalCells(), (long long)cells);
+        EXPECT(data.GetSize() == static_cast<std::size_t>(rep.GetNumberLocalCells()) * N, "axis %d: size", d);
+        EXPECT(rep.GetGlobalResolutionInternal()[d] == 100 + d + 1, "axis %d: internal cells", d);
+        EXPECT(rep.GetName() == std::string("Cartesian3D_") + axis[d], "axis %d: name %s", d, rep.GetName().c_str());
+        // the first internal cell sits on the lower boundary face, the last on the upper one
+        G::Index lo{2, 2, 2}, hi{2, 2, 2};
+        hi[d] = 2 + 100 + d;
+        const double L = 1. + d;
+        EXPECT(std::fabs(rep.GetCoordinatesCenter(lo)[d]) < 1e-15, "axis %d: first centre %g", d,
+               rep.GetCoordinatesCenter(lo)[d]);
+        EXPECT(std::fabs(rep.GetCoordinatesCenter(hi)[d] - L) < 1e-13, "axis %d: last centre %g", d,
+               rep.GetCoordinatesCenter(hi)[d]);
+    }
+    {   // Field of a staggered 1-D representation: level names and sizes
+        using G1 = dare::Grid::Cartesian<1>;
+        G1 grid1({100}, {1.}, 2);
+        auto rep_s = grid1.GetRepresentation({1});
+        Data::Field<G1, double, N> field_s("test_1D_staggered", rep_s, 2);
+        for (std::size_t t = 0; t < 2; t++) {
+            const auto& v = field_s.GetDataVector(t);
+            EXPECT(v.GetSize() == static_cast<std::size_t>(rep_s.GetNumberLocalCells()) * N, "level %zu size", t);
+            EXPECT(v.GetSize() == (100 + 1 + 4) * N, "level %zu: %zu entries", t, v.GetSize());
+            EXPECT(v.GetName() == "test_1D_staggered_" + std::to_string(t), "level %zu name %s", t, v.GetName().c_str());
+        }
+    }
+    {   // a periodic axis has no boundary face to add (CartesianRepresentation.inl:66-68)
+        using G2 = dare::Grid::Cartesian<2>;
+        G2 grid2({10, 11}, {1., 1.}, 2, {1, 0});
+        auto rep = grid2.GetRepresentation({1, 1});
+        EXPECT(rep.GetGlobalResolutionInternal()[0] == 10 && rep.GetGlobalResolutionInternal()[1] == 12,
+               "periodic staggered: %d x %d", rep.GetGlobalResolutionInternal()[0], rep.GetGlobalResolutionInternal()[1]);
+        EXPECT(rep.GetName() == "Cartesian2D_X_Y", "name %s", rep.GetName().c_str());
+    }
+}
+
+static void SolveStaggered1D() {
+    constexpr std::size_t N = 3;
+    using G = dare::Grid::Cartesian<1>;
+    G grid({20}, {1.}, 2);
+    auto g_rep = grid.GetRepresentation({1});
+    Data::GridVector<G, double, N> data("test", g_rep);
+    const double value_west = 0., value_east = 1.;
+    const int64_t last = g_rep.GetGlobalResolutionInternal()[0] - 1;
+    auto functor = [&](auto mblock) {
+        const int64_t node_g = mblock->GetNode();
+        const bool west = node_g == 0, east = node_g == last;
+        for (std::size_t n = 0; n < mblock->GetNumComponents(); n++) {
+            mblock->Resize(n, (west || east) ? 1 : 3);
+            if (west) {
+                mblock->template Get<CN::CENTER>(n) = 1.;
+                mblock->GetRhs(n) = value_west;
+                mblock->SetInitialGuess(n, value_west);
+            } else if (east) {
+                mblock->template Get<CN::CENTER>(n) = 1.;
+                mblock->GetRhs(n) = value_east;
+                mblock->SetInitialGuess(n, value_east);
+            } else {
+                mblock->template Get<CN::CENTER>(n) = 2.;
+                mblock->template Get<CN::WEST>(n) = -1.;
+                mblock->template Get<CN::EAST>(n) = -1.;
+                mblock->GetRhs(n) = 0.;
+            }
+        }
+        mblock->Finalize();
+    };
+    Matrix::Trilinos<double> trilinos;
+    Matrix::TrilinosSolver<double> solver;
+    auto params = std::make_shared<ParameterList>();
+    params->set("Convergence Tolerance", 1e-14);
+    params->set("Maximum Iterations", 5000);
+    trilinos.Build(g_rep, data, functor, false);
+    EXPECT(trilinos.GetX()->getLocalLength() == 21 * N, "rows %zu", trilinos.GetX()->getLocalLength());
+    auto ret = solver.Solve(Matrix::SolverPackage::BumbleBee, "BICGSTAB2", trilinos.GetM(), trilinos.GetA(),
+                            trilinos.GetX(), trilinos.GetB(), params);
+    EXPECT(ret == Matrix::ReturnType::Converged, "SolveStaggered1D did not converge");
+    const double d_phi = (value_east - value_west) / last;
+    auto x = trilinos.GetX()->getData();
+    for (int32_t node_l = 0; node_l < g_rep.GetNumberLocalCellsInternal(); node_l++) {
+        const auto ind = g_rep.MapOrdinalToIndexGlobalInternal(g_rep.MapLocalToGlobalInternal(node_l));
+        for (std::size_t n = 0; n < N; n++) {
+            const double phi = value_west + ind[0] * d_phi;
+            EXPECT(std::fabs(phi - x[node_l * N + n]) < 1e-8, "node %d: %g vs %g", node_l, x[node_l * N + n], phi);
+        }
+    }
+}
+
+// the 2-D functor of test_integration_Cartesian_Trilinos.cpp:427-470 with the staggered axis
+// `sd` carrying the Dirichlet values (identity rows) and the other axis closed (no neighbour)
+template <int SD>
+static void SolveStaggered2DAxis() {
+    constexpr std::size_t N = 3;
+    using G = dare::Grid::Cartesian<2>;
+    G grid({20, 21}, {1., 2.}, 2);
+    G::Options opt{0, 0};
+    opt[SD] = 1;
+    auto g_rep = grid.GetRepresentation(opt);
+    Data::GridVector<G, double, N> data("test", g_rep);
+    const double value_lo = 0., value_hi = 1.;
+    const auto res = g_rep.GetGlobalResolutionInternal();
+    constexpr int OD = 1 - SD;
+    auto functor = [&](auto mblock) {
+        const auto ind = mblock->GetIndexInternal();
+        const bool lo = ind[SD] == 0, hi = ind[SD] == res[SD] - 1;
+        const bool olo = ind[OD] == 0, ohi = ind[OD] == res[OD] - 1;
+        std::size_t stencil_size = 5;
+        if (lo || hi) {
+            stencil_size = 1;
+        } else {
+            stencil_size -= olo;
+            stencil_size -= ohi;
+        }
+        constexpr CN slo = SD == 0 ? CN::WEST : CN::SOUTH, shi = SD == 0 ? CN::EAST : CN::NORTH;
+        constexpr CN olo_nb = OD == 0 ? CN::WEST : CN::SOUTH, ohi_nb = OD == 0 ? CN::EAST : CN::NORTH;
+        for (std::size_t n = 0; n < mblock->GetNumComponents(); n++) {
+            mblock->Resize(n, stencil_size);
+            if (lo || hi) {
+                mblock->template Get<CN::CENTER>(n) = 1.;
+                mblock->GetRhs(n) = lo ? value_lo : value_hi;
+                mblock->SetInitialGuess(n, lo ? value_lo : value_hi);
+            } else {
+                mblock->template Get<CN::CENTER>(n) = (olo || ohi) ? 3. : 4.;
+                mblock->template Get<slo>(n) = -1.;
+                mblock->template Get<shi>(n) = -1.;
+                if (!olo) mblock->template Get<olo_nb>(n) = -1.;
+                if (!ohi) mblock->template Get<ohi_nb>(n) = -1.;
+                mblock->GetRhs(n) = 0.;
+            }
+        }
+        mblock->Finalize();
+    };
+    Matrix::Trilinos<double> trilinos;
+    Matrix::TrilinosSolver<double> solver;
+    auto params = std::make_shared<ParameterList>();
+    params->set("Convergence Tolerance", 1e-14);
+    params->set("Maximum Iterations", 5000);
+    trilinos.Build(g_rep, data, functor, false);
+    auto ret = solver.Solve(Matrix::SolverPackage::BumbleBee, "BICGSTAB2", trilinos.GetM(), trilinos.GetA(),
+                            trilinos.GetX(), trilinos.GetB(), params);
+    EXPECT(ret == Matrix::ReturnType::Converged, "SolveStaggered2D axis %d did not converge", SD);
+    const double d_phi = (value_hi - value_lo) / (res[SD] - 1);
+    auto x = trilinos.GetX()->getData();
+    double err = 0.;
+    for (int32_t node_l = 0; node_l < g_rep.GetNumberLocalCellsInternal(); node_l++) {
+        const auto ind = g_rep.MapOrdinalToIndexGlobalInternal(g_rep.MapLocalToGlobalInternal(node_l));
+        for (std::size_t n = 0; n < N; n++) err = std::fmax(err, std::fabs(value_lo + ind[SD] * d_phi - x[node_l * N + n]));
+    }
+    EXPECT(err < 1e-8, "SolveStaggered2D axis %d: max deviation %g", SD, err);
+    std::printf("  SolveStaggered2D axis %d: %d x %d cells, %d passes, max deviation %.1e\n", SD, res[0], res[1],
+                solver.GetNumIterations(), err);
+}
+
+static void SolveStaggered2D() {
+    SolveStaggered2DAxis<0>();
+    SolveStaggered2DAxis<1>();
+}
+
+static void DeviceStaggered() {
+    using G = dare::Grid::Cartesian<3>;
+    G grid({16, 12, 10}, {1., 1.2, 1.4}, 2);
+    auto rep = grid.GetRepresentation({0, 1, 0});
+    Data::GridVector<G, double, 1> v("v", rep);
+    const auto res = rep.GetLocalResolution();
+    for (int i = 0; i < res[0]; i++)
+        for (int j = 0; j < res[1]; j++)
+            for (int k = 0; k < res[2]; k++) v.At({i, j, k}, 0) = 3. * rep.GetCoordinatesCenter({i, j, k})[1];
+    const auto ni = rep.GetGlobalResolutionInternal();
+    EXPECT(ni[0] == 16 && ni[1] == 13 && ni[2] == 10, "staggered internal cells %d %d %d", ni[0], ni[1], ni[2]);
+    const dare_grid g = v.Descriptor();
+    auto dev = Device::Default();
+    const std::size_t nf[3] = {static_cast<std::size_t>(ni[0] + 1) * ni[1] * ni[2],
+                               static_cast<std::size_t>(ni[0]) * (ni[1] + 1) * ni[2],
+                               static_cast<std::size_t>(ni[0]) * ni[1] * (ni[2] + 1)};
+    DeviceBuffer<double> fx(dev, nf[0]), fy(dev, nf[1]), fz(dev, nf[2]);
+    Check(dare_grad_faces(dev->Get(), &g, v.GetDeviceView(), fx.data(), fy.data(), fz.data()), "dare_grad_faces");
+    std::vector<double> hx(nf[0]), hy(nf[1]), hz(nf[2]);
+    fx.Download(hx.data());
+    fy.Download(hy.data());
+    fz.Download(hz.data());
+    double ex = 0., ey = 0., ez = 0.;
+    for (double e : hx) ex = std::fmax(ex, std::fabs(e));
+    for (double e : hy) ey = std::fmax(ey, std::fabs(e - 3.));
+    for (double e : hz) ez = std::fmax(ez, std::fabs(e));
+    EXPECT(ex == 0. && ez == 0., "cross-axis gradients %g %g", ex, ez);
+    EXPECT(ey < 1e-12, "y gradient deviates from 3 by %g", ey);
+    std::printf("  DeviceStaggered: 16 x 13 x 10 y-staggered cells, |grad_y - 3| <= %.1e\n", ey);
+}
+
 static std::string g_vtk_dir;
 
 static void VTKWrite() {
@@ -276,7 +541,9 @@ int main(int argc, char** argv) {
     if (argc > 1) g_vtk_dir = argv[1];
     const std::vector<std::pair<const char*, std::function<void()>>> cases = {
         {"TrilinosBuild", TrilinosBuild}, {"SolveLaplace", SolveLaplace}, {"SolveScalar1D", SolveScalar1D},
-        {"HaloExchange", HaloExchange}, {"DeviceDiffusion3D", DeviceDiffusion3D}, {"VTKWrite", VTKWrite}};
+        {"HaloExchange", HaloExchange}, {"DeviceDiffusion3D", DeviceDiffusion3D}, {"VTKWrite", VTKWrite},
+        {"StaggeredStorage", StaggeredStorage}, {"SolveStaggered1D", SolveStaggered1D},
+        {"SolveStaggered2D", SolveStaggered2D}, {"DeviceStaggered", DeviceStaggered}, {"SolveJacobi", SolveJacobi}};
     for (const auto& c : cases) {
         const int before = g_fail;
         std::printf("[ RUN ] %s\n", c.first);

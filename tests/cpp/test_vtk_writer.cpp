@@ -72,6 +72,29 @@ int main(int argc, char** argv) {
         dare::io::VTKWriter<G> writer(1.5, 0);
         ok = writer.Write(out, &a) && ok;
     }
+    {   // 3-D, a cell-centred and a y-staggered vector: two grids, "stag" and "stag_Y"
+        using G = dare::Grid::Cartesian<3>;
+        G grid({5, 6, 7}, {1., 1.2, 1.4}, 2);
+        grid.SetName("stag");
+        auto rep = grid.GetRepresentation();
+        auto rep_y = grid.GetRepresentation({0, 1, 0});
+        HostVector<G, 1> a("p", rep);
+        HostVector<G, 1> b("v", rep_y);
+        fill<3, 1>(a);
+        fill<3, 1>(b);
+        dare::io::VTKWriter<G> writer(0.5, 2);
+        ok = writer.Write(out, &a, &b) && ok;
+    }
+    {   // 1-D staggered, no time stamp
+        using G = dare::Grid::Cartesian<1>;
+        G grid({6}, {1.2}, 1);
+        grid.SetName("line");
+        auto rep = grid.GetRepresentation({1});
+        HostVector<G, 1> a("u", rep);
+        fill<1, 1>(a);
+        dare::io::VTKWriter<G> writer(-1., 1);
+        ok = writer.Write(out, &a) && ok;
+    }
     std::printf("%s\n", ok ? "written" : "write failed");
     return ok ? 0 : 1;
 }

@@ -5,7 +5,10 @@ their local index; this test reads the files back (XML header + raw appended blo
 checks what IO/VTKWriter.inl:38-95 and Grid/Cartesian/VTKOptions_Cartesian.h:144-204,264-322
 fix: file layout and names (VTKWriter.cpp:28-46), extents, ghost level, points at the cell
 vertices offset by -g*dx in VTK's x-fastest order, one cell tuple per local cell (ghosts
-included) in that order, array and component names, and the TimeValue field.
+included) in that order, array and component names, and the TimeValue field. A staggered
+representation (CartesianRepresentation.inl:50-77) is written as its own grid "<grid>_Y":
+one more internal cell along the staggered axis and points shifted by -dx/2 there, with dx
+the grid's cell width.
 """
 import os
 import re
@@ -52,12 +55,16 @@ def out(tmp_path_factory):
     return d
 
 
-@pytest.mark.parametrize("name,step,res,size,g,dim,time", [
-    ("Cartesian3D", 7, [5, 6, 7], [1.0, 1.2, 1.4], 2, 3, 0.25),
-    ("plane", 3, [4, 3], [2.0, 1.5], 1, 2, None),
-    ("Cartesian1D", 0, [9], [0.9], 2, 1, 1.5),
+@pytest.mark.parametrize("name,step,grid_res,size,g,dim,time,stag", [
+    ("Cartesian3D", 7, [5, 6, 7], [1.0, 1.2, 1.4], 2, 3, 0.25, [0, 0, 0]),
+    ("plane", 3, [4, 3], [2.0, 1.5], 1, 2, None, [0, 0]),
+    ("Cartesian1D", 0, [9], [0.9], 2, 1, 1.5, [0]),
+    ("stag", 2, [5, 6, 7], [1.0, 1.2, 1.4], 2, 3, 0.5, [0, 0, 0]),
+    ("stag_Y", 2, [5, 6, 7], [1.0, 1.2, 1.4], 2, 3, 0.5, [0, 1, 0]),
+    ("line_X", 1, [6], [1.2], 1, 1, None, [1]),
 ])
-def test_vtk_files(out, name, step, res, size, g, dim, time):
+def test_vtk_files(out, name, step, grid_res, size, g, dim, time, stag):
+    res = [n + s for n, s in zip(grid_res, stag)]      # internal cells of the representation
     sub = f"{name}_{step}"
     pvts = os.path.join(out, sub + ".pvts")
     vts = os.path.join(out, sub, sub + "_0.vts")
@@ -82,12 +89,12 @@ def test_vtk_files(out, name, step, res, size, g, dim, time):
     dx = [0.0, 0.0, 0.0]
     for d in range(dim):
         npt[d] = loc[d] + 1
-        dx[d] = size[d] / res[d]
+        dx[d] = size[d] / grid_res[d]
     if dim == 1:
         npt[1], dx[1] = 2, dx[0]
     pts = arrays["Points"][1].reshape(npt[2], npt[1], npt[0], 3)
     kk, jj, ii = np.meshgrid(np.arange(npt[2]), np.arange(npt[1]), np.arange(npt[0]), indexing="ij")
-    off = [-g * dx[d] if d < dim else 0.0 for d in range(3)]
+    off = [(-0.5 * dx[d] if stag[d] else 0.0) - g * dx[d] if d < dim else 0.0 for d in range(3)]
     np.testing.assert_array_equal(pts[..., 0], dx[0] * ii + off[0])
     np.testing.assert_array_equal(pts[..., 1], dx[1] * jj + off[1])
     np.testing.assert_array_equal(pts[..., 2], dx[2] * kk + off[2])
@@ -119,3 +126,12 @@ def test_vtk_names(out):
     assert "ComponentName0" not in arrays["phi"][0]
     head2, arrays2, _, _ = read_vts(os.path.join(out, "plane_3", "plane_3_0.vts"))
     assert (arrays2["c"][0]["ComponentName0"], arrays2["c"][0]["ComponentName1"]) == ("0", "1")
+
+
+def test_vtk_staggered_grouping(out):
+    """A cell-centred and a staggered vector of one grid go to two grids named after their
+    representations (VTKWriter.inl:124-137 groups by GetName())."""
+    _, a, _, _ = read_vts(os.path.join(out, "stag_2", "stag_2_0.vts"))
+    _, b, _, _ = read_vts(os.path.join(out, "stag_Y_2", "stag_Y_2_0.vts"))
+    assert set(a) == {"TimeValue", "p", "Points"}
+    assert set(b) == {"TimeValue", "v", "Points"}
