@@ -98,10 +98,14 @@ __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict
     // stage are issued before it is written, in batches of eight per thread.
     // K = 32 is two taps x 16 channels (the 28th tap is zero).
     constexpr int SX = 16 * NSEG + 2, SY = 6, NV = SX * SY * 3;
+    // the two channel halves are NVS pieces apart, a multiple of 256 bytes: lanes 0-15 and
+    // 16-31 (32-47 and 48-63) of one operand read then start on the same LDS bank, which keeps
+    // the 16-byte reads free of bank conflicts (a 396- or 1188-piece stride costs about 40 %)
+    constexpr int NVS = (NV + 15) / 16 * 16;
     constexpr int NBB = conv_nbb<COUT>(), NGRP = COUT / (16 * NBB);
     constexpr int NWP = 2 * 27 * 16 * NBB;                     // weight pieces per slice
     constexpr int NPIECE = 2 * NV + NWP;
-    __shared__ uint4 stage[2 * NV];
+    __shared__ uint4 stage[2 * NVS];
     __shared__ uint4 wl[NWP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = blockIdx.y % NGRP;
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict
 #pragma unroll
             for (int r = 0; r < 8; r++) {
                 const int t = b0 + r * kNetThreads + threadIdx.x;
-                if (t < 2 * NV) stage[(t & 1) * NV + (t >> 1)] = u[r];
+                if (t < 2 * NV) stage[(t & 1) * NVS + (t >> 1)] = u[r];
                 else if (t < NPIECE) wl[t - 2 * NV] = u[r];
             }
         }
@@ -156,7 +160,7 @@ __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict
                 for (int nb = 0; nb < NBB; nb++)
                     w[nb] = tap_ok ? __builtin_bit_cast(bf16x8, wl[(half * 27 + tap) * 16 * NBB + nb * 16 + i])
                                    : bf16x8{};
-                const int base = half * NV + ((dz + 1) * SY + (wave + dy + 1)) * SX + (i + dx + 1);
+                const int base = half * NVS + ((dz + 1) * SY + (wave + dy + 1)) * SX + (i + dx + 1);
 #pragma unroll
                 for (int s = 0; s < NSEG; s++)
                     x[s] = tap_ok ? __builtin_bit_cast(bf16x8, stage[base + 16 * s]) : bf16x8{};
@@ -240,13 +244,17 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
                                                         uint16_t* __restrict__ Y2) {
     static_assert(CIN == 16 || CIN == 32, "ring of full-channel planes: 16 or 32 input channels");
     constexpr int SX = 16 * NSEG + 2, SY = 6, NPL = SX * SY;    // voxels of one staged plane
+    constexpr int NPS = (NPL + 15) / 16 * 16;                    // LDS stride of one 8-channel piece
     constexpr int CH8 = CIN / 8;                                 // 16-byte pieces per voxel
     constexpr int NGRP = COUT / 16;                              // 16 outputs per block
     constexpr int NWP = CH8 * 27 * 16;                           // weight pieces
     constexpr int PPL = CH8 * NPL;                               // pieces of one plane
-    constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;   // per thread
+    constexpr int SLOT = CH8 * NPS;                              // LDS pieces of one ring slot
+    constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;   // staged pieces per thread
     constexpr int NSTEP = (CIN / 16) * 14;                       // K = 32 steps
-    __shared__ uint4 ring[3 * PPL];                              // [slot][piece8][voxel]
+    // piece strides of 256 bytes multiples (NPS): the lane groups of one operand read that
+    // differ only in the channel piece start on the same bank, free of bank conflicts
+    __shared__ uint4 ring[3 * SLOT];                             // [slot][piece8][voxel]
     __shared__ uint4 wl[NWP];                                    // [piece8][tap][co]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = blockIdx.y % NGRP;
@@ -255,14 +263,15 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
     const int z1 = (z0 + ZB < D) ? z0 + ZB : D;
     const int y = y0 + wave;
     const int i = lane & 15, kq = lane >> 4, half = kq & 1;
-    // one staged plane: piece t = (voxel, p8), p8 fastest
+    // one staged plane: piece t = (voxel, p8), p8 fastest (adjacent lanes load adjacent
+    // 16-byte pieces of global memory)
     auto load_plane = [&](int gz, uint4* u) {
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
             const int t = r * kNetThreads + threadIdx.x;
+            const int v = t / CH8, p8 = t % CH8;
             u[r] = uint4{0u, 0u, 0u, 0u};
             if (t < PPL && gz >= 0 && gz < D) {
-                const int v = t / CH8, p8 = t % CH8;
                 const int gx = x0 - 1 + v % SX, gy = y0 - 1 + v / SX;
                 if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
                     const long long src = (MODE & kConvUp)
@@ -277,7 +286,7 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
             const int t = r * kNetThreads + threadIdx.x;
-            if (t < PPL) ring[slot * PPL + (t % CH8) * NPL + t / CH8] = u[r];
+            if (t < PPL) ring[slot * SLOT + (t % CH8) * NPS + t / CH8] = u[r];
         }
     };
     // weights and the first three planes (z0-1, z0, z0+1 -> slots by plane mod 3)
@@ -311,7 +320,7 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
                 const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
                 w = tap_ok ? __builtin_bit_cast(bf16x8, wl[(p8 * 27 + tap) * 16 + i]) : bf16x8{};
                 const int slot = (z + dz + 3) % 3;
-                const int base = slot * PPL + p8 * NPL + (wave + dy + 1) * SX + (i + dx + 1);
+                const int base = slot * SLOT + p8 * NPS + (wave + dy + 1) * SX + (i + dx + 1);
 #pragma unroll
                 for (int s = 0; s < NSEG; s++)
                     x[s] = tap_ok ? __builtin_bit_cast(bf16x8, ring[base + 16 * s]) : bf16x8{};
