@@ -231,8 +231,11 @@ __global__ __launch_bounds__(kNetThreads) void k_conv(const uint16_t* __restrict
 
 // z-marching form for 16- and 32-channel inputs: a block owns a 16 NSEG (x) x 4 (y) tile, 16
 // output channels (group grp), and the planes z0 .. z0+ZB-1. Its weights (all taps and input
-// channels of its 16 outputs) stay in LDS for the whole march, and the input planes live in a
-// ring of three LDS slots: plane z+2 is loaded into registers while plane z is computed and
+// channels of its 16 outputs) stay resident for the whole march: with 32 input channels in
+// registers (each lane holds the 28 A-operand fragments it feeds, so the LDS serves only the
+// input operands; 224 VGPRs, the occupancy the LDS allowed anyway), with 16 in LDS (in
+// registers they would cost a wave per SIMD, which loses more than the LDS reads they save).
+// The input planes live in a ring of three LDS slots: plane z+2 is loaded into registers while plane z is computed and
 // written into the slot of plane z-1 afterwards, so each input plane is staged once per block
 // instead of three times and its loads overlap the MFMAs. MODE as k_conv (upsampled input,
 // skip-sum epilogue). Same operand layout, K order and accumulation order as k_conv
@@ -247,7 +250,8 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
     constexpr int NPS = (NPL + 15) / 16 * 16;                    // LDS stride of one 8-channel piece
     constexpr int CH8 = CIN / 8;                                 // 16-byte pieces per voxel
     constexpr int NGRP = COUT / 16;                              // 16 outputs per block
-    constexpr int NWP = CH8 * 27 * 16;                           // weight pieces
+    constexpr bool WREG = CIN == 32;                             // weights in registers
+    constexpr int NWP = WREG ? 1 : CH8 * 27 * 16;                // LDS weight pieces
     constexpr int PPL = CH8 * NPL;                               // pieces of one plane
     constexpr int SLOT = CH8 * NPS;                              // LDS pieces of one ring slot
     constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;   // staged pieces per thread
@@ -255,7 +259,7 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
     // piece strides of 256 bytes multiples (NPS): the lane groups of one operand read that
     // differ only in the channel piece start on the same bank, free of bank conflicts
     __shared__ uint4 ring[3 * SLOT];                             // [slot][piece8][voxel]
-    __shared__ uint4 wl[NWP];                                    // [piece8][tap][co]
+    __shared__ uint4 wl[NWP];                                    // [piece8][tap][co] (CIN 16)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = blockIdx.y % NGRP;
     const int x0 = blockIdx.x * (16 * NSEG), y0 = (blockIdx.y / NGRP) * 4;
@@ -289,11 +293,25 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
             if (t < PPL) ring[slot * SLOT + (t % CH8) * NPS + t / CH8] = u[r];
         }
     };
-    // weights and the first three planes (z0-1, z0, z0+1 -> slots by plane mod 3)
-    for (int t = threadIdx.x; t < NWP; t += kNetThreads) {
-        const int co = t % 16, tap = (t / 16) % 27, p8 = t / (16 * 27);
-        wl[t] = *reinterpret_cast<const uint4*>(Wt + (static_cast<long long>(tap) * COUT + grp * 16 + co) * CIN + 8 * p8);
+    // this lane's weight fragments: step q = 16-channel slice q / 14, taps 2 (q % 14) and
+    // 2 (q % 14) + 1; lane (i, kq) holds output channel grp 16 + i, tap 2 (q % 14) + kq / 2,
+    // channels 8 (2 (q / 14) + kq % 2) .. +7 (zero for the 28th tap)
+    bf16x8 wr[WREG ? NSTEP : 1];
+    if constexpr (WREG) {
+#pragma unroll
+        for (int q = 0; q < NSTEP; q++) {
+            const int p8 = 2 * (q / 14) + half;
+            const int tap = 2 * (q % 14) + (kq >> 1);
+            wr[q] = load8(Wt + (static_cast<long long>(tap) * COUT + grp * 16 + i) * CIN + 8 * p8, tap < 27);
+        }
+    } else {
+        for (int t = threadIdx.x; t < NWP; t += kNetThreads) {
+            const int co = t % 16, tap = (t / 16) % 27, p8 = t / (16 * 27);
+            wl[t] = *reinterpret_cast<const uint4*>(Wt + (static_cast<long long>(tap) * COUT + grp * 16 + co) * CIN +
+                                                    8 * p8);
+        }
     }
+    // the first three planes (z0-1, z0, z0+1 -> slots by plane mod 3)
     {
         uint4 u[RPL];
         for (int q = -1; q <= 1; q++) {
@@ -312,13 +330,14 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
             // operands of step q + 1 are read from LDS before the MFMAs of step q (the LDS
             // latency overlaps the matrix work of the same wave); step q = 16-channel slice
             // q / 14, taps 2 (q % 14) and 2 (q % 14) + 1
-            bf16x8 wa, xb[NSEG];
+            bf16x8 wa{}, xb[NSEG];
             auto fetch = [&](int q, bf16x8& w, bf16x8* x) {
                 const int p8 = 2 * (q / 14) + half;
                 const int tap = 2 * (q % 14) + (kq >> 1);
                 const bool tap_ok = tap < 27;
                 const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-                w = tap_ok ? __builtin_bit_cast(bf16x8, wl[(p8 * 27 + tap) * 16 + i]) : bf16x8{};
+                if constexpr (!WREG)
+                    w = tap_ok ? __builtin_bit_cast(bf16x8, wl[(p8 * 27 + tap) * 16 + i]) : bf16x8{};
                 const int slot = (z + dz + 3) % 3;
                 const int base = slot * SLOT + p8 * NPS + (wave + dy + 1) * SX + (i + dx + 1);
 #pragma unroll
@@ -328,11 +347,12 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
             fetch(0, wa, xb);
 #pragma unroll
             for (int q = 0; q < NSTEP; q++) {
-                bf16x8 wn, xn[NSEG];
+                bf16x8 wn{}, xn[NSEG];
                 if (q + 1 < NSTEP) fetch(q + 1, wn, xn);
                 __builtin_amdgcn_sched_barrier(0);   // keep the reads of q + 1 ahead of these MFMAs
+                const bf16x8 a = WREG ? wr[WREG ? q : 0] : wa;
 #pragma unroll
-                for (int s = 0; s < NSEG; s++) acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb[s], acc[s], 0, 0, 0);
+                for (int s = 0; s < NSEG; s++) acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[s], acc[s], 0, 0, 0);
                 if (q + 1 < NSTEP) {
                     wa = wn;
 #pragma unroll
