@@ -402,6 +402,175 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
     }
 }
 
+// Nearest 2x upsampling followed by the 3x3x3 conv, in sub-pixel form: a fine voxel
+// f = 2p + r (parity r per axis) reads through tap d in {-1, 0, 1} the coarse voxel
+// p + floor((r + d) / 2), so for each of the 8 parity classes the 27 taps fold into a 2x2x2
+// conv on the coarse tensor -- coarse tap o in {0, 1}^3 at offset o - 1 + r, with weight
+// Ws[r][o] = the sum of the taps d in S(r, o) per axis, S(0,0) = {-1}, S(0,1) = {0, 1},
+// S(1,0) = {-1, 0}, S(1,1) = {1}. 8 taps instead of 27 per output, the same sums up to the
+// rounding of the folded weights (summed in FP32 from the BF16 taps, rounded once).
+// Ws layout: [r = 4 rz + 2 ry + rx][o = 4 oz + 2 oy + ox][co][ci], BF16.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(kNetThreads) void k_sub_weights(const uint16_t* __restrict__ Wt,
+                                                            uint16_t* __restrict__ Ws) {
+    const int n = 64 * COUT * CIN;
+    for (int idx = blockIdx.x * kNetThreads + threadIdx.x; idx < n; idx += gridDim.x * kNetThreads) {
+        const int ci = idx % CIN, co = (idx / CIN) % COUT, o = (idx / (CIN * COUT)) % 8, r = idx / (8 * CIN * COUT);
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int ra = (r >> (2 - a)) & 1, oa = (o >> (2 - a)) & 1;
+            lo[a] = oa == 0 ? -1 : (ra == 0 ? 0 : 1);
+            hi[a] = oa == 0 ? (ra == 0 ? -1 : 0) : 1;
+        }
+        float acc = 0.f;
+        for (int dz = lo[0]; dz <= hi[0]; dz++)
+            for (int dy = lo[1]; dy <= hi[1]; dy++)
+                for (int dx = lo[2]; dx <= hi[2]; dx++) {
+                    const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                    acc += bf2f(Wt[(tap * COUT + co) * CIN + ci]);
+                }
+        Ws[idx] = f2bf(acc);
+    }
+}
+
+// The sub-pixel conv: Y (fine D x H x W, COUT channels) from X (coarse, CIN channels). A block
+// owns one fine plane z, four fine rows y0..y0+3 (one per wave: its z and y parities are
+// fixed), 32 NSEG fine voxels along x (16 NSEG coarse columns, both x parities) and 16 output
+// channels. It stages the coarse box it reads -- 2 planes x 4 rows x (16 NSEG + 2) columns,
+// zero outside the grid -- in LDS as [8-channel piece][voxel] with a 256-byte multiple piece
+// stride. A K = 32 step is one coarse tap (oz, oy) x 32 channels; its B fragments at coarse x
+// offsets -1, 0, +1 serve the four (rx, ox) products, and each lane keeps all its weight
+// fragments (2 x-parities x 8 taps x CIN/32 slices) in registers. ADD: the epilogue also
+// writes Y2 = bf16(Y + E), as kConvAdd.
+template <int CIN, int COUT, bool RELU, int NSEG, bool ADD>
+__global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Ws,
+                                                         uint16_t* __restrict__ Y, int D, int H, int W,
+                                                         const uint16_t* __restrict__ E, uint16_t* __restrict__ Y2) {
+    constexpr int CX = 16 * NSEG + 2, NPL = CX * 4 * 2;          // staged coarse voxels
+    constexpr int NPS = (NPL + 15) / 16 * 16;                    // LDS stride of one 8-channel piece
+    constexpr int CH8 = CIN / 8, KS = CIN / 32;
+    constexpr int PPL = CH8 * NPS;                               // staged pieces (padding included)
+    constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;
+    constexpr int NGRP = COUT / 16;
+    __shared__ uint4 st[PPL];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = blockIdx.y % NGRP;
+    const int y0 = (blockIdx.y / NGRP) * 4, z = blockIdx.z;
+    const int Dc = D >> 1, Hc = H >> 1, Wc = W >> 1;
+    const int px0 = blockIdx.x * (16 * NSEG);
+    const int y = y0 + wave, ry = wave & 1, rz = z & 1;
+    const int i = lane & 15, kq = lane >> 4;
+    const int cz0 = (z >> 1) - 1 + rz, cy0 = (y0 >> 1) - 1;
+    // weight fragments of this lane (issued first; they come from L2)
+    bf16x8 wr[2][8][KS];
+#pragma unroll
+    for (int rx = 0; rx < 2; rx++)
+#pragma unroll
+        for (int o = 0; o < 8; o++)
+#pragma unroll
+            for (int c = 0; c < KS; c++)
+                wr[rx][o][c] = load8(Ws + ((((rz * 4 + ry * 2 + rx) * 8 + o) * COUT + grp * 16 + i) * CIN + 32 * c + 8 * kq),
+                                     true);
+    // stage: 16 consecutive threads move 16 consecutive voxels of one 8-channel piece
+    {
+        uint4 u[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int t = r * kNetThreads + threadIdx.x;
+            const int g = t >> 4, p8 = g % CH8, v = (g / CH8) * 16 + (t & 15);
+            u[r] = uint4{0u, 0u, 0u, 0u};
+            if (v < NPL) {
+                const int gx = px0 - 1 + v % CX, gy = cy0 + (v / CX) % 4, gz = cz0 + v / (CX * 4);
+                if (gx >= 0 && gx < Wc && gy >= 0 && gy < Hc && gz >= 0 && gz < Dc)
+                    u[r] = *reinterpret_cast<const uint4*>(
+                        X + ((static_cast<long long>(gz) * Hc + gy) * Wc + gx) * CIN + 8 * p8);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int t = r * kNetThreads + threadIdx.x;
+            const int g = t >> 4, p8 = g % CH8, v = (g / CH8) * 16 + (t & 15);
+            if (t < PPL) st[p8 * NPS + v] = u[r];
+        }
+    }
+    __syncthreads();
+    if (y >= H) return;                                         // wave-uniform; no barrier follows
+    floatx4 acc[2][NSEG];
+#pragma unroll
+    for (int rx = 0; rx < 2; rx++)
+#pragma unroll
+        for (int s = 0; s < NSEG; s++) acc[rx][s] = floatx4{0.f, 0.f, 0.f, 0.f};
+    constexpr int NST = 4 * KS;                                 // (oz, oy, c) steps
+    bf16x8 bm[NSEG], b0[NSEG], bp[NSEG];
+    auto fetch = [&](int q, bf16x8* m, bf16x8* c0, bf16x8* p) {
+        const int oz = q / (2 * KS), oy = (q / KS) % 2, c = q % KS;
+        const int base = (4 * c + kq) * NPS + (oz * 4 + (wave >> 1) + oy + ry) * CX + i + 1;
+#pragma unroll
+        for (int s = 0; s < NSEG; s++) {
+            m[s] = __builtin_bit_cast(bf16x8, st[base + 16 * s - 1]);
+            c0[s] = __builtin_bit_cast(bf16x8, st[base + 16 * s]);
+            p[s] = __builtin_bit_cast(bf16x8, st[base + 16 * s + 1]);
+        }
+    };
+    fetch(0, bm, b0, bp);
+#pragma unroll
+    for (int q = 0; q < NST; q++) {
+        bf16x8 nm[NSEG], n0[NSEG], np[NSEG];
+        if (q + 1 < NST) fetch(q + 1, nm, n0, np);
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads of q + 1 ahead of these MFMAs
+        const int oz = q / (2 * KS), oy = (q / KS) % 2, c = q % KS;
+        const int o0 = oz * 4 + oy * 2;                          // ox = 0; ox = 1 is o0 + 1
+#pragma unroll
+        for (int s = 0; s < NSEG; s++) {
+            acc[0][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[0][o0][c], bm[s], acc[0][s], 0, 0, 0);
+            acc[1][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[1][o0][c], b0[s], acc[1][s], 0, 0, 0);
+            acc[0][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[0][o0 + 1][c], b0[s], acc[0][s], 0, 0, 0);
+            acc[1][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[1][o0 + 1][c], bp[s], acc[1][s], 0, 0, 0);
+        }
+        if (q + 1 < NST) {
+#pragma unroll
+            for (int s = 0; s < NSEG; s++) {
+                bm[s] = nm[s];
+                b0[s] = n0[s];
+                bp[s] = np[s];
+            }
+        }
+    }
+    const long long vrow = (static_cast<long long>(z) * H + y) * W;
+#pragma unroll
+    for (int s = 0; s < NSEG; s++)
+#pragma unroll
+        for (int rx = 0; rx < 2; rx++) {
+            const int x = 2 * (px0 + 16 * s + i) + rx;
+            if (x >= W) continue;
+            uint16_t b[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float o = acc[rx][s][j];
+                if (RELU) o = o > 0.f ? o : 0.f;
+                b[j] = f2bf(o);
+            }
+            const long long at = (vrow + x) * COUT + grp * 16 + kq * 4;
+            uint2 pk;
+            pk.x = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 16);
+            pk.y = static_cast<uint32_t>(b[2]) | (static_cast<uint32_t>(b[3]) << 16);
+            *reinterpret_cast<uint2*>(Y + at) = pk;
+            if constexpr (ADD) {
+                const uint2 ev = *reinterpret_cast<const uint2*>(E + at);
+                const uint16_t e[4] = {static_cast<uint16_t>(ev.x & 0xffffu), static_cast<uint16_t>(ev.x >> 16),
+                                       static_cast<uint16_t>(ev.y & 0xffffu), static_cast<uint16_t>(ev.y >> 16)};
+                uint16_t t[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) t[j] = f2bf(bf2f(b[j]) + bf2f(e[j]));
+                uint2 sk;
+                sk.x = static_cast<uint32_t>(t[0]) | (static_cast<uint32_t>(t[1]) << 16);
+                sk.y = static_cast<uint32_t>(t[2]) | (static_cast<uint32_t>(t[3]) << 16);
+                *reinterpret_cast<uint2*>(Y2 + at) = sk;
+            }
+        }
+}
+
 // Weight gradient partials: P[c][tap][co][ci] = sum over the voxels of chunk c of
 // G[v][co] * X[v + off(tap)][ci]. Wave = one chunk of (z, y) rows for one 16x16 (co, ci)
 // block; A[co][voxel] = G, B[voxel][ci] = shifted X; 27 accumulators, one per tap.
@@ -809,6 +978,7 @@ struct dare_convnet {
     long long nparams = 0;
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     uint16_t *wf = nullptr, *wb = nullptr;
+    uint16_t *ws6 = nullptr, *ws8 = nullptr;   // folded sub-pixel weights of layers 6 and 8
     int adam_t = 0;
     // forward activations
     uint16_t *x0 = nullptr, *a1 = nullptr, *e1 = nullptr, *p1 = nullptr, *a3 = nullptr, *e2 = nullptr, *p2 = nullptr,
@@ -913,6 +1083,29 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
         hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1, MODE>), dim3((W + 15) / 16, gy, D), blk, 0, sm, X, Wt, Y,
                            Yf, D, H, W, E, Y2);
     }
+}
+
+// decoder layer: ReLU(conv(upsample(X))) and the skip sum Y2 = Y + E, in sub-pixel form (the
+// 27 taps folded into 8 per output parity, Ws refreshed from the current BF16 weights first)
+template <int CIN, int COUT>
+void conv_up(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Ws, uint16_t* Y, const int* d, int level,
+             const uint16_t* E, uint16_t* Y2) {
+    static_assert(CIN % 32 == 0 && COUT % 16 == 0, "sub-pixel conv: 32-channel K slices, 16-channel groups");
+    const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
+    const dim3 blk(kNetThreads);
+    hipLaunchKernelGGL((k_sub_weights<CIN, COUT>), dim3((64 * CIN * COUT + kNetThreads - 1) / kNetThreads), blk, 0, sm,
+                       Wt, Ws);
+    const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
+    constexpr int NMAX = CIN == 32 ? 4 : 2;                     // registers: 2 x 8 x CIN/32 weight fragments
+    if (NMAX == 4 && W >= 128)
+        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, NMAX, true>), dim3((W + 127) / 128, gy, D), blk, 0, sm, X, Ws,
+                           Y, D, H, W, E, Y2);
+    else if (W >= 64)
+        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 2, true>), dim3((W + 63) / 64, gy, D), blk, 0, sm, X, Ws, Y, D,
+                           H, W, E, Y2);
+    else
+        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 1, true>), dim3((W + 31) / 32, gy, D), blk, 0, sm, X, Ws, Y, D,
+                           H, W, E, Y2);
 }
 
 // dispatch on (cin, cout) of the network's layers
@@ -1024,6 +1217,8 @@ int dare_convnet_create(dare_ctx* ctx, int32_t in_channels, const int32_t* max_d
     A(&net->v, net->nparams);
     A(&net->wf, net->nparams);
     A(&net->wb, net->nparams);
+    A(&net->ws6, 64LL * kDefs[6].cin * kDefs[6].cout);
+    A(&net->ws8, 64LL * kDefs[8].cin * kDefs[8].cout);
     A(&net->x0, V0 * 16);
     A(&net->a1, V0 * 16);
     A(&net->e1, V0 * 16);
@@ -1073,7 +1268,7 @@ int dare_convnet_destroy(dare_convnet* net) {
     if (!net) return DARE_OK;
     (void)hipSetDevice(net->device);
     (void)hipDeviceSynchronize();
-    void* ptrs[] = {net->w, net->g, net->m, net->v, net->wf, net->wb, net->x0, net->a1, net->e1, net->p1, net->a3,
+    void* ptrs[] = {net->w, net->g, net->m, net->v, net->wf, net->wb, net->ws6, net->ws8, net->x0, net->a1, net->e1, net->p1, net->a3,
                     net->e2, net->p2, net->a5, net->e3, net->u2, net->a7, net->s2, net->d2, net->u1, net->a9,
                     net->s1, net->d1, net->out, net->dout, net->t0a, net->t0b, net->t0c, net->t0w, net->h1a,
                     net->h1b, net->h1c, net->h1w, net->q2a, net->q2b, net->wpart, net->scales, net->part, net->loss};
@@ -1174,9 +1369,9 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
     conv<64, 64, true, 0>(sm, net->a5, W + o[5], net->e3, nullptr, d, 2);
     // decoder: the upsampling is read inside the conv (u2, u1 are written by the backward pass,
     // which needs them) and the skip sums s2 = a7 + e2, s1 = a9 + e1 come out of its epilogue
-    conv<64, 32, true, 0, kConvUp | kConvAdd>(sm, net->e3, W + o[6], net->a7, nullptr, d, 1, net->e2, net->s2);
+    conv_up<64, 32>(sm, net->e3, W + o[6], net->ws6, net->a7, d, 1, net->e2, net->s2);
     conv<32, 32, true, 0>(sm, net->s2, W + o[7], net->d2, nullptr, d, 1);
-    conv<32, 16, true, 0, kConvUp | kConvAdd>(sm, net->d2, W + o[8], net->a9, nullptr, d, 0, net->e1, net->s1);
+    conv_up<32, 16>(sm, net->d2, W + o[8], net->ws8, net->a9, d, 0, net->e1, net->s1);
     conv<16, 16, true, 0>(sm, net->s1, W + o[9], net->d1, nullptr, d, 0);
     conv<16, 16, false, 1>(sm, net->d1, W + o[10], nullptr, net->out, d, 0);
     DARE_LAUNCH_CHECK();

@@ -3,8 +3,9 @@
 A numpy restatement of the 3-level encoder-decoder built in dare_amd/csrc/convnet.hip: same
 layers, same tap order, same pooling/upsampling/skip structure, forward and backward. With
 `exact=False` every tensor the GPU stores in BF16 is rounded to BF16 at the same point
-(round to nearest even), so GPU and oracle differ only by the FP32 summation order of the
-matrix-core accumulations; with `exact=True` nothing is rounded (float64 throughout), which is
+(round to nearest even) -- including the folded weights of the two decoder convs that read an
+upsampled tensor, which both compute in sub-pixel form -- so GPU and oracle differ only by the
+FP32 summation order of the matrix-core accumulations; with `exact=True` nothing is rounded (float64 throughout), which is
 the mode the finite-difference check of the backward pass uses.
 
 The network has no counterpart in the reference; what pins it is the mathematics: the
@@ -106,6 +107,40 @@ def relu(x):
     return np.maximum(x, 0.0)
 
 
+def fold(w, rnd=lambda a: a):
+    """Sub-pixel weights of a conv that reads the 2x nearest upsampling: ws[r][o] ([8, 8, cout,
+    cin], r = 4 rz + 2 ry + rx output parity, o = 4 oz + 2 oy + ox coarse tap) is the sum of the
+    taps d whose coarse offset floor((r + d) / 2) equals o - 1 + r on every axis; `rnd` rounds
+    the sums (the GPU stores them in BF16)."""
+    ws = np.zeros((8, 8) + w.shape[1:])
+    for r in range(8):
+        rr = ((r >> 2) & 1, (r >> 1) & 1, r & 1)
+        for o in range(8):
+            oo = ((o >> 2) & 1, (o >> 1) & 1, o & 1)
+            for t, dz, dy, dx in _taps():
+                if all((ra + d) // 2 == oa - 1 + ra for ra, oa, d in zip(rr, oo, (dz, dy, dx))):
+                    ws[r, o] += w[t]
+    return rnd(ws)
+
+
+def conv_up(x, w, rnd=lambda a: a):
+    """conv(up(x), w) in sub-pixel form: y[2p + r] = sum_o x[p + o - 1 + r] ws[r][o]^T over
+    the 8 coarse taps of each output parity r (zero padding). Equal to conv(up(x), w) up to the
+    rounding `rnd` applies to the folded weights."""
+    D, H, W, _ = x.shape
+    ws = fold(w, rnd)
+    xp = np.pad(x, ((1, 1), (1, 1), (1, 1), (0, 0)))
+    y = np.zeros((2 * D, 2 * H, 2 * W, w.shape[1]))
+    for r in range(8):
+        rz, ry, rx = (r >> 2) & 1, (r >> 1) & 1, r & 1
+        acc = np.zeros((D, H, W, w.shape[1]))
+        for o in range(8):
+            oz, oy, ox = (o >> 2) & 1, (o >> 1) & 1, o & 1
+            acc += _shift(xp, oz - 1 + rz, oy - 1 + ry, ox - 1 + rx, (D, H, W)) @ ws[r, o].T
+        y[rz::2, ry::2, rx::2] = acc
+    return y
+
+
 def forward(x0, weights, exact=False):
     """x0: [D, H, W, 16]. Returns (out [D, H, W], activations dict)."""
     r = (lambda a: a) if exact else bf16
@@ -119,12 +154,15 @@ def forward(x0, weights, exact=False):
     A["p2"] = r(pool(A["e2"]))
     A["a5"] = r(relu(conv(A["p2"], W[4])))
     A["e3"] = r(relu(conv(A["a5"], W[5])))
+    # decoder convs over the upsampled tensors: the GPU computes them in sub-pixel form with
+    # BF16-folded weights; exactly (float64) that form equals conv(up(x), w)
+    cu = (lambda x, w: conv(up(x), w)) if exact else (lambda x, w: conv_up(x, w, bf16))
     A["u2"] = up(A["e3"])
-    A["a7"] = r(relu(conv(A["u2"], W[6])))
+    A["a7"] = r(relu(cu(A["e3"], W[6])))
     A["s2"] = r(A["a7"] + A["e2"])
     A["d2"] = r(relu(conv(A["s2"], W[7])))
     A["u1"] = up(A["d2"])
-    A["a9"] = r(relu(conv(A["u1"], W[8])))
+    A["a9"] = r(relu(cu(A["d2"], W[8])))
     A["s1"] = r(A["a9"] + A["e1"])
     A["d1"] = r(relu(conv(A["s1"], W[9])))
     out = conv(A["d1"], W[10])[..., 0]

@@ -40,6 +40,30 @@ def test_conv_adjoint_and_resampling_identities():
     assert np.isclose(np.sum(cn.up(b) * a), np.sum(b * cn.up_bwd(a)))
 
 
+def test_subpixel_conv_equals_conv_of_upsampling(rng):
+    """conv(up(x), w) = conv_up(x, w) in float64 (the sub-pixel form the GPU computes for the
+    decoder convs): brute-force conv of the explicitly upsampled tensor on a ragged box, both
+    channel shapes of the network's decoder layers. The fold puts every tap in exactly one
+    coarse tap per parity, so the folded weights of each parity sum to the sum of all taps."""
+    for cin, cout, shape in ((32, 16, (3, 2, 4)), (64, 32, (2, 3, 2))):
+        x = rng.normal(size=shape + (cin,))
+        w = rng.normal(size=(27, cout, cin))
+        y0 = cn.conv(cn.up(x), w)
+        y1 = cn.conv_up(x, w)
+        assert np.max(np.abs(y1 - y0)) <= 1e-12 * np.max(np.abs(y0))
+        ws = cn.fold(w)
+        for r in range(8):
+            assert np.allclose(ws[r].sum(axis=0), w.sum(axis=0), rtol=0, atol=1e-12)
+    # one-hot input and weight: the single coarse voxel lands on exactly the fine voxels the
+    # tap reaches through the upsampling
+    x = np.zeros((2, 2, 2, 32))
+    x[1, 0, 1, 3] = 1.0
+    for t in (0, 4, 13, 22, 26):
+        w = np.zeros((27, 16, 32))
+        w[t, 5, 3] = 1.0
+        np.testing.assert_array_equal(cn.conv_up(x, w), cn.conv(cn.up(x), w))
+
+
 def test_oracle_gradients_match_finite_differences():
     rng = np.random.default_rng(2)
     W = _weights(rng)
