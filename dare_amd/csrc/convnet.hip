@@ -1047,10 +1047,10 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
     const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
     const dim3 blk(kNetThreads);
     const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / (16 * conv_nbb<COUT>()));
-    if constexpr (CIN == 16 || (CIN == 32 && COUT == 16)) {
+    if constexpr (CIN == 16 || CIN == 32) {
         // z-marching form: the planes per block are chosen so that the whole grid is resident at
         // once (no tail round): z-blocks = resident blocks / (x, y, channel-group) columns. 32-
-        // channel rings use tiles of at most 32 voxels along x (LDS).
+        // channel rings use tiles of at most 32 voxels along x (registers).
         const int nseg = (W >= 64 && CIN == 16) ? 4 : (W >= 32 ? 2 : 1);
         const unsigned gx = static_cast<unsigned>((W + 16 * nseg - 1) / (16 * nseg));
         const unsigned gzm = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
