@@ -462,7 +462,22 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
     const int y = y0 + wave, ry = wave & 1, rz = z & 1;
     const int i = lane & 15, kq = lane >> 4;
     const int cz0 = (z >> 1) - 1 + rz, cy0 = (y0 >> 1) - 1;
-    // weight fragments of this lane (issued first; they come from L2)
+    const long long vrow = (static_cast<long long>(z) * H + (y < H ? y : H - 1)) * W;
+    // skip operand of the epilogue, issued first so its latency hides behind the staging and
+    // the MFMAs (lanes past the row end read nothing)
+    uint2 ev[2][NSEG];
+    if constexpr (ADD) {
+#pragma unroll
+        for (int s = 0; s < NSEG; s++)
+#pragma unroll
+            for (int rx = 0; rx < 2; rx++) {
+                const int x = 2 * (px0 + 16 * s + i) + rx;
+                ev[rx][s] = (y < H && x < W)
+                    ? *reinterpret_cast<const uint2*>(E + (vrow + x) * COUT + grp * 16 + kq * 4)
+                    : uint2{0u, 0u};
+            }
+    }
+    // weight fragments of this lane (they come from L2)
     bf16x8 wr[2][8][KS];
 #pragma unroll
     for (int rx = 0; rx < 2; rx++)
@@ -537,7 +552,6 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
             }
         }
     }
-    const long long vrow = (static_cast<long long>(z) * H + y) * W;
 #pragma unroll
     for (int s = 0; s < NSEG; s++)
 #pragma unroll
@@ -557,9 +571,9 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
             pk.y = static_cast<uint32_t>(b[2]) | (static_cast<uint32_t>(b[3]) << 16);
             *reinterpret_cast<uint2*>(Y + at) = pk;
             if constexpr (ADD) {
-                const uint2 ev = *reinterpret_cast<const uint2*>(E + at);
-                const uint16_t e[4] = {static_cast<uint16_t>(ev.x & 0xffffu), static_cast<uint16_t>(ev.x >> 16),
-                                       static_cast<uint16_t>(ev.y & 0xffffu), static_cast<uint16_t>(ev.y >> 16)};
+                const uint2 e2 = ev[rx][s];
+                const uint16_t e[4] = {static_cast<uint16_t>(e2.x & 0xffffu), static_cast<uint16_t>(e2.x >> 16),
+                                       static_cast<uint16_t>(e2.y & 0xffffu), static_cast<uint16_t>(e2.y >> 16)};
                 uint16_t t[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) t[j] = f2bf(bf2f(b[j]) + bf2f(e[j]));
