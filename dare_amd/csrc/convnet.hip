@@ -435,49 +435,39 @@ __global__ __launch_bounds__(kNetThreads) void k_sub_weights(const uint16_t* __r
 }
 
 // The sub-pixel conv: Y (fine D x H x W, COUT channels) from X (coarse, CIN channels). A block
-// owns one fine plane z, four fine rows y0..y0+3 (one per wave: its z and y parities are
-// fixed), 32 NSEG fine voxels along x (16 NSEG coarse columns, both x parities) and 16 output
-// channels. It stages the coarse box it reads -- 2 planes x 4 rows x (16 NSEG + 2) columns,
-// zero outside the grid -- in LDS as [8-channel piece][voxel] with a 256-byte multiple piece
-// stride. A K = 32 step is one coarse tap (oz, oy) x 32 channels; its B fragments at coarse x
-// offsets -1, 0, +1 serve the four (rx, ox) products, and each lane keeps all its weight
-// fragments (2 x-parities x 8 taps x CIN/32 slices) in registers. ADD: the epilogue also
-// writes Y2 = bf16(Y + E), as kConvAdd.
+// owns one fine z parity rz, the fine planes z = 2k + rz for k in [k0, k0 + ZB), four fine rows
+// y0..y0+3 (one per wave, so its y parity is fixed), 32 NSEG fine voxels along x (16 NSEG
+// coarse columns, both x parities) and 16 output channels. Each lane loads its weight
+// fragments once (2 x-parities x 8 coarse taps x CIN/32 slices, kept in registers). Fine plane
+// 2k + rz reads coarse planes k - 1 + rz and k + rz, so the march keeps the coarse planes in a
+// ring of three LDS slots (4 rows x (16 NSEG + 2) columns each, zero outside the grid, stored
+// as [8-channel piece][voxel] with a 256-byte multiple piece stride): while plane k is
+// computed, coarse plane k + 1 + rz is loaded into registers, then written into the slot
+// nobody reads in step k; one barrier per step. A K = 32 step is one coarse tap (oz, oy) x 32
+// channels; its B fragments at coarse x offsets -1, 0, +1 serve the four (rx, ox) products.
+// ADD: the epilogue also writes Y2 = bf16(Y + E), with E loaded before the MFMAs.
 template <int CIN, int COUT, bool RELU, int NSEG, bool ADD>
 __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Ws,
-                                                         uint16_t* __restrict__ Y, int D, int H, int W,
+                                                         uint16_t* __restrict__ Y, int D, int H, int W, int ZB,
                                                          const uint16_t* __restrict__ E, uint16_t* __restrict__ Y2) {
-    constexpr int CX = 16 * NSEG + 2, NPL = CX * 4 * 2;          // staged coarse voxels
+    constexpr int CX = 16 * NSEG + 2, NPL = CX * 4;              // coarse voxels of one staged plane
     constexpr int NPS = (NPL + 15) / 16 * 16;                    // LDS stride of one 8-channel piece
     constexpr int CH8 = CIN / 8, KS = CIN / 32;
-    constexpr int PPL = CH8 * NPS;                               // staged pieces (padding included)
-    constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;
+    constexpr int SLOT = CH8 * NPS;                              // LDS pieces of one ring slot
+    constexpr int RPL = (SLOT + kNetThreads - 1) / kNetThreads;
     constexpr int NGRP = COUT / 16;
-    __shared__ uint4 st[PPL];
+    __shared__ uint4 ring[3 * SLOT];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = blockIdx.y % NGRP;
-    const int y0 = (blockIdx.y / NGRP) * 4, z = blockIdx.z;
+    const int y0 = (blockIdx.y / NGRP) * 4;
+    const int rz = blockIdx.z & 1;
     const int Dc = D >> 1, Hc = H >> 1, Wc = W >> 1;
+    const int k0 = (blockIdx.z >> 1) * ZB;
+    const int k1 = (k0 + ZB < Dc) ? k0 + ZB : Dc;
     const int px0 = blockIdx.x * (16 * NSEG);
-    const int y = y0 + wave, ry = wave & 1, rz = z & 1;
+    const int y = y0 + wave, ry = wave & 1;
     const int i = lane & 15, kq = lane >> 4;
-    const int cz0 = (z >> 1) - 1 + rz, cy0 = (y0 >> 1) - 1;
-    const long long vrow = (static_cast<long long>(z) * H + (y < H ? y : H - 1)) * W;
-    // skip operand of the epilogue, issued first so its latency hides behind the staging and
-    // the MFMAs (lanes past the row end read nothing)
-    uint2 ev[2][NSEG];
-    if constexpr (ADD) {
-#pragma unroll
-        for (int s = 0; s < NSEG; s++)
-#pragma unroll
-            for (int rx = 0; rx < 2; rx++) {
-                const int x = 2 * (px0 + 16 * s + i) + rx;
-                ev[rx][s] = (y < H && x < W)
-                    ? *reinterpret_cast<const uint2*>(E + (vrow + x) * COUT + grp * 16 + kq * 4)
-                    : uint2{0u, 0u};
-            }
-    }
-    // weight fragments of this lane (they come from L2)
+    const int cy0 = (y0 >> 1) - 1;
     bf16x8 wr[2][8][KS];
 #pragma unroll
     for (int rx = 0; rx < 2; rx++)
@@ -487,102 +477,134 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
             for (int c = 0; c < KS; c++)
                 wr[rx][o][c] = load8(Ws + ((((rz * 4 + ry * 2 + rx) * 8 + o) * COUT + grp * 16 + i) * CIN + 32 * c + 8 * kq),
                                      true);
-    // stage: 16 consecutive threads move 16 consecutive voxels of one 8-channel piece
-    {
-        uint4 u[RPL];
+    // one coarse plane cz: 16 consecutive threads move 16 consecutive voxels of one piece
+    auto load_plane = [&](int cz, uint4* u) {
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
             const int t = r * kNetThreads + threadIdx.x;
             const int g = t >> 4, p8 = g % CH8, v = (g / CH8) * 16 + (t & 15);
             u[r] = uint4{0u, 0u, 0u, 0u};
-            if (v < NPL) {
-                const int gx = px0 - 1 + v % CX, gy = cy0 + (v / CX) % 4, gz = cz0 + v / (CX * 4);
-                if (gx >= 0 && gx < Wc && gy >= 0 && gy < Hc && gz >= 0 && gz < Dc)
+            if (v < NPL && cz >= 0 && cz < Dc) {
+                const int gx = px0 - 1 + v % CX, gy = cy0 + v / CX;
+                if (gx >= 0 && gx < Wc && gy >= 0 && gy < Hc)
                     u[r] = *reinterpret_cast<const uint4*>(
-                        X + ((static_cast<long long>(gz) * Hc + gy) * Wc + gx) * CIN + 8 * p8);
+                        X + ((static_cast<long long>(cz) * Hc + gy) * Wc + gx) * CIN + 8 * p8);
             }
         }
+    };
+    auto store_plane = [&](int cz, const uint4* u) {
+        const int slot = (cz + 3) % 3;
 #pragma unroll
         for (int r = 0; r < RPL; r++) {
             const int t = r * kNetThreads + threadIdx.x;
             const int g = t >> 4, p8 = g % CH8, v = (g / CH8) * 16 + (t & 15);
-            if (t < PPL) st[p8 * NPS + v] = u[r];
+            if (t < SLOT) ring[slot * SLOT + p8 * NPS + v] = u[r];
+        }
+    };
+    {
+        uint4 u[RPL];
+        for (int q = 0; q < 2; q++) {                            // coarse planes k0 - 1 + rz, k0 + rz
+            load_plane(k0 - 1 + rz + q, u);
+            store_plane(k0 - 1 + rz + q, u);
         }
     }
     __syncthreads();
-    if (y >= H) return;                                         // wave-uniform; no barrier follows
-    floatx4 acc[2][NSEG];
+    for (int k = k0; k < k1; k++) {
+        const int z = 2 * k + rz;
+        uint4 nxt[RPL];
+        if (k + 1 < k1) load_plane(k + 1 + rz, nxt);            // block-uniform condition
+        const long long vrow = (static_cast<long long>(z) * H + (y < H ? y : H - 1)) * W;
+        uint2 ev[2][NSEG];
+        if constexpr (ADD) {
 #pragma unroll
-    for (int rx = 0; rx < 2; rx++)
+            for (int s = 0; s < NSEG; s++)
 #pragma unroll
-        for (int s = 0; s < NSEG; s++) acc[rx][s] = floatx4{0.f, 0.f, 0.f, 0.f};
-    constexpr int NST = 4 * KS;                                 // (oz, oy, c) steps
-    bf16x8 bm[NSEG], b0[NSEG], bp[NSEG];
-    auto fetch = [&](int q, bf16x8* m, bf16x8* c0, bf16x8* p) {
-        const int oz = q / (2 * KS), oy = (q / KS) % 2, c = q % KS;
-        const int base = (4 * c + kq) * NPS + (oz * 4 + (wave >> 1) + oy + ry) * CX + i + 1;
-#pragma unroll
-        for (int s = 0; s < NSEG; s++) {
-            m[s] = __builtin_bit_cast(bf16x8, st[base + 16 * s - 1]);
-            c0[s] = __builtin_bit_cast(bf16x8, st[base + 16 * s]);
-            p[s] = __builtin_bit_cast(bf16x8, st[base + 16 * s + 1]);
+                for (int rx = 0; rx < 2; rx++) {
+                    const int x = 2 * (px0 + 16 * s + i) + rx;
+                    ev[rx][s] = (y < H && x < W)
+                        ? *reinterpret_cast<const uint2*>(E + (vrow + x) * COUT + grp * 16 + kq * 4)
+                        : uint2{0u, 0u};
+                }
         }
-    };
-    fetch(0, bm, b0, bp);
+        if (y < H) {                                             // wave-uniform
+            floatx4 acc[2][NSEG];
 #pragma unroll
-    for (int q = 0; q < NST; q++) {
-        bf16x8 nm[NSEG], n0[NSEG], np[NSEG];
-        if (q + 1 < NST) fetch(q + 1, nm, n0, np);
-        __builtin_amdgcn_sched_barrier(0);   // keep the reads of q + 1 ahead of these MFMAs
-        const int oz = q / (2 * KS), oy = (q / KS) % 2, c = q % KS;
-        const int o0 = oz * 4 + oy * 2;                          // ox = 0; ox = 1 is o0 + 1
+            for (int rx = 0; rx < 2; rx++)
 #pragma unroll
-        for (int s = 0; s < NSEG; s++) {
-            acc[0][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[0][o0][c], bm[s], acc[0][s], 0, 0, 0);
-            acc[1][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[1][o0][c], b0[s], acc[1][s], 0, 0, 0);
-            acc[0][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[0][o0 + 1][c], b0[s], acc[0][s], 0, 0, 0);
-            acc[1][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[1][o0 + 1][c], bp[s], acc[1][s], 0, 0, 0);
-        }
-        if (q + 1 < NST) {
+                for (int s = 0; s < NSEG; s++) acc[rx][s] = floatx4{0.f, 0.f, 0.f, 0.f};
+            constexpr int NST = 4 * KS;                          // (oz, oy, c) steps
+            bf16x8 bm[NSEG], b0[NSEG], bp[NSEG];
+            auto fetch = [&](int q, bf16x8* m, bf16x8* c0, bf16x8* p) {
+                const int oz = q / (2 * KS), oy = (q / KS) % 2, c = q % KS;
+                const int slot = (k - 1 + rz + oz + 3) % 3;
+                const int base = slot * SLOT + (4 * c + kq) * NPS + ((wave >> 1) + oy + ry) * CX + i + 1;
 #pragma unroll
-            for (int s = 0; s < NSEG; s++) {
-                bm[s] = nm[s];
-                b0[s] = n0[s];
-                bp[s] = np[s];
+                for (int s = 0; s < NSEG; s++) {
+                    m[s] = __builtin_bit_cast(bf16x8, ring[base + 16 * s - 1]);
+                    c0[s] = __builtin_bit_cast(bf16x8, ring[base + 16 * s]);
+                    p[s] = __builtin_bit_cast(bf16x8, ring[base + 16 * s + 1]);
+                }
+            };
+            fetch(0, bm, b0, bp);
+#pragma unroll
+            for (int q = 0; q < NST; q++) {
+                bf16x8 nm[NSEG], n0[NSEG], np[NSEG];
+                if (q + 1 < NST) fetch(q + 1, nm, n0, np);
+                __builtin_amdgcn_sched_barrier(0);   // keep the reads of q + 1 ahead of these MFMAs
+                const int oz = q / (2 * KS), oy = (q / KS) % 2, c = q % KS;
+                const int o0 = oz * 4 + oy * 2;                  // ox = 0; ox = 1 is o0 + 1
+#pragma unroll
+                for (int s = 0; s < NSEG; s++) {
+                    acc[0][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[0][o0][c], bm[s], acc[0][s], 0, 0, 0);
+                    acc[1][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[1][o0][c], b0[s], acc[1][s], 0, 0, 0);
+                    acc[0][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[0][o0 + 1][c], b0[s], acc[0][s], 0, 0, 0);
+                    acc[1][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[1][o0 + 1][c], bp[s], acc[1][s], 0, 0, 0);
+                }
+                if (q + 1 < NST) {
+#pragma unroll
+                    for (int s = 0; s < NSEG; s++) {
+                        bm[s] = nm[s];
+                        b0[s] = n0[s];
+                        bp[s] = np[s];
+                    }
+                }
             }
+#pragma unroll
+            for (int s = 0; s < NSEG; s++)
+#pragma unroll
+                for (int rx = 0; rx < 2; rx++) {
+                    const int x = 2 * (px0 + 16 * s + i) + rx;
+                    if (x >= W) continue;
+                    uint16_t b[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        float o = acc[rx][s][j];
+                        if (RELU) o = o > 0.f ? o : 0.f;
+                        b[j] = f2bf(o);
+                    }
+                    const long long at = (vrow + x) * COUT + grp * 16 + kq * 4;
+                    uint2 pk;
+                    pk.x = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 16);
+                    pk.y = static_cast<uint32_t>(b[2]) | (static_cast<uint32_t>(b[3]) << 16);
+                    *reinterpret_cast<uint2*>(Y + at) = pk;
+                    if constexpr (ADD) {
+                        const uint2 e2 = ev[rx][s];
+                        const uint16_t e[4] = {static_cast<uint16_t>(e2.x & 0xffffu), static_cast<uint16_t>(e2.x >> 16),
+                                               static_cast<uint16_t>(e2.y & 0xffffu), static_cast<uint16_t>(e2.y >> 16)};
+                        uint16_t t[4];
+#pragma unroll
+                        for (int j = 0; j < 4; j++) t[j] = f2bf(bf2f(b[j]) + bf2f(e[j]));
+                        uint2 sk;
+                        sk.x = static_cast<uint32_t>(t[0]) | (static_cast<uint32_t>(t[1]) << 16);
+                        sk.y = static_cast<uint32_t>(t[2]) | (static_cast<uint32_t>(t[3]) << 16);
+                        *reinterpret_cast<uint2*>(Y2 + at) = sk;
+                    }
+                }
         }
+        // the slot of coarse plane k + 1 + rz held plane k - 2 + rz, last read in step k - 1
+        if (k + 1 < k1) store_plane(k + 1 + rz, nxt);
+        __syncthreads();
     }
-#pragma unroll
-    for (int s = 0; s < NSEG; s++)
-#pragma unroll
-        for (int rx = 0; rx < 2; rx++) {
-            const int x = 2 * (px0 + 16 * s + i) + rx;
-            if (x >= W) continue;
-            uint16_t b[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float o = acc[rx][s][j];
-                if (RELU) o = o > 0.f ? o : 0.f;
-                b[j] = f2bf(o);
-            }
-            const long long at = (vrow + x) * COUT + grp * 16 + kq * 4;
-            uint2 pk;
-            pk.x = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 16);
-            pk.y = static_cast<uint32_t>(b[2]) | (static_cast<uint32_t>(b[3]) << 16);
-            *reinterpret_cast<uint2*>(Y + at) = pk;
-            if constexpr (ADD) {
-                const uint2 e2 = ev[rx][s];
-                const uint16_t e[4] = {static_cast<uint16_t>(e2.x & 0xffffu), static_cast<uint16_t>(e2.x >> 16),
-                                       static_cast<uint16_t>(e2.y & 0xffffu), static_cast<uint16_t>(e2.y >> 16)};
-                uint16_t t[4];
-#pragma unroll
-                for (int j = 0; j < 4; j++) t[j] = f2bf(bf2f(b[j]) + bf2f(e[j]));
-                uint2 sk;
-                sk.x = static_cast<uint32_t>(t[0]) | (static_cast<uint32_t>(t[1]) << 16);
-                sk.y = static_cast<uint32_t>(t[2]) | (static_cast<uint32_t>(t[3]) << 16);
-                *reinterpret_cast<uint2*>(Y2 + at) = sk;
-            }
-        }
 }
 
 // Weight gradient partials: P[c][tap][co][ci] = sum over the voxels of chunk c of
@@ -1110,16 +1132,24 @@ void conv_up(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Ws
     hipLaunchKernelGGL((k_sub_weights<CIN, COUT>), dim3((64 * CIN * COUT + kNetThreads - 1) / kNetThreads), blk, 0, sm,
                        Wt, Ws);
     const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
-    constexpr int NMAX = CIN == 32 ? 4 : 2;                     // registers: 2 x 8 x CIN/32 weight fragments
-    if (NMAX == 4 && W >= 128)
-        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, NMAX, true>), dim3((W + 127) / 128, gy, D), blk, 0, sm, X, Ws,
-                           Y, D, H, W, E, Y2);
-    else if (W >= 64)
-        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 2, true>), dim3((W + 63) / 64, gy, D), blk, 0, sm, X, Ws, Y, D,
-                           H, W, E, Y2);
+    // tile width by registers (2 x 8 x CIN/32 weight fragments + the operands of two K steps):
+    // two waves per SIMD at 32 (CIN 32) or 16 (CIN 64) coarse columns
+    constexpr int NMAX = CIN == 32 ? 2 : 1;
+    const int nseg = std::min(NMAX, W >= 128 ? 4 : (W >= 64 ? 2 : 1));
+    const unsigned gx = static_cast<unsigned>((W + 32 * nseg - 1) / (32 * nseg));
+    // z-march per parity: blocks per parity column sized so that the whole grid is resident
+    int per_cu = 1;
+    if (nseg == 2) per_cu = resident_blocks<k_conv_sub<CIN, COUT, true, 2, true>>();
+    else per_cu = resident_blocks<k_conv_sub<CIN, COUT, true, 1, true>>();
+    const int Dc = D >> 1;
+    const long long cols = 2LL * gx * gy;
+    const long long nzb = std::max<long long>(1, static_cast<long long>(device_cus()) * per_cu / cols);
+    const int zb = static_cast<int>((Dc + nzb - 1) / nzb);
+    const dim3 grid(gx, gy, 2u * static_cast<unsigned>((Dc + zb - 1) / zb));
+    if (nseg == 2)
+        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 2, true>), grid, blk, 0, sm, X, Ws, Y, D, H, W, zb, E, Y2);
     else
-        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 1, true>), dim3((W + 31) / 32, gy, D), blk, 0, sm, X, Ws, Y, D,
-                           H, W, E, Y2);
+        hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 1, true>), grid, blk, 0, sm, X, Ws, Y, D, H, W, zb, E, Y2);
 }
 
 // dispatch on (cin, cout) of the network's layers
