@@ -495,9 +495,24 @@ def cpu_baseline(args, W, ctx):
     t1 = time.perf_counter()
     x, ok, it, res, _ = orc.bicgstab2(rp, col, val, rhs, x0, TOL * float(np.linalg.norm(rhs)), 1000)
     t2 = time.perf_counter()
+    # the same step on one thread (the survey asks for both), and what the host is
+    orc.set_threads(1)
+    t3 = time.perf_counter()
+    rp, col, val, rhs, x0 = orc.assemble(go, t, phi)
+    orc.bicgstab2(rp, col, val, rhs, x0, TOL * float(np.linalg.norm(rhs)), 1000)
+    t4 = time.perf_counter()
+    orc.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(1e3 * (t2 - t0), 1), "unit": "ms", "cores": threads, "kind": "port",
             "sample": f"one full {args.n}^3 implicit step: assembly {1e3 * (t1 - t0):.0f} ms + "
-                      f"BiCGStab2 {1e3 * (t2 - t1):.0f} ms ({it} passes, converged={ok})"}
+                      f"BiCGStab2 {1e3 * (t2 - t1):.0f} ms ({it} passes, converged={ok})",
+            "single_thread_ms": round(1e3 * (t4 - t3), 1),
+            "host": {"model": model, "nproc": os.cpu_count()}}
 
 
 if __name__ == "__main__":
