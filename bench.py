@@ -159,6 +159,13 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     copy_gbs = 2 * src.numel() * 8 / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
+    # and the write-only rate (1 GiB fill), the ceiling of write-heavy kernels like the gradient
+    e0.record()
+    for _ in range(reps):
+        dst.fill_(1.0)
+    e1.record()
+    torch.cuda.synchronize()
+    write_gbs = dst.numel() * 8 / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
     del src, dst
 
     # the stencil kernels, same method: TVD step (phi + 3 face velocities in, phi out), face
@@ -242,6 +249,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": spmv_bytes, "bytes_per_row": 80, "avg_launch_us": round(spmv_s * 1e6, 2),
             "attainable_copy_GBs": round(copy_gbs, 1),
+            "attainable_write_GBs": round(write_gbs, 1),
             "frac_of_attainable": round(achieved / copy_gbs, 4),
         },
         "stencil_rooflines": stencils,
