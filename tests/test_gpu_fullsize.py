@@ -6,6 +6,9 @@ the same seeded input, bit for bit. Two fields are used: phi ~ U[0,1) everywhere
 ghost layers (the operator-comparison field), and the step cube, whose flat regions give the
 0/0 and x/0 slope ratios that the limiters map to 0 (FluxLimiter.h:129,170).
 
+The C2 implicit step at 256^3: the device assembly against the oracle's bit for bit, and the
+FP64 and mixed-precision solves against the oracle's solve.
+
 64^3 Gaussian blob (D = 1e-3, dt = 1e-2, ten implicit Euler steps, Dirichlet 0): the device
 path (assembly, BiCGStab2, copy back) against the oracle step by step, and the result against
 the free-space heat kernel sigma^2 -> sigma^2 + 2 D t, amplitude (sigma^2/(sigma^2+2Dt))^(3/2).
@@ -137,4 +140,52 @@ def test_c1_blob_ten_steps_oracle_and_heat_kernel(ctx):
     assert err <= 5e-4
     # and the decay of the peak has the right size (not a trivially small change)
     assert abs(got.max() - 1.0) > 10 * err
+    A.close()
+
+
+def test_c2_assembly_and_solve_full_size(ctx, c2):
+    """The C2 implicit step at 256^3 (16.8 M rows, 117 M entries): the device assembly of the
+    TVD-advected field against the oracle's, bit for bit (row pointers, columns, values, rhs,
+    x0); the FP64 BiCGStab2 solve against the oracle's solve of the same system (pass counts
+    within one, solutions within 1e-12 relative when equal, the reference stopping rule at
+    1e-8 relative); and the mixed-precision solve to the same FP64 true residual."""
+    import torch
+    W, vel = c2
+    go = orc.grid([256] * 3, size=[1.0, 1.0, 1.0], ghost=2)
+    g = _grid(go)
+    print("[c2 solve] advect and assemble", flush=True)
+    da.ghost_fill(ctx, g, W.phi, W.ghost_modes, W.ghost_values)
+    da.tvd_explicit_step(ctx, g, da.VANLEER, W.phi, W.vel, W.dt, W.phi_work)
+    A = da.Csr(ctx)
+    da.build_structure(ctx, A, g, W.terms)
+    rhs = torch.zeros(256 ** 3, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(rhs)
+    da.fill_values(ctx, A, g, W.terms, W.phi_work, rhs, x)
+    torch.cuda.synchronize()
+    phi = W.phi_work.cpu().numpy()
+    t = orc.terms(dt=W.dt, diff=W.D, bc_type=[1] * 6, bc_value=[0.0] * 6)
+    rp, col, val, orhs, ox0 = orc.assemble(go, t, phi)
+    drp, dcol, dval = A.download()
+    assert np.array_equal(drp, rp) and np.array_equal(dcol, col), "structure"
+    assert bits_equal(dval, val), "values"
+    assert bits_equal(rhs.cpu().numpy(), orhs) and bits_equal(x.cpu().numpy(), ox0), "rhs / x0"
+    del drp, dcol, dval
+    print("[c2 solve] FP64 BiCGStab2 on both sides", flush=True)
+    tol = 1e-8 * float(np.linalg.norm(orhs))
+    xo, ok, it, res, _ = orc.bicgstab2(rp, col, val, orhs, ox0, tol, 100)
+    assert ok
+    x64 = x.clone()
+    r = da.bicgstab2(ctx, A, rhs, x64, tol=tol, tol_kind=0, max_it=100)
+    assert r["converged"] == 1 and abs(r["iterations"] - it) <= 1, (r["iterations"], it)
+    xd = x64.cpu().numpy()
+    # same pass count: the two solves differ by summation order only; one pass apart, by up to
+    # what one pass changes near the tolerance
+    bound = 1e-12 if r["iterations"] == it else 1e-6
+    assert np.max(np.abs(xd - xo)) <= bound * np.max(np.abs(xo))
+    assert np.linalg.norm(orhs - orc.spmv(rp, col, val, xd)) <= tol * 1.001
+    print("[c2 solve] mixed precision", flush=True)
+    xm = x.clone()
+    r = da.refine_mixed(ctx, A, rhs, xm, tol=1e-8, tol_kind=1, inner_tol=1e-4)
+    assert r["converged"] == 1
+    assert np.linalg.norm(orhs - orc.spmv(rp, col, val, xm.cpu().numpy())) <= tol
     A.close()
