@@ -262,6 +262,7 @@ def main():
 
 
 WEIGHTS = os.path.join(ROOT, "dare_amd", "weights")
+WEIGHTS_OVERRIDE = None        # a weights file for the target leg's network (experiments)
 
 
 def _true_rel(ctx, A, b, x):
@@ -285,7 +286,12 @@ def target_leg(ctx, n, steps, inner_tol=1e-4):
     log(f"[bench] target: {n}^3 variable-diffusivity steps")
     W = VarDiffusion(ctx, seed=5000, n=n)
     net = None
-    wpath = os.path.join(WEIGHTS, "ws_vardiff_128.npz")
+    # the network fine-tuned at this resolution when there is one (crops of 256^3 trajectories),
+    # else the 128^3 one
+    wpath = os.path.join(WEIGHTS, f"ws_vardiff_{n}.npz")
+    if not os.path.exists(wpath):
+        wpath = os.path.join(WEIGHTS, "ws_vardiff_128.npz")
+    wpath = WEIGHTS_OVERRIDE or wpath
     if os.path.exists(wpath):
         w, meta = WarmStartNet.load_params(wpath)
         net = WarmStartNet(ctx, [n] * 3, in_channels=meta["in_channels"], params=w)
