@@ -2,6 +2,9 @@
 
     python -m dare_amd.warmstart train --n 128 --train-seeds 200 --steps 200 --out ws.npz
     python -m dare_amd.warmstart eval  --n 128 --weights ws.npz --eval-seeds 20
+    python -m dare_amd.warmstart train --n 256 --workload vardiff --train-seeds 400 --steps 6 \
+        --crop 64 --lr 1e-3 --lr-final 5e-5 --weights dare_amd/weights/ws_vardiff_128.npz \
+        --out dare_amd/weights/ws_vardiff_256.npz        (fine-tuning at 256^3)
 
 train: for every training seed, the plain FP64 BiCGStab2 advances the advection-diffusion
 trajectory (C5AdvDiff) from the reference's guess (phi^n); each step's (phi^n, phi^{n-1},
@@ -159,6 +162,7 @@ def main(argv=None):
     ap.add_argument("--eval-seeds", type=int, default=20)
     ap.add_argument("--eval-steps", type=int, default=None)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-final", type=float, default=1e-4)
     ap.add_argument("--train-every", type=int, default=1)
     ap.add_argument("--crop", type=int, default=None)
     ap.add_argument("--max-seconds", type=float, default=None)
@@ -175,7 +179,7 @@ def main(argv=None):
             init, _ = WarmStartNet.load_params(a.weights)
         net = WarmStartNet(ctx, [a.n] * 3, in_channels=WORKLOADS[a.workload].in_channels_for(), params=init)
         net, info = train(ctx, n=a.n, train_seeds=range(1000, 1000 + a.train_seeds), steps=a.steps, lr=a.lr,
-                          net=net, train_every=a.train_every, max_seconds=a.max_seconds, workload=a.workload,
+                          lr_final=a.lr_final, net=net, train_every=a.train_every, max_seconds=a.max_seconds, workload=a.workload,
                           crop=a.crop)
         report["train"] = info
         _log(f"[train] {info}")
