@@ -365,10 +365,11 @@ __device__ __forceinline__ void tvd_march3(const Geo& G, int lim, const double* 
     }
 }
 
-// Walks logical block L's column-plane range [L*chunk, (L+1)*chunk) (columns of 64 (k) x 4
-// (j) cells ordered k-tile fastest, planes along i) and calls march(k, j, i0, i1) for each
-// column piece; blocks are dealt XCD by XCD (blocks b = x mod 8 own a contiguous run).
-template <class F>
+// Walks logical block L's column-plane range [L*chunk, (L+1)*chunk) (columns of 64 PW (k) x 4
+// (j) cells ordered k-tile fastest, planes along i; lane l takes cells PW l .. PW l + PW - 1 of
+// its column) and calls march(k, j, i0, i1) for each column piece with k its first cell; blocks
+// are dealt XCD by XCD (blocks b = x mod 8 own a contiguous run).
+template <int PW = 1, class F>
 __device__ __forceinline__ void march_columns(const Geo& G, int tiles_k, long long total, long long chunk,
                                               int per_xcd, F&& march) {
     const long long L = static_cast<long long>(blockIdx.x % kStepXcd) * per_xcd + blockIdx.x / kStepXcd;
@@ -379,7 +380,7 @@ __device__ __forceinline__ void march_columns(const Geo& G, int tiles_k, long lo
         const long long col = c / n0;
         const int i0 = static_cast<int>(c - col * n0);
         const int i1 = (cend - c < n0 - i0) ? i0 + static_cast<int>(cend - c) : n0;
-        const int k = static_cast<int>(col % tiles_k) * kStepTileK + (threadIdx.x & 63);
+        const int k = (static_cast<int>(col % tiles_k) * kStepTileK + (threadIdx.x & 63)) * PW;   // PW cells per lane
         const int j = static_cast<int>(col / tiles_k) * kStepTileJ + (threadIdx.x >> 6);
         if (k < G.n[2] && j < G.n[1]) march(k, j, i0, i1);
         c += i1 - i0;
@@ -555,6 +556,59 @@ __global__ __launch_bounds__(256) void k_grad3(Geo G, const double* __restrict__
     });
 }
 
+// k_grad3 with two k-adjacent cells per lane (n2 even, 16-byte aligned arrays): phi and the x
+// and y faces move as 16-byte vectors, the z faces too where the row starts on an even face
+// (z-face rows hold n2 + 1 faces). Each face is the same difference times the same factor as
+// in k_grad3, so the results are bit-identical.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_grad3_pair(Geo G, const double* __restrict__ phi, double* __restrict__ fx,
+                                                    double* __restrict__ fy, double* __restrict__ fz, int tiles_k,
+                                                    long long total, long long chunk, int per_xcd) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    auto ld2 = [](const double* p) { return *reinterpret_cast<const d2*>(p); };
+    auto st2 = [](double* p, d2 v) {
+        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<d2*>(p));
+        else *reinterpret_cast<d2*>(p) = v;
+    };
+    march_columns<2>(G, tiles_k, total, chunk, per_xcd, [&](int k, int j, int i0, int i1) {
+        const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+        const long long sI = G.sl[0], sJ = G.sl[1];
+        const double d0 = G.dnr[0], d1 = G.dnr[1], dz2 = G.dnr[2];
+        const bool lastj = j == n1 - 1, lastk = k == n2 - 2;     // cell k + 1 is the last one
+        long long lp = (static_cast<long long>(i0 + G.g) * G.np[1] + (j + G.g)) * G.np[2] + (k + G.g);
+        const long long fxs = static_cast<long long>(n1) * n2;
+        long long ox = static_cast<long long>(i0) * fxs + static_cast<long long>(j) * n2 + k;
+        long long oy = (static_cast<long long>(i0) * (n1 + 1) + j) * n2 + k;
+        const long long fys = static_cast<long long>(n1 + 1) * n2;
+        long long oz = (static_cast<long long>(i0) * n1 + j) * (n2 + 1) + k;
+        const long long fzs = static_cast<long long>(n1) * (n2 + 1);
+        d2 pm = ld2(phi + lp - sI);
+        for (int i = i0; i < i1; i++) {
+            const d2 p0 = ld2(phi + lp), pS = ld2(phi + lp - sJ);
+            const double pB = phi[lp - 1];
+            const d2 pN = ld2(phi + (lastj ? lp + sJ : lp));
+            const double pT = phi[lastk ? lp + 2 : lp + 1];
+            st2(fx + ox, (p0 - pm) * d0);
+            st2(fy + oy, (p0 - pS) * d1);
+            if (lastj) st2(fy + oy + n2, (pN - p0) * d1);
+            const double z0 = (p0.x - pB) * dz2, z1 = (p0.y - p0.x) * dz2;
+            if ((oz & 1) == 0) {
+                st2(fz + oz, d2{z0, z1});
+            } else {
+                put<NT>(fz + oz, z0);
+                put<NT>(fz + oz + 1, z1);
+            }
+            if (lastk) put<NT>(fz + oz + 2, (pT - p0.y) * dz2);
+            pm = p0;
+            lp += sI;
+            ox += fxs;
+            oy += fys;
+            oz += fzs;
+        }
+        if (i1 == n0) st2(fx + ox, (ld2(phi + lp) - pm) * d0);   // lp is now the ghost plane n0
+    });
+}
+
 // 3-D, one component: value divergence A0 E - A0 W + A1 N - A1 S + A2 T - A2 B (the order of
 // k_div_faces) marching along i with the x face shared by consecutive planes in a register.
 template <bool NT>
@@ -618,9 +672,10 @@ struct MarchPlan {
     unsigned grid;
 };
 
-inline MarchPlan march_plan(const dare_ctx* ctx, const Geo& G, int per_cu, int tile_j = kStepTileJ) {
+inline MarchPlan march_plan(const dare_ctx* ctx, const Geo& G, int per_cu, int tile_j = kStepTileJ,
+                            int tile_k = kStepTileK) {
     MarchPlan P;
-    P.tiles_k = static_cast<int>(ceil_div(G.n[2], kStepTileK));
+    P.tiles_k = static_cast<int>(ceil_div(G.n[2], tile_k));
     P.total = static_cast<long long>(P.tiles_k) * ceil_div(G.n[1], tile_j) * G.n[0];
     long long nb = static_cast<long long>(ctx->num_cu) * per_cu;
     const long long by_work = ceil_div(P.total, kStepMinPlanes);
@@ -649,6 +704,15 @@ void launch_step3(dare_ctx* ctx, const Geo& G, const double* phi, const double* 
 
 template <bool NT>
 void launch_grad3(dare_ctx* ctx, const Geo& G, const double* phi, double* fx, double* fy, double* fz) {
+    const auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    if (G.n[2] % 2 == 0 && G.np[2] % 2 == 0 && G.g % 2 == 0 && aligned(phi) && aligned(fx) && aligned(fy) &&
+        aligned(fz)) {
+        static const int per_cu2 = resident_per_cu(k_grad3_pair<NT>);
+        const MarchPlan P = march_plan(ctx, G, per_cu2, kStepTileJ, 2 * kStepTileK);
+        hipLaunchKernelGGL((k_grad3_pair<NT>), dim3(P.grid), dim3(256), 0, ctx->stream, G, phi, fx, fy, fz, P.tiles_k,
+                           P.total, P.chunk, P.per_xcd);
+        return;
+    }
     static const int per_cu = resident_per_cu(k_grad3<NT>);
     const MarchPlan P = march_plan(ctx, G, per_cu);
     hipLaunchKernelGGL((k_grad3<NT>), dim3(P.grid), dim3(256), 0, ctx->stream, G, phi, fx, fy, fz, P.tiles_k, P.total,
