@@ -107,6 +107,14 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
     for (int d = 0; d < dim; d++) lp += G.sl[d] * (q[d] + G.g);
     const int N = G.N;
 
+    // a cell on a DARE_BC_FIXED face (the first axis that has one, low face first) is an
+    // identity row: every neighbour and the cross-component entry removed
+    int fixed_face = -1;
+    for (int d = 0; d < dim && fixed_face < 0; d++) {
+        if (q[d] == 0 && T.bc_type[2 * d] == DARE_BC_FIXED) fixed_face = 2 * d;
+        else if (q[d] == G.n[d] - 1 && T.bc_type[2 * d + 1] == DARE_BC_FIXED) fixed_face = 2 * d + 1;
+    }
+    const bool fixed = fixed_face >= 0;
     // boundary faces hit by an edit (they only depend on the structure)
     int bc_face[3] = {-1, -1, -1};
     for (int d = 0; d < dim; d++) {
@@ -116,6 +124,8 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
         if (bc_face[d] == 2 * d) removed[2 * d + 1] = true;
         else if (bc_face[d] == 2 * d + 1) removed[2 * d + 2] = true;
     }
+    if (fixed)
+        for (int pos = 1; pos < 7; pos++) removed[pos] = true;
 
     if (VALUES) {
         const double* phi = T.phi;
@@ -215,6 +225,12 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
         }
         *rhs_out = rhs;
         *x0_out = phi[lp * N + c];
+        if (fixed) {                                     // Get<CENTER> = 1, rhs = x0 = value
+            nbv[0] = 1.;
+            cross = 0.;
+            *rhs_out = T.bc_value[fixed_face];
+            *x0_out = T.bc_value[fixed_face];
+        }
         if (slotv) {      // diagonal view: W,S,B,C,T,N,E,cross; removed entries hold 0
             for (int sl = 0; sl < 7; sl++) {
                 const int pos = kPosOfSlot[sl];
@@ -233,7 +249,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
     long long kc[kMaxEntries];
     double kv[kMaxEntries];
     int k = 0;
-    if (has_cross) {
+    if (has_cross && !fixed) {
         kc[k] = node * N + (1 - c);
         kv[k] = cross;
         k++;
@@ -461,7 +477,7 @@ int make_terms(const Geo& G, const dare_terms* t, const double* phi, Terms* T) {
     for (int f = 0; f < 6; f++) {
         T->bc_type[f] = f < 2 * G.dim ? t->bc_type[f] : 0;
         T->bc_value[f] = t->bc_value[f];
-        if (T->bc_type[f] < 0 || T->bc_type[f] > DARE_BC_NEUMANN) {
+        if (T->bc_type[f] < 0 || T->bc_type[f] > DARE_BC_FIXED) {
             set_error("invalid boundary type");
             return DARE_ERR_INVALID;
         }

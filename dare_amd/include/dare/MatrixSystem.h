@@ -295,6 +295,9 @@ struct Terms {
     Terms& Reaction(const double* dev_blocks) { t.react = dev_blocks; return *this; }
     Terms& Dirichlet(int face, double v) { t.bc_type[face] = DARE_BC_DIRICHLET; t.bc_value[face] = v; return *this; }
     Terms& Neumann(int face) { t.bc_type[face] = DARE_BC_NEUMANN; return *this; }
+    // the boundary cells of this face carry v themselves (identity rows): Dirichlet on a
+    // staggered axis, whose first / last cells sit on the boundary faces
+    Terms& Fixed(int face, double v) { t.bc_type[face] = DARE_BC_FIXED; t.bc_value[face] = v; return *this; }
 };
 
 // A built preconditioner (the RCP<OperatorType> M of the reference): which right

@@ -47,6 +47,10 @@ extern "C" {
 #define DARE_BC_NONE 0
 #define DARE_BC_DIRICHLET 1
 #define DARE_BC_NEUMANN 2
+/* the boundary cell itself carries the value: identity row (centre 1, rhs = x0 = value, no
+ * neighbours) -- the Dirichlet condition of a staggered axis, whose first and last cells sit
+ * on the boundary faces (the functor of test_integration_Cartesian_Trilinos.cpp:155-220) */
+#define DARE_BC_FIXED 3
 
 /* Right preconditioners of the solver (dare_solver_params.precond). */
 #define DARE_PRECOND_NONE 0

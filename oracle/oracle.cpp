@@ -457,10 +457,27 @@ struct MB {
 // The assembled operator: the per-cell functor a reference user writes for these terms
 // (pattern of counter_diffusion2D.cpp:106-162 and advection_periodic2D.cpp:108-125):
 //   mb = ddt(field); mb -= div(D_f * grad(mb)); mb += div(tvd * field);
-//   rhs += V s;  reaction; boundary edits (Dirichlet mirror / Neumann) + Remove.
+//   rhs += V s;  reaction; boundary edits (Dirichlet mirror / Neumann) + Remove; fixed
+//   boundary cells (identity rows) instead of all of it.
 void functor(const Geo& G, const orc_terms* t, const double* phi, MB* mb) {
     const Ind p = internal_to_padded(G, mb->ind_int);
     const int64_t lp = local_ord(G, p);
+    // a cell on a fixed face (bc type 3; the first axis that has one, low face first) carries
+    // the boundary value itself: the functor of the reference's staggered tests
+    // (test_integration_Cartesian_Trilinos.cpp:155-220): Get<CENTER> = 1, GetRhs = value,
+    // SetInitialGuess(value), nothing else
+    for (int d = 0; d < G.dim; d++) {
+        int face = -1;
+        if (mb->ind_int.v[d] == 0 && t->bc_type[2 * d] == 3) face = 2 * d;
+        else if (mb->ind_int.v[d] == G.n[d] - 1 && t->bc_type[2 * d + 1] == 3) face = 2 * d + 1;
+        if (face < 0) continue;
+        for (int c = 0; c < G.N; c++) {
+            mb->get(c, 0) = 1.;
+            mb->rhs[c] = t->bc_value[face];
+            mb->x0[c] = t->bc_value[face];
+        }
+        return;
+    }
     if (t->react && G.N == 2) {                    // explicit cross-component entries first
         for (int c = 0; c < 2; c++) mb->resize(c, 1);
     }

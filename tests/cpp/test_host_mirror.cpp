@@ -13,8 +13,9 @@
 //                       staggered representations), plus their cell positions
 //   SolveStaggered1D  — test/test_integration_Cartesian_Trilinos.cpp:155-220
 //   SolveStaggered2D  — test/test_integration_Cartesian_Trilinos.cpp:410-585 (X and Y)
-//   DeviceStaggered   — the face gradient of a y-staggered linear field on the device: the
-//                       descriptor of a staggered representation keeps the grid's dx
+//   DeviceStaggered   — the face gradient of a y-staggered linear field on the device (the
+//                       descriptor of a staggered representation keeps the grid's dx), and a
+//                       device-path solve with fixed boundary cells (SolveStaggeredY's profile)
 //   SolveJacobi       — SolveLaplace with Ifpack2 RELAXATION/Jacobi as M
 //                       (TrilinosSolver.inl:97-175), and the device fast path with and without it
 // Exit code 0 when every case passes.
@@ -513,6 +514,30 @@ static void DeviceStaggered() {
     EXPECT(ex == 0. && ez == 0., "cross-axis gradients %g %g", ex, ez);
     EXPECT(ey < 1e-12, "y gradient deviates from 3 by %g", ey);
     std::printf("  DeviceStaggered: 16 x 13 x 10 y-staggered cells, |grad_y - 3| <= %.1e\n", ey);
+    // the device fast path on the staggered representation: steady diffusion with the boundary
+    // cells of the S/N faces fixed at 0 and 1 (identity rows), Neumann elsewhere -> phi linear
+    // in the staggered index, j / 12 (the expectation of SolveStaggeredY,
+    // test_integration_Cartesian_Trilinos.cpp:497-585)
+    Matrix::Terms terms;
+    terms.Diffusion(1.).Fixed(2, 0.).Fixed(3, 1.).Neumann(0).Neumann(1).Neumann(4).Neumann(5);
+    Matrix::Trilinos<double> msystem;
+    Matrix::TrilinosSolver<double> solver;
+    auto params = std::make_shared<ParameterList>();
+    params->set("Convergence Tolerance", 1e-13);
+    params->set("Maximum Iterations", 2000);
+    msystem.Build(rep, v, terms, false);
+    auto ret = solver.Solve(Matrix::SolverPackage::BumbleBee, "BICGSTAB2", msystem.GetA(), msystem.GetX(),
+                            msystem.GetB(), params);
+    EXPECT(ret == Matrix::ReturnType::Converged, "staggered device solve did not converge");
+    auto x = msystem.GetX()->getData();
+    double err = 0.;
+    for (int i = 0; i < ni[0]; i++)
+        for (int j = 0; j < ni[1]; j++)
+            for (int k = 0; k < ni[2]; k++)
+                err = std::fmax(err, std::fabs(x[(static_cast<std::size_t>(i) * ni[1] + j) * ni[2] + k] - j / 12.));
+    EXPECT(err < 1e-9, "staggered device solve: max deviation from j / 12 is %g", err);
+    std::printf("  DeviceStaggered: fixed S/N cells, %d passes, max deviation from the linear profile %.1e\n",
+                solver.GetNumIterations(), err);
 }
 
 static std::string g_vtk_dir;

@@ -125,7 +125,8 @@ def test_tvd_flux_and_step_match_oracle(ctx, rng, lim, case):
     assert bits_equal(host(out), exp_step)
 
 
-ASSEMBLY = ["diff_dirichlet", "var_conv_src", "periodic_conv", "react2", "neumann_2d", "steady_1d"]
+ASSEMBLY = ["diff_dirichlet", "var_conv_src", "periodic_conv", "react2", "neumann_2d", "steady_1d", "fixed_x",
+            "fixed_react2"]
 
 
 def assembly_case(rng, name):
@@ -149,6 +150,14 @@ def assembly_case(rng, name):
         go = orc.grid([7, 8, 9], ghost=2, ncomp=2)
         kw = dict(dt=0.1, diff_comp=[1e-3, 5e-3], react=True, bc_type=[1] * 6,
                   bc_value=[0.0] * 6)
+    elif name == "fixed_x":           # fixed boundary cells (identity rows) on W/E, one cell longer in x
+        go = orc.grid([10, 9, 8], size=[10 / 9, 1, 1], ghost=2)
+        kw = dict(dt=5e-3, diff_cell=True, limiter=4, source=True, bc_type=[3, 3, 1, 2, 1, 1],
+                  bc_value=[0.5, -0.25, 0.1, 0, 0.2, 0.3])
+    elif name == "fixed_react2":      # fixed faces drop the cross-component entry too
+        go = orc.grid([7, 8, 9], ghost=2, ncomp=2)
+        kw = dict(dt=0.1, diff_comp=[1e-3, 5e-3], react=True, bc_type=[1, 1, 3, 1, 1, 3],
+                  bc_value=[0.0, 0.0, 0.75, 0.0, 0.0, -0.5])
     elif name == "neumann_2d":
         go = orc.grid([13, 11], ghost=2, ncomp=3)
         kw = dict(dt=0.2, diff=0.7, limiter=2, bc_type=[2, 1, 1, 2], bc_value=[0, 1.0, 0.5, 0])
@@ -281,7 +290,8 @@ def test_mixed_refinement_reaches_fp64_residual(ctx, rng, name):
     assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
 
 
-DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "periodic_even", "wide_dirichlet", "react2"]
+DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "periodic_even", "wide_dirichlet", "react2",
+             "fixed_x", "fixed_react2"]
 
 
 def device_assembly(ctx, rng, name):

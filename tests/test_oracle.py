@@ -465,6 +465,34 @@ def test_bicgstab2_3d_linear_profile_term_list():
     assert np.max(np.abs(x - (i + 0.5) / 20)) < 1e-8
 
 
+def test_fixed_faces_staggered_linear_profile():
+    """Fixed boundary cells (bc type 3, identity rows) on the x faces of a 2-D grid one cell
+    longer along x, as a staggered representation is (CartesianRepresentation.inl:65-77),
+    with the reference's staggered solve as the expectation
+    (test_integration_Cartesian_Trilinos.cpp:410-496 SolveStaggeredX): phi = i / (n_x - 1)
+    exactly, for N = 3 components; Neumann on the y faces. The fixed rows are identity rows
+    (one entry, 1) with rhs = x0 = the face value, and rows next to them keep their neighbour
+    in the fixed cell."""
+    nx, ny, N = 21, 21, 3
+    g = orc.grid([nx, ny], size=[21 / 20, 2.0], ghost=2, ncomp=N)
+    t = orc.terms(dt=0.0, diff=1.0, bc_type=[3, 3, 2, 2], bc_value=[0.0, 1.0, 0, 0])
+    phi = np.full((nx + 4) * (ny + 4) * N, 0.25)
+    rp, col, val, rhs, x0 = orc.assemble(g, t, phi)
+    i = np.repeat(np.arange(nx * ny) // ny, N)
+    edge = (i == 0) | (i == nx - 1)
+    rows = np.arange(len(rhs))
+    assert np.all(np.diff(rp)[edge] == 1) and np.all(col[rp[:-1][edge]] == rows[edge])
+    assert np.all(val[rp[:-1][edge]] == 1.0)
+    assert np.array_equal(rhs[edge], np.where(i[edge] == 0, 0.0, 1.0)) and np.array_equal(x0[edge], rhs[edge])
+    assert np.all(x0[~edge] == 0.25)
+    # the first interior row still couples to the fixed cell (W neighbour present)
+    r1 = N * ny                      # cell (1, 0), component 0
+    assert (r1 - N * ny) in col[rp[r1]:rp[r1 + 1]]
+    x, ok, it, res, _ = orc.bicgstab2(rp, col, val, rhs, x0, 1e-14, 5000)
+    assert ok
+    assert np.max(np.abs(x - i / (nx - 1))) < 1e-8
+
+
 def _random_problem(rng, n=(5, 6, 7), ncomp=1, conv=True, react=False, periodic=False):
     g = orc.grid(list(n), size=[1.0, 1.2, 0.9], ghost=2, ncomp=ncomp,
                  periodic=[1, 1, 1] if periodic else None)
