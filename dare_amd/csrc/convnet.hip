@@ -1053,27 +1053,24 @@ long long level_voxels(const int* d, int level) {
 }
 
 // compute units of the current device and resident 256-thread blocks of a kernel (queried
-// once per instantiation)
+// once per instantiation; function-local statics are initialised once, thread-safely)
 inline int device_cus() {
-    static int n = 0;
-    if (n == 0) {
+    static const int n = [] {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-            n = v;
-        else
-            n = 256;
-    }
+            return v;
+        return 256;
+    }();
     return n;
 }
 
 template <auto K>   // one cached value per kernel (the kernels share one function-pointer type)
 int resident_blocks() {
-    static int n = 0;
-    if (n == 0) {
+    static const int n = [] {
         int v = 0;
-        n = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, K, kNetThreads, 0) == hipSuccess && v > 0) ? v : 1;
-    }
+        return (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, K, kNetThreads, 0) == hipSuccess && v > 0) ? v : 1;
+    }();
     return n;
 }
 
