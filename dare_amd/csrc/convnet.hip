@@ -37,12 +37,13 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNetThreads = 256;
 constexpr int kLayers = 11;
-constexpr int kInChannels = 16;
+constexpr int kInChannels = 8;     // network input: 3 solver channels + up to 5 static fields
+constexpr int kOutWidth = 16;       // channels of the output layer's tensor (channel 0 is used)
 
 struct LayerDef {
     int cin, cout, level;    // level 0 full, 1 half, 2 quarter resolution
 };
-constexpr LayerDef kDefs[kLayers] = {{16, 16, 0}, {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2}, {64, 64, 2},
+constexpr LayerDef kDefs[kLayers] = {{kInChannels, 16, 0}, {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2}, {64, 64, 2},
                                      {64, 32, 1}, {32, 32, 1}, {32, 16, 0}, {16, 16, 0}, {16, 16, 0}};
 
 // FP32 -> BF16, round to nearest even; NaN stays NaN.
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
                                                         uint16_t* __restrict__ Y, float* __restrict__ Yf, int D, int H,
                                                         int W, int ZB, const uint16_t* __restrict__ E,
                                                         uint16_t* __restrict__ Y2) {
-    static_assert(CIN == 16 || CIN == 32, "ring of full-channel planes: 16 or 32 input channels");
+    static_assert(CIN == 8 || CIN == 16 || CIN == 32, "ring of full-channel planes: 8, 16 or 32 input channels");
     constexpr int SX = 16 * NSEG + 2, SY = 6, NPL = SX * SY;    // voxels of one staged plane
     constexpr int NPS = (NPL + 15) / 16 * 16;                    // LDS stride of one 8-channel piece
     constexpr int CH8 = CIN / 8;                                 // 16-byte pieces per voxel
@@ -255,7 +256,8 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
     constexpr int PPL = CH8 * NPL;                               // pieces of one plane
     constexpr int SLOT = CH8 * NPS;                              // LDS pieces of one ring slot
     constexpr int RPL = (PPL + kNetThreads - 1) / kNetThreads;   // staged pieces per thread
-    constexpr int NSTEP = (CIN / 16) * 14;                       // K = 32 steps
+    // K = 32 steps: two taps x 16 channels (CIN 16, 32), or four taps x 8 channels (CIN 8)
+    constexpr int NSTEP = CIN == 8 ? 7 : (CIN / 16) * 14;
     // piece strides of 256 bytes multiples (NPS): the lane groups of one operand read that
     // differ only in the channel piece start on the same bank, free of bank conflicts
     __shared__ uint4 ring[3 * SLOT];                             // [slot][piece8][voxel]
@@ -332,8 +334,8 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_zm(const uint16_t* __restr
             // q / 14, taps 2 (q % 14) and 2 (q % 14) + 1
             bf16x8 wa{}, xb[NSEG];
             auto fetch = [&](int q, bf16x8& w, bf16x8* x) {
-                const int p8 = 2 * (q / 14) + half;
-                const int tap = 2 * (q % 14) + (kq >> 1);
+                const int p8 = CIN == 8 ? 0 : 2 * (q / 14) + half;
+                const int tap = CIN == 8 ? 4 * q + kq : 2 * (q % 14) + (kq >> 1);
                 const bool tap_ok = tap < 27;
                 const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
                 if constexpr (!WREG)
@@ -617,9 +619,10 @@ __global__ __launch_bounds__(kNetThreads) void k_wgrad(const uint16_t* __restric
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int chunk = blockIdx.x * 4 + wave;
     if (chunk >= nchunks) return;
-    constexpr int CB = CIN / 16;
+    constexpr int CB = CIN >= 16 ? CIN / 16 : 1;                // an 8-channel input is one half block
     const int cob = blockIdx.y / CB, cib = blockIdx.y % CB;
     const int i = lane & 15, kq = lane >> 4;
+    const bool ci_ok = cib * 16 + i < CIN;
     floatx4 acc[27];
 #pragma unroll
     for (int t = 0; t < 27; t++) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -652,7 +655,7 @@ __global__ __launch_bounds__(kNetThreads) void k_wgrad(const uint16_t* __restric
                         for (int e = 0; e < 4; e++) {
                             const int xv = xs + kq * 4 + e;
                             const int xx = xv + dx;
-                            const bool ok = xv < W && xx >= 0 && xx < W;
+                            const bool ok = ci_ok && xv < W && xx >= 0 && xx < W;
                             b[e] = ok ? static_cast<short>(X[(srow + xx) * CIN + cib * 16 + i]) : short(0);
                         }
                         const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
@@ -662,6 +665,7 @@ __global__ __launch_bounds__(kNetThreads) void k_wgrad(const uint16_t* __restric
             }
         }
     }
+    if (!ci_ok) return;
 #pragma unroll
     for (int t = 0; t < 27; t++)
 #pragma unroll
@@ -942,8 +946,8 @@ __global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* _
         const double t = (xt[r] - base[r]) * inv;
         const double d = static_cast<double>(out[q]) - t;
         acc += d * d;
-        dout[q * kInChannels] = f2bf(static_cast<float>(2. * d / static_cast<double>(n)));
-        for (int c = 1; c < kInChannels; c++) dout[q * kInChannels + c] = 0;
+        dout[q * kOutWidth] = f2bf(static_cast<float>(2. * d / static_cast<double>(n)));
+        for (int c = 1; c < kOutWidth; c++) dout[q * kOutWidth + c] = 0;
     }
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -1080,15 +1084,15 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
     const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
     const dim3 blk(kNetThreads);
     const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / (16 * conv_nbb<COUT>()));
-    if constexpr (CIN == 16 || CIN == 32) {
+    if constexpr (CIN == 8 || CIN == 16 || CIN == 32) {
         // z-marching form: the planes per block are chosen so that the whole grid is resident at
         // once (no tail round): z-blocks = resident blocks / (x, y, channel-group) columns. 32-
         // channel rings use tiles of at most 32 voxels along x (registers).
-        const int nseg = (W >= 64 && CIN == 16) ? 4 : (W >= 32 ? 2 : 1);
+        const int nseg = (W >= 64 && CIN <= 16) ? 4 : (W >= 32 ? 2 : 1);
         const unsigned gx = static_cast<unsigned>((W + 16 * nseg - 1) / (16 * nseg));
         const unsigned gzm = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
         int per_cu = 1;
-        if (nseg == 4) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, (CIN == 16 ? 4 : 2), MODE>>();
+        if (nseg == 4) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, (CIN <= 16 ? 4 : 2), MODE>>();
         else if (nseg == 2) per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 2, MODE>>();
         else per_cu = resident_blocks<k_conv_zm<CIN, COUT, RELU, OUT, 1, MODE>>();
         const long long cols = static_cast<long long>(gx) * gzm;
@@ -1096,7 +1100,7 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
         const int zb = static_cast<int>((D + nzb - 1) / nzb);
         const dim3 grid(gx, gzm, static_cast<unsigned>((D + zb - 1) / zb));
         if (nseg == 4)
-            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, (CIN == 16 ? 4 : 2), MODE>), grid, blk, 0, sm, X, Wt, Y,
+            hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, (CIN <= 16 ? 4 : 2), MODE>), grid, blk, 0, sm, X, Wt, Y,
                                Yf, D, H, W, zb, E, Y2);
         else if (nseg == 2)
             hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 2, MODE>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb,
@@ -1104,17 +1108,17 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
         else
             hipLaunchKernelGGL((k_conv_zm<CIN, COUT, RELU, OUT, 1, MODE>), grid, blk, 0, sm, X, Wt, Y, Yf, D, H, W, zb,
                                E, Y2);
-        return;
-    }
-    if (W >= 64) {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 4, MODE>), dim3((W + 63) / 64, gy, D), blk, 0, sm, X, Wt, Y,
-                           Yf, D, H, W, E, Y2);
-    } else if (W >= 32) {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 2, MODE>), dim3((W + 31) / 32, gy, D), blk, 0, sm, X, Wt, Y,
-                           Yf, D, H, W, E, Y2);
     } else {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1, MODE>), dim3((W + 15) / 16, gy, D), blk, 0, sm, X, Wt, Y,
-                           Yf, D, H, W, E, Y2);
+        if (W >= 64) {
+            hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 4, MODE>), dim3((W + 63) / 64, gy, D), blk, 0, sm, X, Wt,
+                               Y, Yf, D, H, W, E, Y2);
+        } else if (W >= 32) {
+            hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 2, MODE>), dim3((W + 31) / 32, gy, D), blk, 0, sm, X, Wt,
+                               Y, Yf, D, H, W, E, Y2);
+        } else {
+            hipLaunchKernelGGL((k_conv<CIN, COUT, RELU, OUT, 1, MODE>), dim3((W + 15) / 16, gy, D), blk, 0, sm, X, Wt,
+                               Y, Yf, D, H, W, E, Y2);
+        }
     }
 }
 
@@ -1172,7 +1176,7 @@ int wgrad_any(dare_convnet* net, hipStream_t sm, int layer, const uint16_t* X, c
     const int cin = kDefs[layer].cin, cout = kDefs[layer].cout, level = kDefs[layer].level;
     const int D = net->dims[0] >> level, H = net->dims[1] >> level, W = net->dims[2] >> level;
     const int rows = D * H;
-    const int pairs = (cin / 16) * (cout / 16);
+    const int pairs = std::max(1, cin / 16) * (cout / 16);
     long long cap = std::min<long long>(512, kWpartFloats / (27LL * cin * cout));
     cap = std::min<long long>(cap, std::max(1, 4096 / pairs));
     const int chunks0 = static_cast<int>(std::max<long long>(1, std::min<long long>(rows, cap)));
@@ -1183,6 +1187,7 @@ int wgrad_any(dare_convnet* net, hipStream_t sm, int layer, const uint16_t* X, c
     if (cin == CI && cout == CO)                                                                          \
         hipLaunchKernelGGL((k_wgrad<CI, CO>), grid, dim3(kNetThreads), 0, sm, X, G, D, H, W, rpc, nchunks, \
                            net->wpart);
+    DARE_WG_CASE(kInChannels, 16)
     DARE_WG_CASE(16, 16)
     DARE_WG_CASE(16, 32)
     DARE_WG_CASE(32, 16)
@@ -1235,7 +1240,7 @@ extern "C" {
 
 int dare_convnet_create(dare_ctx* ctx, int32_t in_channels, const int32_t* max_dims, dare_convnet** out) {
     DARE_CHECK_ARG(ctx && max_dims && out, "NULL argument");
-    DARE_CHECK_ARG(in_channels >= 3 && in_channels <= kInChannels, "in_channels must be in [3, 16]");
+    DARE_CHECK_ARG(in_channels >= 3 && in_channels <= kInChannels, "in_channels must be in [3, 8]");
     for (int d = 0; d < 3; d++) DARE_CHECK_ARG(max_dims[d] >= 4 && max_dims[d] % 4 == 0, "dims must be multiples of 4");
     *out = nullptr;
     DARE_HIP_TRY(hipSetDevice(ctx->device));
@@ -1260,7 +1265,7 @@ int dare_convnet_create(dare_ctx* ctx, int32_t in_channels, const int32_t* max_d
     A(&net->wb, net->nparams);
     A(&net->ws6, 64LL * kDefs[6].cin * kDefs[6].cout);
     A(&net->ws8, 64LL * kDefs[8].cin * kDefs[8].cout);
-    A(&net->x0, V0 * 16);
+    A(&net->x0, V0 * kInChannels);
     A(&net->a1, V0 * 16);
     A(&net->e1, V0 * 16);
     A(&net->p1, V1 * 16);
@@ -1398,7 +1403,7 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
     const long long V0 = level_voxels(d, 0);
     const uint16_t* W = net->wf;
     const long long* o = net->off;
-    conv<16, 16, true, 0>(sm, net->x0, W + o[0], net->a1, nullptr, d, 0);
+    conv<kInChannels, 16, true, 0>(sm, net->x0, W + o[0], net->a1, nullptr, d, 0);
     conv<16, 16, true, 0>(sm, net->a1, W + o[1], net->e1, nullptr, d, 0);
     hipLaunchKernelGGL(k_pool, dim3(ew_grid(V1 * 16)), dim3(kNetThreads), 0, sm, net->e1, net->p1, 16, d[0] >> 1,
                        d[1] >> 1, d[2] >> 1);

@@ -17,7 +17,7 @@ import numpy as np
 
 from . import DareError, _ptr, check, lib
 
-LAYERS = [(16, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+LAYERS = [(8, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
           (64, 32, 1), (32, 32, 1), (32, 16, 0), (16, 16, 0), (16, 16, 0)]
 
 

@@ -256,7 +256,8 @@ int dare_convnet_get_params(dare_ctx* ctx, const dare_convnet* net, float* host_
  * (dt / V for the implicit Euler term) maps to field units. Channels: phi^n / max|phi^n|,
  * (phi^n - phi^{n-1}) / max|.|, resid * resid_scale / S with S = max|resid * resid_scale|,
  * then the extras. The guess (unpack) is x_base + S * out; the loss compares out with
- * (x_true - x_base) / S. */
+ * (x_true - x_base) / S. The input tensor has 8 BF16 channels (at most 5 extras); layer 0 maps
+ * them to 16. */
 int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi, const double* phi_prev,
                       const double* resid, double resid_scale, int32_t nextra, const double* const* extras,
                       const double* extra_scale, const double* extra_offset, const int32_t* crop);

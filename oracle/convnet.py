@@ -15,7 +15,7 @@ GPU kernels are checked against this oracle.
 import numpy as np
 
 # (cin, cout, level): level 0 full resolution, 1 half, 2 quarter
-LAYERS = [(16, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+LAYERS = [(8, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
           (64, 32, 1), (32, 32, 1), (32, 16, 0), (16, 16, 0), (16, 16, 0)]
 
 
@@ -142,7 +142,7 @@ def conv_up(x, w, rnd=lambda a: a):
 
 
 def forward(x0, weights, exact=False):
-    """x0: [D, H, W, 16]. Returns (out [D, H, W], activations dict)."""
+    """x0: [D, H, W, 8]. Returns (out [D, H, W], activations dict)."""
     r = (lambda a: a) if exact else bf16
     W = weights
     A = {"x0": x0}
@@ -217,11 +217,11 @@ def scales(phi_int, phim_int, resid_int, resid_scale):
     return s0, s1, s2
 
 
-def pack(phi_int, phim_int, resid_int, resid_scale, extras=(), in_ch=16):
-    """Input tensor [D, H, W, 16] as BF16 values (arrays of shape [D, H, W] over the box);
+def pack(phi_int, phim_int, resid_int, resid_scale, extras=(), in_ch=8):
+    """Input tensor [D, H, W, 8] as BF16 values (arrays of shape [D, H, W] over the box);
     resid = b - A phi^n, resid_scale maps it to field units."""
     s0, s1, s2 = scales(phi_int, phim_int, resid_int, resid_scale)
-    x = np.zeros(phi_int.shape + (16,))
+    x = np.zeros(phi_int.shape + (in_ch,))
     x[..., 0] = bf16(phi_int * (1.0 / s0))
     x[..., 1] = bf16((phi_int - phim_int) * (1.0 / s1))
     x[..., 2] = bf16(resid_int * resid_scale * (1.0 / s2))

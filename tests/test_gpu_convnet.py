@@ -64,7 +64,7 @@ def setup(ctx, rng, in_ch=4):
 def test_pack_bit_exact(ctx, rng):
     net, g, dev, (phi, phim, rhs, ext), w, _ = setup(ctx, rng)
     ptr, scales = net.tensors()
-    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [8])
     exp, s = cn.pack(phi, phim, rhs.reshape(N), 0.7, extras=[(ext, 2.0, 0.5)])
     assert np.allclose(scales, s, rtol=0, atol=0)
     assert np.array_equal(x0, exp)
@@ -80,7 +80,7 @@ def test_pack_crop_matches_oracle(ctx, rng):
     o = [4, 2, 3]
     net.pack(g, dev[0], dev[1], dev[2], 0.7, crop=o + N)
     ptr, scales = net.tensors()
-    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [8])
     sl = tuple(slice(o[d], o[d] + N[d]) for d in range(3))
     exp, s = cn.pack(phi[sl], phim[sl], rhs.reshape(big)[sl], 0.7)
     assert np.allclose(scales, s, rtol=0, atol=0)
@@ -93,7 +93,7 @@ def test_forward_matches_oracle(ctx, rng):
     net.forward()
     ctx.synchronize()
     ptr, _ = net.tensors()
-    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [8])
     out_ref, A = cn.forward(x0, cn.split(cn.bf16(w)), exact=False)
     shapes = {"a1": (0, 16), "e1": (0, 16), "p1": (1, 16), "a3": (1, 32), "e2": (1, 32), "p2": (2, 32),
               "a5": (2, 64), "e3": (2, 64), "a7": (1, 32), "s2": (1, 32), "d2": (1, 32), "a9": (0, 16),
@@ -121,7 +121,7 @@ def test_forward_wide_box_matches_oracle(ctx, rng):
     net.forward()
     ctx.synchronize()
     ptr, _ = net.tensors()
-    x0 = d2h_bf16(ctx, ptr["x0"], nb + [16])
+    x0 = d2h_bf16(ctx, ptr["x0"], nb + [8])
     out_ref, A = cn.forward(x0, cn.split(cn.bf16(w)), exact=False)
     shapes = {"e1": (0, 16), "e2": (1, 32), "e3": (2, 64), "s2": (1, 32), "s1": (0, 16), "d1": (0, 16)}
     for name, (lv, c) in shapes.items():
@@ -142,7 +142,7 @@ def test_backward_matches_oracle(ctx, rng):
     net.backward()
     grads = net.grads()
     ptr, scales = net.tensors()
-    x0 = d2h_bf16(ctx, ptr["x0"], N + [16])
+    x0 = d2h_bf16(ctx, ptr["x0"], N + [8])
     Wb = cn.split(cn.bf16(w))
     out_ref, A = cn.forward(x0, Wb, exact=False)
     out_gpu = d2h_f32(ctx, ptr["out"], N)

@@ -67,7 +67,7 @@ def test_subpixel_conv_equals_conv_of_upsampling(rng):
 def test_oracle_gradients_match_finite_differences():
     rng = np.random.default_rng(2)
     W = _weights(rng)
-    x0 = np.zeros((4, 4, 8, 16))
+    x0 = np.zeros((4, 4, 8, 8))
     x0[..., :3] = rng.normal(size=(4, 4, 8, 3))
     t = rng.normal(size=(4, 4, 8))
 
