@@ -43,6 +43,10 @@ CASES = [
     # two k tiles (one partial), j not a multiple of 4: the marching 3-D kernels' column pieces
     # then start and end inside columns and cross column boundaries
     dict(n=[20, 70, 67], ncomp=1, periodic=[0, 1, 0]),
+    # the same with an even k extent (the gradient's two-cells-per-lane form, two k tiles, one
+    # partial) and an odd i extent: pieces of odd and even length, two planes per step plus
+    # the remainder plane
+    dict(n=[21, 70, 130], ncomp=1, periodic=[0, 1, 0]),
 ]
 
 
