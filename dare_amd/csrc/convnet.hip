@@ -588,7 +588,7 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
                     uint2 pk;
                     pk.x = static_cast<uint32_t>(b[0]) | (static_cast<uint32_t>(b[1]) << 16);
                     pk.y = static_cast<uint32_t>(b[2]) | (static_cast<uint32_t>(b[3]) << 16);
-                    *reinterpret_cast<uint2*>(Y + at) = pk;
+                    if (Y) *reinterpret_cast<uint2*>(Y + at) = pk;     // NULL: inference, sum only
                     if constexpr (ADD) {
                         const uint2 e2 = ev[rx][s];
                         const uint16_t e[4] = {static_cast<uint16_t>(e2.x & 0xffffu), static_cast<uint16_t>(e2.x >> 16),
@@ -1020,6 +1020,8 @@ struct dare_convnet {
     uint16_t *wf = nullptr, *wb = nullptr;
     uint16_t *ws6 = nullptr, *ws8 = nullptr;   // folded sub-pixel weights of layers 6 and 8
     int adam_t = 0;
+    bool inference = false;    // forward skips a7, a9 (read only by the backward pass)
+    bool acts_for_bwd = false; // the last forward pass wrote every activation bwd reads
     // forward activations
     uint16_t *x0 = nullptr, *a1 = nullptr, *e1 = nullptr, *p1 = nullptr, *a3 = nullptr, *e2 = nullptr, *p2 = nullptr,
              *a5 = nullptr, *e3 = nullptr, *u2 = nullptr, *a7 = nullptr, *s2 = nullptr, *d2 = nullptr, *u1 = nullptr,
@@ -1123,7 +1125,8 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
 }
 
 // decoder layer: ReLU(conv(upsample(X))) and the skip sum Y2 = Y + E, in sub-pixel form (the
-// 27 taps folded into 8 per output parity, Ws refreshed from the current BF16 weights first)
+// 27 taps folded into 8 per output parity, Ws refreshed from the current BF16 weights first).
+// Y may be NULL (inference: only the sum is written).
 template <int CIN, int COUT>
 void conv_up(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Ws, uint16_t* Y, const int* d, int level,
              const uint16_t* E, uint16_t* Y2) {
@@ -1415,12 +1418,20 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
     conv<64, 64, true, 0>(sm, net->a5, W + o[5], net->e3, nullptr, d, 2);
     // decoder: the upsampling is read inside the conv (u2, u1 are written by the backward pass,
     // which needs them) and the skip sums s2 = a7 + e2, s1 = a9 + e1 come out of its epilogue
-    conv_up<64, 32>(sm, net->e3, W + o[6], net->ws6, net->a7, d, 1, net->e2, net->s2);
+    // in inference mode the ReLU outputs a7, a9 (the backward pass's masks) are not stored
+    conv_up<64, 32>(sm, net->e3, W + o[6], net->ws6, net->inference ? nullptr : net->a7, d, 1, net->e2, net->s2);
     conv<32, 32, true, 0>(sm, net->s2, W + o[7], net->d2, nullptr, d, 1);
-    conv_up<32, 16>(sm, net->d2, W + o[8], net->ws8, net->a9, d, 0, net->e1, net->s1);
+    conv_up<32, 16>(sm, net->d2, W + o[8], net->ws8, net->inference ? nullptr : net->a9, d, 0, net->e1, net->s1);
     conv<16, 16, true, 0>(sm, net->s1, W + o[9], net->d1, nullptr, d, 0);
     conv<16, 16, false, 1>(sm, net->d1, W + o[10], nullptr, net->out, d, 0);
     DARE_LAUNCH_CHECK();
+    net->acts_for_bwd = !net->inference;
+    return DARE_OK;
+}
+
+int dare_convnet_set_inference(dare_convnet* net, int32_t on) {
+    DARE_CHECK_ARG(net, "NULL argument");
+    net->inference = on != 0;
     return DARE_OK;
 }
 
@@ -1470,6 +1481,7 @@ int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
 // Back-propagation of the last loss through the last forward pass: weight gradients (FP32).
 int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net) {
     DARE_CHECK_ARG(ctx && net && net->dout, "call dare_convnet_loss first");
+    DARE_CHECK_ARG(net->acts_for_bwd, "backward needs a forward pass outside inference mode");
     if (!net->bwd_ready) {
         const long long V0 = net->max_v0, V1 = V0 / 8, V2 = V0 / 64;
         int rc = DARE_OK;

@@ -36,6 +36,7 @@ def _sig():
         "dare_convnet_get_params": (i32, [vp, vp, vp, vp]),
         "dare_convnet_pack": (i32, [vp, vp, G, vp, vp, vp, dbl, i32, ctypes.POINTER(vp), vp, vp, vp]),
         "dare_conv_net_fwd": (i32, [vp, vp]),
+        "dare_convnet_set_inference": (i32, [vp, i32]),
         "dare_convnet_unpack": (i32, [vp, vp, G, vp, vp, vp]),
         "dare_convnet_loss": (i32, [vp, vp, G, vp, vp, ctypes.POINTER(dbl), vp]),
         "dare_conv_net_bwd": (i32, [vp, vp]),
@@ -73,6 +74,7 @@ class WarmStartNet:
         self.ctx = ctx
         self.in_channels = in_channels
         self.h = ctypes.c_void_p()
+        self._inference = False
         dims = (ctypes.c_int32 * 3)(*max_dims)
         check(L.dare_convnet_create(ctx.h, in_channels, dims, ctypes.byref(self.h)), "dare_convnet_create")
         n = ctypes.c_int64()
@@ -123,7 +125,11 @@ class WarmStartNet:
                                        float(resid_scale), ne, fptr, sc.ctypes.data, of.ctypes.data,
                                        self._crop(crop)), "dare_convnet_pack")
 
-    def forward(self):
+    def forward(self, inference=False):
+        """inference=True skips the decoder activations that only backward() reads."""
+        if inference != self._inference:
+            check(_sig().dare_convnet_set_inference(self.h, int(inference)), "dare_convnet_set_inference")
+            self._inference = inference
         check(_sig().dare_conv_net_fwd(self.ctx.h, self.h), "dare_conv_net_fwd")
 
     def unpack(self, grid, x_base, x_out, crop=None):
@@ -133,7 +139,7 @@ class WarmStartNet:
     def predict(self, grid, phi, phi_prev, x_base, resid, resid_scale, x_out, extras=()):
         """x_out (solver rows) = x_base + the network's correction; resid = b - A x_base."""
         self.pack(grid, phi, phi_prev, resid, resid_scale, extras)
-        self.forward()
+        self.forward(inference=True)
         self.unpack(grid, x_base, x_out)
 
     def loss(self, grid, x_base, x_true, readback=True, crop=None):

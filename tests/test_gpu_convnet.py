@@ -132,6 +132,50 @@ def test_forward_wide_box_matches_oracle(ctx, rng):
     net.close()
 
 
+def test_inference_mode_skips_only_the_backward_masks(ctx, rng):
+    """Inference mode: bit-identical output and skip sums, a7 / a9 left untouched (a sentinel
+    survives), and the backward pass refused until a forward pass with the mode off."""
+    import torch
+    nb = [16, 64, 256]                      # both sub-pixel tilings of the decoder convs
+    net = WarmStartNet(ctx, nb, in_channels=3, seed=5)
+    g = da.Grid.make(nb, size=[1, 1, 1], ghost=2)
+    dev, _ = fields(rng, nb)
+    net.pack(g, dev[0], dev[1], dev[2], 0.7)
+    net.forward()
+    ctx.synchronize()
+    ptr, _ = net.tensors()
+    sizes = {"a7": int(np.prod(nb)) // 8 * 32, "a9": int(np.prod(nb)) * 16}
+
+    def raw(name, n):
+        u = np.empty(n, np.uint16)
+        da.check(da.lib().dare_memcpy_d2h(ctx.h, u.ctypes.data, ptr[name], 2 * n), "d2h")
+        return u
+    ref = {k: raw(k, n) for k, n in [("s2", sizes["a7"]), ("s1", sizes["a9"]), ("a9", sizes["a9"])]}
+    ref_out = raw("out", 2 * int(np.prod(nb)))
+    for name, n in sizes.items():
+        sentinel = np.full(n, 0x7fc1, np.uint16)
+        da.check(da.lib().dare_memcpy_h2d(ctx.h, ptr[name], sentinel.ctypes.data, 2 * n), "h2d")
+    net.forward(inference=True)
+    ctx.synchronize()
+    assert np.array_equal(raw("out", 2 * int(np.prod(nb))), ref_out)
+    assert np.array_equal(raw("s2", sizes["a7"]), ref["s2"])
+    assert np.array_equal(raw("s1", sizes["a9"]), ref["s1"])
+    for name, n in sizes.items():
+        assert np.all(raw(name, n) == 0x7fc1), name
+    x_true = torch.from_numpy(rng.normal(size=int(np.prod(nb)))).cuda()
+    base = dev[0].view(*[m + 4 for m in nb])[2:-2, 2:-2, 2:-2].contiguous().view(-1)
+    net.loss(g, base, x_true)
+    with pytest.raises(da.DareError) as e:
+        net.backward()
+    assert "outside inference mode" in str(e.value)
+    net.forward()
+    ctx.synchronize()
+    assert np.array_equal(raw("a9", sizes["a9"]), ref["a9"])
+    net.loss(g, base, x_true)
+    net.backward()
+    net.close()
+
+
 def test_backward_matches_oracle(ctx, rng):
     import torch
     net, g, dev, (phi, phim, rhs, ext), w, _ = setup(ctx, rng)
