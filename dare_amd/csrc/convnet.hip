@@ -511,13 +511,11 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
         }
     }
     __syncthreads();
-    for (int k = k0; k < k1; k++) {
-        const int z = 2 * k + rz;
-        uint4 nxt[RPL];
-        if (k + 1 < k1) load_plane(k + 1 + rz, nxt);            // block-uniform condition
-        const long long vrow = (static_cast<long long>(z) * H + (y < H ? y : H - 1)) * W;
-        uint2 ev[2][NSEG];
+    // E of output plane z = 2k + rz (the skip operand), loaded before the MFMAs that precede
+    // its use
+    auto load_e = [&](int k, uint2 (&ev)[2][NSEG]) {
         if constexpr (ADD) {
+            const long long vrow = (static_cast<long long>(2 * k + rz) * H + (y < H ? y : H - 1)) * W;
 #pragma unroll
             for (int s = 0; s < NSEG; s++)
 #pragma unroll
@@ -528,6 +526,11 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
                         : uint2{0u, 0u};
                 }
         }
+    };
+    // output plane z = 2k + rz from the ring's coarse planes k - 1 + rz and k + rz
+    auto step = [&](int k, const uint2 (&ev)[2][NSEG]) {
+        const int z = 2 * k + rz;
+        const long long vrow = (static_cast<long long>(z) * H + (y < H ? y : H - 1)) * W;
         if (y < H) {                                             // wave-uniform
             floatx4 acc[2][NSEG];
 #pragma unroll
@@ -603,8 +606,27 @@ __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __rest
                     }
                 }
         }
-        // the slot of coarse plane k + 1 + rz held plane k - 2 + rz, last read in step k - 1
-        if (k + 1 < k1) store_plane(k + 1 + rz, nxt);
+    };
+    // Two steps per loop iteration with two register sets (pa/ea, pb/eb): the coarse plane a
+    // step stores into the ring and the E operand of the next step are loaded one step ahead,
+    // so their latency overlaps a whole step, and no register copy has to wait for them. The
+    // slot of coarse plane k + 1 + rz, written at the end of step k, held plane k - 2 + rz,
+    // last read in step k - 1. Every condition below is block-uniform.
+    uint4 pa[RPL], pb[RPL];
+    uint2 ea[2][NSEG], eb[2][NSEG];
+    if (k0 + 1 < k1) load_plane(k0 + 1 + rz, pa);
+    load_e(k0, ea);
+    for (int k = k0; k < k1; k += 2) {
+        if (k + 2 < k1) load_plane(k + 2 + rz, pb);
+        if (k + 1 < k1) load_e(k + 1, eb);
+        step(k, ea);
+        if (k + 1 < k1) store_plane(k + 1 + rz, pa);
+        __syncthreads();
+        if (k + 1 >= k1) break;
+        if (k + 3 < k1) load_plane(k + 3 + rz, pa);
+        if (k + 2 < k1) load_e(k + 2, ea);
+        step(k + 1, eb);
+        if (k + 2 < k1) store_plane(k + 2 + rz, pb);
         __syncthreads();
     }
 }
