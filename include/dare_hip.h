@@ -275,7 +275,8 @@ int dare_conv_net_bwd(dare_ctx* ctx, dare_convnet* net);
 int dare_conv_net_step(dare_ctx* ctx, dare_convnet* net, float lr, float beta1, float beta2, float eps);
 /* Device pointers of x0, out, dout and the activations (a1 e1 p1 a3 e2 p2 a5 e3 u2 a7 s2 d2 u1
  * a9 s1 d1) and the three input scales. The upsampled u2, u1 are only read inside the decoder
- * convs by the forward pass; the backward pass writes them. */
+ * convs by the forward pass; the backward pass writes them. a7 and a9 keep the values of the
+ * last forward pass outside inference mode. */
 int dare_convnet_tensors(const dare_convnet* net, void** ptrs, double* scales_host);
 
 #ifdef __cplusplus
