@@ -443,11 +443,12 @@ __global__ __launch_bounds__(kNetThreads) void k_sub_weights(const uint16_t* __r
 // fragments once (2 x-parities x 8 coarse taps x CIN/32 slices, kept in registers). Fine plane
 // 2k + rz reads coarse planes k - 1 + rz and k + rz, so the march keeps the coarse planes in a
 // ring of three LDS slots (4 rows x (16 NSEG + 2) columns each, zero outside the grid, stored
-// as [8-channel piece][voxel] with a 256-byte multiple piece stride): while plane k is
-// computed, coarse plane k + 1 + rz is loaded into registers, then written into the slot
-// nobody reads in step k; one barrier per step. A K = 32 step is one coarse tap (oz, oy) x 32
-// channels; its B fragments at coarse x offsets -1, 0, +1 serve the four (rx, ox) products.
-// ADD: the epilogue also writes Y2 = bf16(Y + E), with E loaded before the MFMAs.
+// as [8-channel piece][voxel] with a 256-byte multiple piece stride): coarse plane k + 1 + rz,
+// loaded into registers during step k - 1, is written after step k into the slot nobody reads
+// in step k; one barrier per step. A K = 32 step is one coarse tap (oz, oy) x 32 channels; its
+// B fragments at coarse x offsets -1, 0, +1 serve the four (rx, ox) products.
+// ADD: the epilogue also writes Y2 = bf16(Y + E), with E loaded one step ahead. Y may be NULL
+// (inference: only Y2 is stored).
 template <int CIN, int COUT, bool RELU, int NSEG, bool ADD>
 __global__ __launch_bounds__(kNetThreads) void k_conv_sub(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Ws,
                                                          uint16_t* __restrict__ Y, int D, int H, int W, int ZB,
