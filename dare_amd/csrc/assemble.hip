@@ -585,6 +585,7 @@ int dare_csr_destroy(dare_csr* A) {
     if (A->val32) hipFree(A->val32);
     if (A->dia) hipFree(A->dia);
     if (A->dia32) hipFree(A->dia32);
+    if (A->asym32) hipFree(A->asym32);
     if (A->dinv) hipFree(A->dinv);
     if (A->dia_pre) hipFree(A->dia_pre);
     delete A;
@@ -643,6 +644,7 @@ int dare_csr_build_structure(dare_ctx* ctx, dare_csr* A, const dare_grid* g, con
     A->cross = T.cross;
     A->vals_stale = 0;
     A->dia32_fresh = 0;
+    A->sym_fill = 0;
     // diagonal view for the solver (3-D, 1 or 2 components)
     A->has_dia = 0;
     if (G.dim == 3 && (G.N == 1 || G.N == 2)) {
@@ -709,6 +711,9 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         DARE_LAUNCH_CHECK();
         A->dia32_fresh = f32 ? 1 : 0;
         A->vals_stale = 1;
+        bool sym = G.N == 1 && t->conv_limiter < 0 && t->diff_cell == nullptr && t->react == nullptr;
+        for (int f = 0; f < 6; f++) sym = sym && t->bc_type[f] != DARE_BC_FIXED;
+        A->sym_fill = sym ? 1 : 0;
         return DARE_OK;
     }
     hipLaunchKernelGGL(k_fill_values, dim3(ceil_div(nrows, 256)), dim3(256), 0, ctx->stream, G, T, nrows,
@@ -741,6 +746,7 @@ int dare_csr_upload_host(dare_ctx* ctx, dare_csr* A, int64_t nrows, int64_t nnz,
     A->has_dia = 0;
     A->vals_stale = 0;
     A->dia32_fresh = 0;
+    A->sym_fill = 0;
     return DARE_OK;
 }
 

@@ -110,6 +110,13 @@ struct dare_csr {
     // the FP32 copy for mixed solves: once a mixed solve has asked for it (dia32_mode), every
     // fill writes it next to the FP64 slots; dia32_fresh says it holds float(dia) of the last fill
     int dia32_mode = 0, dia32_fresh = 0;
+    // device int: 0 when the FP32 diagonal view is exactly symmetric (E, N, T of each row equal
+    // W, S, B of its neighbour), set by the check each mixed solve runs on a fresh copy
+    int* asym32 = nullptr;
+    // 1: the last device fill's operator is exactly symmetric by construction (one component,
+    // constant diffusivity, no convection, reaction or fixed faces: every face coefficient is
+    // the same expression for both rows it couples)
+    int sym_fill = 0;
     // Jacobi right preconditioning: 1 / diag(A) and the column-scaled FP64 operator A diag(A)^-1
     // (diagonal-view slots, or CSR values for a matrix without a diagonal view)
     double* dinv = nullptr;

@@ -519,6 +519,20 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         }
         op32.d = make_dia<float>(A->dia_view, A->dia32, true);
         op32.dia = kXcd * op32.d.per_xcd <= kReduceBlocks;
+        if (op32.dia && op32.d.pair == 4 && !pre) {
+            // exactly symmetric FP32 view (a symmetric operator, Jacobi column scaling off):
+            // the quad traversal may then read E, N, T from the neighbours' W, S, B slots
+            if (!A->asym32 && hipMalloc(&A->asym32, sizeof(int)) != hipSuccess) {
+                hipGetLastError();
+                set_error("symmetry flag allocation failed");
+                return DARE_ERR_OOM;
+            }
+            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, ctx->stream, A->asym32, 0);
+            if (!A->sym_fill)   // not symmetric by construction: compare the slots
+                hipLaunchKernelGGL(k_dia32_asym, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, A->dia_view,
+                                   static_cast<const float*>(A->dia32), A->asym32);
+            op32.d.asym = A->asym32;
+        }
     } else {
         int erc = csr_ensure_values(A, ctx->stream);
         if (erc) return erc;

@@ -211,6 +211,7 @@ struct DiaOp {
     int tile_i;         // planes per tile (grown until the grid fits the reduction slots)
     int per_xcd;        // tiles owned by each XCD (grid = kXcd * per_xcd)
     int pair;           // cells per lane along k (2: tiles of 128 (k) x 4 (j), one component, n2 even)
+    const int* asym = nullptr;   // FP32 quad form: *asym == 0 selects the symmetric traversal
 };
 
 template <typename VT>
@@ -657,6 +658,115 @@ __device__ inline void dia_column_quad(const DiaView& V, const float* __restrict
     }
 }
 
+// dia_column_quad for an exactly symmetric FP32 view (checked by k_dia32_asym): the E, N and T
+// coefficients of a row are read as the W, S and B coefficients of its east, north and top
+// neighbours, so four slots stream from HBM instead of seven. The partner values are the same
+// floats and the arithmetic is that of dia_column_quad, so the results are bit-identical. The
+// W, S and B slots are read twice (as a row's own and as its neighbour's partner), the second
+// time from the caches, so they carry no non-temporal hint.
+template <class Epi>
+__device__ inline void dia_column_quad_sym(const DiaView& V, const float* __restrict__ D, const float* __restrict__ x,
+                                           int k, int j, int i0, int i1, const Epi& epi, double* acc) {
+    const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
+    const long long sI = static_cast<long long>(n1) * n2, sJ = n2;
+    const long long nr = V.nrows;
+    const bool hasS = !(j == 0 && V.rm_lo[1]);
+    const bool hasN = !(j == n1 - 1 && V.rm_hi[1] && !(j == 0 && V.rm_lo[1]));
+    const bool hasB = !(k == 0 && V.rm_lo[2]);
+    const bool hasT = !(k + 3 == n2 - 1 && V.rm_hi[2]);
+    const long long dS = !hasS ? 0 : (j == 0) ? (long long)(n1 - 1) * sJ : -sJ;
+    const long long dN = !hasN ? 0 : (j == n1 - 1) ? -(long long)(n1 - 1) * sJ : sJ;
+    const long long dB = !hasB ? 0 : (k == 0) ? (long long)(n2 - 1) : -1;
+    const long long dT = !hasT ? 0 : (k + 3 == n2 - 1) ? -(long long)(n2 - 1) : 1;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    auto ld4 = [](const float* p) { return *reinterpret_cast<const f4*>(p); };
+    auto ld4nt = [](const float* p) {
+        if constexpr (Epi::kNT) return __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+        else return *reinterpret_cast<const f4*>(p);
+    };
+    const float* DW = D;
+    const float* DS = D + nr;
+    const float* DB = D + 2 * nr;
+    const float* DC = D + 3 * nr;
+    long long node = (static_cast<long long>(i0) * n1 + j) * n2 + k;
+    f4 xp, xc;
+    {
+        const bool need = !(i0 == 0 && V.rm_lo[0]);
+        const long long nprev = !need ? node : (i0 == 0) ? node + (long long)(n0 - 1) * sI : node - sI;
+        xc = ld4(x + node);
+        xp = ld4(x + nprev);
+    }
+    for (int i = i0; i < i1; i++) {
+        const bool hasW = !(i == 0 && V.rm_lo[0]);
+        const bool hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
+        const long long nnext = !hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
+        const f4 xn = ld4(x + nnext), xs = ld4(x + node + dS), xq = ld4(x + node + dN);
+        const float xb = x[node + dB], xt = x[node + 3 + dT];
+        const f4 cW = ld4(DW + node), cS = ld4(DS + node), cB = ld4(DB + node), cC = ld4nt(DC + node);
+        const f4 cN = ld4(DS + node + dN);          // S of the north neighbour (unused without it)
+        const f4 cE = ld4(DW + nnext);              // W of the east neighbour
+        const float cT3 = DB[node + 3 + dT];        // B of the top neighbour of cell k + 3
+        const double p0 = epi.load(node), p1 = epi.load(node + 1), p2 = epi.load(node + 2), p3 = epi.load(node + 3);
+        const float xcv[4] = {xc.x, xc.y, xc.z, xc.w}, xpv[4] = {xp.x, xp.y, xp.z, xp.w};
+        const float xsv[4] = {xs.x, xs.y, xs.z, xs.w}, xqv[4] = {xq.x, xq.y, xq.z, xq.w};
+        const float xnv[4] = {xn.x, xn.y, xn.z, xn.w};
+        const float wv[4] = {cW.x, cW.y, cW.z, cW.w}, sv[4] = {cS.x, cS.y, cS.z, cS.w};
+        const float bv[4] = {cB.x, cB.y, cB.z, cB.w}, cv[4] = {cC.x, cC.y, cC.z, cC.w};
+        const float tv[4] = {cB.y, cB.z, cB.w, cT3};
+        const float nv[4] = {cN.x, cN.y, cN.z, cN.w}, ev[4] = {cE.x, cE.y, cE.z, cE.w};
+        double y[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            float s = 0.f;
+            s += hasW ? wv[m] * xpv[m] : 0.f;
+            s += hasS ? sv[m] * xsv[m] : 0.f;
+            if (m == 0) s += hasB ? bv[m] * xb : 0.f;
+            else s += bv[m] * xcv[m - 1];
+            s += cv[m] * xcv[m];
+            if (m == 3) s += hasT ? tv[m] * xt : 0.f;
+            else s += tv[m] * xcv[m + 1];
+            s += hasN ? nv[m] * xqv[m] : 0.f;
+            s += hasE ? ev[m] * xnv[m] : 0.f;
+            y[m] = static_cast<double>(s);
+        }
+        const double xcd[4] = {xcv[0], xcv[1], xcv[2], xcv[3]}, pre[4] = {p0, p1, p2, p3};
+        epi_quad(epi, node, y, xcd, pre, acc);
+        xp = xc;
+        xc = xn;
+        node += sI;
+    }
+}
+
+// Symmetry check of a one-component FP32 view: flag = 1 if the E, N or T coefficient of any
+// row differs (bit for bit) from the W, S or B coefficient of the neighbour it couples to, with
+// the neighbour and presence rules of dia_column_quad. flag must be 0 before the launch.
+__global__ __launch_bounds__(kThreads) void k_dia32_asym(DiaView V, const float* __restrict__ D, int* __restrict__ flag) {
+    const long long nr = V.nrows;
+    const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
+    const long long sI = static_cast<long long>(n1) * n2, sJ = n2;
+    auto bits = [](float v) { return __float_as_uint(v); };
+    bool bad = false;
+    for (long long row = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x; row < nr;
+         row += static_cast<long long>(gridDim.x) * kThreads) {
+        const int k = static_cast<int>(row % n2), j = static_cast<int>((row / n2) % n1), i = static_cast<int>(row / sI);
+        if (!(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]))) {
+            const long long nb = (i == n0 - 1) ? row - (long long)(n0 - 1) * sI : row + sI;
+            bad |= bits(D[6 * nr + row]) != bits(D[nb]);
+        }
+        if (!(j == n1 - 1 && V.rm_hi[1] && !(j == 0 && V.rm_lo[1]))) {
+            const long long nb = (j == n1 - 1) ? row - (long long)(n1 - 1) * sJ : row + sJ;
+            bad |= bits(D[5 * nr + row]) != bits(D[nr + nb]);
+        }
+        if (!(k == n2 - 1 && V.rm_hi[2])) {
+            const long long nb = (k == n2 - 1) ? row - (n2 - 1) : row + 1;
+            bad |= bits(D[4 * nr + row]) != bits(D[2 * nr + nb]);
+        }
+    }
+    if (bad) flag[0] = 1;
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) { p[0] = v; }
+
 template <int NC, typename VT, typename XT, class Epi, int PAIR = 1>
 __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
                                                        KState* st, int tail, int check_status) {
@@ -678,7 +788,8 @@ __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* _
             const int i1 = (i0 + A.tile_i < A.V.n[0]) ? i0 + A.tile_i : A.V.n[0];
             if constexpr (PAIR == 4) {
                 static_assert(std::is_same_v<VT, float> && std::is_same_v<XT, float>, "quad traversal is FP32");
-                dia_column_quad<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
+                if (A.asym != nullptr && *A.asym == 0) dia_column_quad_sym<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
+                else dia_column_quad<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
             } else if constexpr (PAIR == 2) {
                 dia_column_pair<VT, XT, Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
             } else {

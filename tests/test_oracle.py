@@ -583,3 +583,38 @@ def test_gaussian_blob_heat_kernel_c1_small():
     inner = tuple(slice(2, -2) for _ in range(3))
     err = np.max(np.abs(phi.reshape(r2.shape)[inner] - exact[inner]))
     assert err < 2e-3, err
+
+
+def _offdiag_bits(rp, col, val):
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = rows != col
+    return {(int(r), int(c)): int(v) for r, c, v in zip(rows[off], col[off], val[off].view(np.int64))}
+
+
+@pytest.mark.parametrize("case", [
+    dict(n=[6, 7, 8], periodic=None, bc=[1] * 6, dt=1e-2, size=[1.0, 1.0, 1.0]),
+    dict(n=[5, 9, 12], periodic=None, bc=[1, 2, 2, 1, 1, 2], dt=5e-3, size=[0.5, 0.9, 1.7]),
+    dict(n=[6, 7, 12], periodic=[1, 1, 1], bc=None, dt=5e-3, size=[1.0, 1.0, 1.0]),
+    dict(n=[4, 6, 8], periodic=[1, 0, 1], bc=[0, 0, 1, 2, 0, 0], dt=0.0, size=[2.0, 1.0, 0.3]),
+])
+def test_constant_diffusion_assembly_is_exactly_symmetric(case):
+    """One component, constant diffusivity, no convection, reaction or fixed faces: every
+    off-diagonal entry equals its transpose bit for bit, the property under which the device
+    fill lets the mixed solve's FP32 operator read a row's E, N, T coefficients from its
+    neighbours' W, S, B. Convection breaks it. (Per-cell harmonic-mean diffusivity keeps it on
+    some grids, not across a periodic wrap, so the device fill does not assume it.)"""
+    rng = np.random.default_rng(11)
+    go = orc.grid(case["n"], size=case["size"], ghost=2, periodic=case["periodic"])
+    npad = orc.lib().orc_num_cells_padded(ctypes.byref(go))
+    phi = rng.normal(size=npad)
+    kw = dict(dt=case["dt"], diff=1.7e-3)
+    if case["bc"] is not None:
+        kw.update(bc_type=case["bc"], bc_value=list(rng.normal(size=6)))
+    rp, col, val, _, _ = orc.assemble(go, orc.terms(**kw), phi)
+    a = _offdiag_bits(rp, col, val)
+    assert a == {(c, r): v for (r, c), v in a.items()}
+    vel = [rng.uniform(-1, 1, size=orc.lib().orc_num_faces(ctypes.byref(go), d)) for d in range(3)]
+    rp, col, val, _, _ = orc.assemble(go, orc.terms(limiter=4, vel=vel, **kw), phi)
+    a = _offdiag_bits(rp, col, val)
+    assert a != {(c, r): v for (r, c), v in a.items()}
