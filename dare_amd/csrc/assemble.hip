@@ -10,6 +10,7 @@
 // Tpetra's fillComplete. Structure: count -> exclusive scan -> columns. Values are re-run
 // every step without touching the structure.
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "common.h"
@@ -536,6 +537,14 @@ int csr_ensure_values(dare_csr* A, hipStream_t stream) {
 
 using namespace dare_hip;
 
+namespace {
+struct DeviceFree {
+    void operator()(void* p) const {
+        if (p) (void)hipFree(p);
+    }
+};
+}  // namespace
+
 static int ensure_rows(dare_csr* A, int64_t nrows) {
     if (A->cap_rows >= nrows + 1 && A->row_ptr) return DARE_OK;
     if (A->row_ptr) hipFree(A->row_ptr);
@@ -608,34 +617,35 @@ int dare_csr_build_structure(dare_ctx* ctx, dare_csr* A, const dare_grid* g, con
             DARE_CHECK_ARG(t->bc_type[f] != DARE_BC_NONE,
                            "non-periodic boundary face without a boundary edit (out-of-range columns)");
     }
+    // A boundary edit touches one face per axis (the reference functor's if / else-if on the low
+    // and the high index, counter_diffusion2D.cpp:125-161): on a single-cell non-periodic axis the
+    // high neighbour stays and maps past the last row. Refuse such grids.
+    for (int d = 0; d < G.dim; d++)
+        DARE_CHECK_ARG(G.n[d] > 1 || G.per[d],
+                       "single-cell non-periodic axis (its high neighbour column would be out of range)");
     const long long nrows = G.ncell_int * G.N;
     rc = ensure_rows(A, nrows);
     if (rc) return rc;
     const long long ntiles = (nrows + kScanTile - 1) / kScanTile;
-    long long* tiles = nullptr;
-    DARE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tiles), sizeof(long long) * (ntiles + 1)));
-    hipLaunchKernelGGL(k_count_tiles, dim3(ntiles), dim3(kScanThreads), 0, ctx->stream, G, T, nrows, tiles);
+    long long* tiles_raw = nullptr;
+    DARE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tiles_raw), sizeof(long long) * (ntiles + 1)));
+    // freed on every return path; hipFree waits for the kernels that still use it
+    std::unique_ptr<long long, DeviceFree> tiles(tiles_raw);
+    hipLaunchKernelGGL(k_count_tiles, dim3(ntiles), dim3(kScanThreads), 0, ctx->stream, G, T, nrows, tiles.get());
     DARE_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, ctx->stream, tiles, ntiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, ctx->stream, tiles.get(), ntiles);
     DARE_LAUNCH_CHECK();
     long long total = 0;
-    DARE_HIP_TRY(hipMemcpyAsync(&total, tiles + ntiles, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    DARE_HIP_TRY(hipMemcpyAsync(&total, tiles.get() + ntiles, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
     DARE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (total >= (1LL << 31)) {
-        hipFree(tiles);
-        set_error("nnz exceeds 32-bit row pointers");
-        return DARE_ERR_INVALID;
-    }
+    DARE_CHECK_ARG(total < (1LL << 31), "nnz exceeds 32-bit row pointers");
     rc = ensure_nnz(A, total);
-    if (rc) {
-        hipFree(tiles);
-        return rc;
-    }
-    hipLaunchKernelGGL(k_write_structure, dim3(ntiles), dim3(kScanThreads), 0, ctx->stream, G, T, nrows, tiles,
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_write_structure, dim3(ntiles), dim3(kScanThreads), 0, ctx->stream, G, T, nrows, tiles.get(),
                        A->row_ptr, A->col);
     DARE_LAUNCH_CHECK();
     DARE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    DARE_HIP_TRY(hipFree(tiles));
+    DARE_HIP_TRY(hipFree(tiles.release()));
     A->nrows = nrows;
     A->nnz = total;
     A->stencil_structured = 1;
@@ -687,6 +697,14 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
     if (rc) return rc;
     const long long nrows = G.ncell_int * G.N;
     DARE_CHECK_ARG(A->stencil_structured && A->nrows == nrows, "CSR structure does not match the grid");
+    // the diagonal view and the lazy CSR values walk the build-time grid: the fill must use the same one
+    {
+        const dare_grid& B = A->grid_desc;
+        bool same = B.dim == g->dim && B.ncomp == g->ncomp;
+        for (int d = 0; d < g->dim && same; d++)
+            same = B.n[d] == g->n[d] && (B.periodic[d] != 0) == (g->periodic[d] != 0);
+        DARE_CHECK_ARG(same, "grid differs from the one the CSR structure was built on (dim, ncomp, n, periodic)");
+    }
     DARE_CHECK_ARG(T.cross == A->cross, "reaction terms differ from the ones the structure was built with");
     for (int f = 0; f < 6; f++)
         DARE_CHECK_ARG(T.bc_type[f] == A->bc_type[f], "boundary edits differ from the structure's");
@@ -713,6 +731,9 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         A->vals_stale = 1;
         bool sym = G.N == 1 && t->conv_limiter < 0 && t->diff_cell == nullptr && t->react == nullptr;
         for (int f = 0; f < 6; f++) sym = sym && t->bc_type[f] != DARE_BC_FIXED;
+        // a single-cell axis couples a cell to itself through both faces; the structure refuses
+        // it unless periodic, and the symmetric traversal is not claimed for it either way
+        for (int d = 0; d < G.dim; d++) sym = sym && G.n[d] > 1;
         A->sym_fill = sym ? 1 : 0;
         return DARE_OK;
     }

@@ -265,7 +265,7 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net);
 /* Inference mode (off after create): the forward pass does not store the decoder's ReLU
  * outputs a7 and a9, which only the backward pass reads (at 256^3 that is 0.67 GB less HBM
  * traffic per forward pass); every other tensor and the output are unchanged. While the last
- * forward pass ran in this mode dare_conv_net_bwd refuses with DARE_ERR_ARG. */
+ * forward pass ran in this mode dare_conv_net_bwd refuses with DARE_ERR_INVALID. */
 int dare_convnet_set_inference(dare_convnet* net, int32_t on);
 int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* x_base, double* x_guess,
                         const int32_t* crop);

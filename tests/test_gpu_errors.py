@@ -48,12 +48,23 @@ def test_invalid_arguments_are_refused_and_the_context_survives(ctx):
     _raises(lambda: da.build_structure(ctx, A, da.Grid.make([1400, 1400, 1400], ghost=2),
                                        da.Terms.make(dt=1.0, diff=1.0, bc_type=[1] * 6)),
             "grid too large for 32-bit ordinals")
+    # a single-cell non-periodic axis: the edit of the low face leaves the high neighbour, whose
+    # column would lie past the last row
+    _raises(lambda: da.build_structure(ctx, A, da.Grid.make([1, 6, 12], ghost=2),
+                                       da.Terms.make(dt=1.0, diff=1.0, bc_type=[1] * 6)),
+            "single-cell non-periodic axis")
+    _raises(lambda: da.build_structure(ctx, A, da.Grid.make([7, 1], ghost=1),
+                                       da.Terms.make(dt=1.0, diff=1.0, bc_type=[2] * 4)),
+            "single-cell non-periodic axis")
     t = da.Terms.make(dt=1.0, diff=1.0, bc_type=[1] * 6, bc_value=[0.0] * 6)
     da.build_structure(ctx, A, g, t)
     rhs = torch.zeros(120, dtype=torch.float64, device="cuda")
     x = torch.zeros_like(rhs)
     _raises(lambda: da.fill_values(ctx, A, g, da.Terms.make(dt=1.0, diff=1.0, bc_type=[2] * 6), phi, rhs, x),
             "boundary edits differ from the structure's")
+    # same cell count, other shape or periodicity: refused (the diagonal view walks the build grid)
+    for other in (da.Grid.make([4, 5, 6], ghost=2), da.Grid.make([6, 5, 4], ghost=2, periodic=[1, 0, 0])):
+        _raises(lambda: da.fill_values(ctx, A, other, t, phi, rhs, x), "grid differs from the one the CSR structure")
 
     # host CSR upload: pointers, monotonicity, columns
     B = da.Csr(ctx)

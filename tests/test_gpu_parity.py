@@ -152,6 +152,11 @@ def assembly_case(rng, name):
         # across the wrap of every axis
         go = orc.grid([6, 7, 12], ghost=2, periodic=[1, 1, 1])
         kw = dict(dt=5e-3, diff=2e-3)
+    elif name == "periodic_single":   # one cell along a periodic x: W and E couple the cell to
+        # itself (merged into the CSR diagonal); the FP32 view is then checked for symmetry on
+        # the device instead of being taken as symmetric by construction
+        go = orc.grid([1, 6, 12], ghost=2, periodic=[1, 0, 0])
+        kw = dict(dt=5e-3, diff=2e-3, bc_type=[0, 0, 2, 2, 1, 1], bc_value=[0, 0, 0, 0, 0.5, -0.25])
     elif name == "wide_dirichlet":    # two 128-cell k tiles, the second partial
         go = orc.grid([6, 5, 140], size=[0.5, 0.4, 1.0], ghost=2)
         kw = dict(dt=1e-2, diff=1e-3, bc_type=[1, 2, 2, 1, 1, 1], bc_value=[0.5, 0, 0, 1.0, 0, -0.5])
@@ -299,8 +304,8 @@ def test_mixed_refinement_reaches_fp64_residual(ctx, rng, name):
     assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo))
 
 
-DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "periodic_even", "periodic_sym", "wide_dirichlet",
-             "react2", "fixed_x", "fixed_react2"]
+DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "periodic_even", "periodic_sym", "periodic_single",
+             "wide_dirichlet", "react2", "fixed_x", "fixed_react2"]
 
 
 def device_assembly(ctx, rng, name):
