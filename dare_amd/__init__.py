@@ -202,6 +202,7 @@ def lib():
         "dare_axpby": (i32, [vp, i64, dbl, vp, dbl, vp]),
         "dare_residual": (i32, [vp, vp, vp, vp, vp, ctypes.POINTER(dbl)]),
         "dare_bicgstab2_solve": (i32, [vp, vp, vp, vp, P, R]),
+        "dare_bicgstab2_solve_noalloc": (i32, [vp, vp, vp, vp, P, R]),
         "dare_refine_solve_mixed": (i32, [vp, vp, vp, vp, P, R]),
         "dare_batch_create": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
         "dare_batch_destroy": (i32, [vp]),
@@ -399,6 +400,18 @@ def bicgstab2(ctx, csr, b, x, tol=1e-8, tol_kind=0, max_it=100, history=0, preco
     r = SolverResult()
     check(lib().dare_bicgstab2_solve(ctx.h, csr.h, _ptr(b), _ptr(x), ctypes.byref(p), ctypes.byref(r)),
           "dare_bicgstab2_solve")
+    out = {k: getattr(r, k) for k, _ in SolverResult._fields_}
+    out["history"] = list(hist[:r.history_len])
+    return out
+
+
+def bicgstab2_noalloc(ctx, csr, b, x, tol=1e-8, tol_kind=0, max_it=100, history=0):
+    """dare_bicgstab2_solve_noalloc: the FP64 solve with its workspace in the library's static
+    device scratch (no device allocation; at most 262144 rows, unpreconditioned)."""
+    p, hist = _params(tol, tol_kind, max_it, 0.0, 0, 0, history, 0)
+    r = SolverResult()
+    check(lib().dare_bicgstab2_solve_noalloc(ctx.h, csr.h, _ptr(b), _ptr(x), ctypes.byref(p), ctypes.byref(r)),
+          "dare_bicgstab2_solve_noalloc")
     out = {k: getattr(r, k) for k, _ in SolverResult._fields_}
     out["history"] = list(hist[:r.history_len])
     return out

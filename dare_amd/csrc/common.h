@@ -133,6 +133,10 @@ namespace dare_hip {
 
 int make_geo(const dare_grid* gr, Geo* G);
 
+// Looks up, with `device` current, the address of the allocation-free solve's static scratch
+// on that device (once per device; dare_ctx_create calls it).
+int noalloc_scratch_init(int device);
+
 // Writes the CSR values of a matrix whose last fill only wrote the diagonal view.
 int csr_ensure_values(dare_csr* A, hipStream_t stream);
 

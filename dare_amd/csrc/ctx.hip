@@ -80,6 +80,8 @@ int dare_ctx_create(int32_t device, dare_ctx** out) {
     DARE_HIP_TRY(hipGetDeviceCount(&ndev));
     DARE_CHECK_ARG(device >= 0 && device < ndev, "device index out of range");
     DARE_HIP_TRY(hipSetDevice(device));
+    const int src = noalloc_scratch_init(device);
+    if (src) return src;
     dare_ctx* c = new dare_ctx();
     c->device = device;
     hipDeviceProp_t prop;

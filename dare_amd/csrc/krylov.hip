@@ -13,6 +13,8 @@
 // Matrix values then stream fully coalesced and x is read from HBM about once per SpMV.
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "krylov_core.h"
@@ -100,7 +102,7 @@ static_assert(sizeof(KState) <= 64 * sizeof(double), "pinned readback block too 
 template <typename VT, typename XT, bool NT>
 int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
                      double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                     bool* converged, XT* work, int min_passes, bool x_zero) {
+                     bool* converged, XT* work, int min_passes, bool x_zero, KState* st, ReduceWs ws) {
     const long long n = A.n;
     XT* r = work;
     XT* rh = r + n;
@@ -109,10 +111,8 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
     XT* s = v + n;
     XT* w = s + n;
     XT* t = w + n;
-    KState* st = static_cast<KState*>(ctx->solve_state);
     const unsigned gr = grid_for(n);
     hipStream_t sm = ctx->stream;
-    ReduceWs ws = ctx->ws;
     *hist_len = 0;
     *status = 0;
     if (x_zero) {   // x = 0 written here: no operator sweep needed for r = b - A x
@@ -179,29 +179,36 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
     return DARE_OK;
 }
 
+// st / ws: the device scalars and reduction partials of this solve (the context's own, or the
+// allocation-free entry point's static scratch)
 template <typename VT, typename XT>
 int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
                   double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                  bool* converged, XT* work, int min_passes, bool x_zero = false) {
+                  bool* converged, XT* work, int min_passes, bool x_zero, KState* st, ReduceWs ws) {
     if (A.n >= kStreamRows)
         return bicgstab2_run_nt<VT, XT, true>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
-                                              status, converged, work, min_passes, x_zero);
+                                              status, converged, work, min_passes, x_zero, st, ws);
     return bicgstab2_run_nt<VT, XT, false>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
-                                           status, converged, work, min_passes, x_zero);
+                                           status, converged, work, min_passes, x_zero, st, ws);
+}
+
+static int dot_dev_on(dare_ctx* ctx, long long n, const double* a, const double* b, double* out, KState* st,
+                      ReduceWs ws) {
+    hipLaunchKernelGGL((k_dot<double>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, a, b, ws, st);
+    launch_finish<1>(ctx->stream, ws, grid_for(n), st, TAIL_STORE, 0);
+    DARE_LAUNCH_CHECK();
+    KState h;
+    int rc = read_state(ctx, st, &h);
+    if (rc) return rc;
+    *out = h.dots[0];
+    return DARE_OK;
 }
 
 static int dot_dev(dare_ctx* ctx, long long n, const double* a, const double* b, double* out) {
     Workspace W;
     int rc = ws_reserve(ctx, 0, &W);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_dot<double>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, a, b, ctx->ws, W.st);
-    launch_finish<1>(ctx->stream, ctx->ws, grid_for(n), W.st, TAIL_STORE, 0);
-    DARE_LAUNCH_CHECK();
-    KState h;
-    rc = read_state(ctx, W.st, &h);
-    if (rc) return rc;
-    *out = h.dots[0];
-    return DARE_OK;
+    return dot_dev_on(ctx, n, a, b, out, W.st, ctx->ws);
 }
 
 // ---- Jacobi right preconditioning (M = diag(A)). The method runs on A M^-1 with y = M x:
@@ -343,6 +350,56 @@ static SolveOp<double> op64(const dare_csr* A, bool allow_dia) {
 }  // namespace
 }  // namespace dare_hip
 
+// ---- static scratch of the allocation-free solve (dare_bicgstab2_solve_noalloc) -------------
+// Seven FP64 work vectors of at most kNoAllocMaxRows rows, the reduction partials and the
+// device scalar block. Each device that loads this library holds its own copy until the
+// process ends. Host code never passes `scratch` itself to a kernel: its address on a device
+// comes from hipGetSymbolAddress (noalloc_scratch_init, run by dare_ctx_create with that
+// device current) and is kept per device.
+namespace dare_hip {
+namespace {
+constexpr long long kNoAllocMaxRows = 1LL << 18;                                   // 262,144 rows (64^3)
+constexpr long long kScratchVec = 7 * kNoAllocMaxRows;                            // doubles
+constexpr long long kScratchPartial = static_cast<long long>(kReduceBlocks) * kMaxDots;
+constexpr long long kScratchState = 64;                                           // KState slot
+constexpr long long kScratchCount = 32;                                           // ReduceWs::count slot
+constexpr long long kScratchDoubles = kScratchVec + kScratchPartial + kScratchState + kScratchCount;
+static_assert(sizeof(KState) <= kScratchState * sizeof(double), "scratch state slot too small");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+}  // namespace dare_hip
+
+static __device__ __align__(256) double scratch[dare_hip::kScratchDoubles];
+
+namespace dare_hip {
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device index beyond the allocation-free scratch table");
+        return DARE_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return DARE_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return DARE_ERR_HIP;
+    }
+    g_scratch[device] = static_cast<double*>(p);
+    return DARE_OK;
+}
+
+static double* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
+}
+
+}  // namespace dare_hip
+
 using namespace dare_hip;
 
 extern "C" {
@@ -460,12 +517,62 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     double rn = 0.;
     bool conv = false;
     rc = bicgstab2_run<double, double>(ctx, op, b, x, tol_abs, p->max_it, p->history, p->history_cap, &hl, &iters,
-                                       &rn, &status, &conv, static_cast<double*>(W.buf), 2);
+                                       &rn, &status, &conv, static_cast<double*>(W.buf), 2, false, W.st, ctx->ws);
     if (rc) return rc;
     if (pre) {
         hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 1);  // x = M^-1 y
         DARE_LAUNCH_CHECK();
     }
+    res->converged = conv ? 1 : 0;
+    res->iterations = iters;
+    res->status = status ? DARE_ERR_BREAKDOWN : 0;
+    res->refinements = 0;
+    res->residual = rn;
+    res->bnorm = bnorm;
+    res->history_len = hl;
+    return DARE_OK;
+}
+
+int dare_bicgstab2_solve_noalloc(dare_ctx* ctx, const dare_csr* A, const double* b, double* x,
+                                 const dare_solver_params* p, dare_solver_result* res) {
+    DARE_CHECK_ARG(ctx && A && b && x && p && res, "NULL argument");
+    DARE_CHECK_ARG(A->nrows > 0 && A->row_ptr && A->col && A->val, "empty matrix");
+    DARE_CHECK_ARG(A->nrows <= kNoAllocMaxRows,
+                   "system exceeds the allocation-free maximum of 262144 rows (use dare_bicgstab2_solve)");
+    if (p->precond != DARE_PRECOND_NONE) {
+        set_error("the allocation-free solve runs unpreconditioned (precond must be 0)");
+        return DARE_ERR_UNSUPPORTED;
+    }
+    double* base = noalloc_scratch(ctx->device);
+    if (!base) {
+        set_error("allocation-free scratch not set up on this device (create the context first)");
+        return DARE_ERR_HIP;
+    }
+    const long long n = A->nrows;
+    double* work = base;
+    KState* st = reinterpret_cast<KState*>(base + kScratchVec + kScratchPartial);
+    ReduceWs ws{base + kScratchVec,
+                reinterpret_cast<unsigned int*>(base + kScratchVec + kScratchPartial + kScratchState)};
+    // the scratch keeps the previous call's contents: the scalar block starts from zero, every
+    // work vector and partial is written before it is read
+    hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(64), 0, ctx->stream, st);
+    DARE_LAUNCH_CHECK();
+    double bnorm2 = 0.;
+    int rc = dot_dev_on(ctx, n, b, b, &bnorm2, st, ws);
+    if (rc) return rc;
+    const double bnorm = std::sqrt(bnorm2);
+    const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
+    SolveOp<double> op = op64(A, true);
+    if (!op.dia) {
+        rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
+        if (rc) return rc;
+    }
+    int iters = 0, status = 0, hl = 0;
+    double rn = 0.;
+    bool conv = false;
+    rc = bicgstab2_run<double, double>(ctx, op, b, x, tol_abs, p->max_it, p->history, p->history_cap, &hl, &iters,
+                                       &rn, &status, &conv, work, 2, false, st, ws);
+    if (rc) return rc;
     res->converged = conv ? 1 : 0;
     res->iterations = iters;
     res->status = status ? DARE_ERR_BREAKDOWN : 0;
@@ -603,7 +710,7 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         bool iconv = false;
         rc = bicgstab2_run<float, float>(ctx, op32, r32, d32, inner_threshold(inner_tol, rn, tol_abs), inner_max,
                                          nullptr, 0, &dummy_len, &it, &irn, &ist, &iconv, work, 2,
-                                         true);   // d32 = 0 to start
+                                         true, W.st, ctx->ws);   // d32 = 0 to start
         if (rc) return rc;
         total_it += it;
         if (pre)    // the inner solution is y = M d

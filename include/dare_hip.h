@@ -196,6 +196,17 @@ int dare_residual(dare_ctx* ctx, const dare_csr* csr, const double* b, const dou
  * BiCGStab2::Solve (TrilinosSolver.inl:71-93,150-152; BiCGStab2.inl:167-582). x holds x0. */
 int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* csr, const double* b, double* x,
                          const dare_solver_params* p, dare_solver_result* res);
+/* Allocation-free FP64 BiCGStab(2): the same method, kernels and results as
+ * dare_bicgstab2_solve, with its work vectors, reduction partials and scalars in a static
+ * device array (15.2 MB per device, held until the process ends) instead of context memory.
+ * No counterpart in the reference: for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc / hipFree.
+ * Limits: at most 262144 rows (a 64^3 one-component grid; larger systems return
+ * DARE_ERR_INVALID), no preconditioner (precond != 0 returns DARE_ERR_UNSUPPORTED), and the
+ * matrix must already hold its values (dare_csr_fill_values or dare_csr_upload_host). Every
+ * call on one device shares that device's array: calls that overlap on one GPU race. */
+int dare_bicgstab2_solve_noalloc(dare_ctx* ctx, const dare_csr* csr, const double* b, double* x,
+                                 const dare_solver_params* p, dare_solver_result* res);
 /* FP32 inner BiCGStab(2) inside FP64 iterative refinement; r = b - A x, the update of x and
  * the stopping test in FP64. No reference counterpart (north-star precision form). */
 int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* csr, const double* b, double* x,
