@@ -4,6 +4,7 @@ ctypes wrapper over oracle/liboracle.so (oracle.cpp, a C++ restatement of the re
 where it was built, oracle/_ref/libdare_ref.so (the reference's own standalone headers). Only
 tests/, __graft_entry__.smoke() and bench.py's cpu_baseline use this package, as the checker.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -55,44 +56,61 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
+@contextlib.contextmanager
+def using(path):
+    """Routes every wrapper of this module to another build of oracle.cpp for the duration of
+    the block (the planted-mistake checks of tests/ load mutant builds this way)."""
+    global _lib
+    saved = _lib
+    _lib = _load(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
-        G = ctypes.POINTER(OGrid)
-        T = ctypes.POINTER(OTerms)
-        sig = {
-            "orc_limiter": (dbl, [i32, dbl]),
-            "orc_diffusion_n0d1": (dbl, [dbl, dbl, i32]),
-            "orc_num_cells_padded": (i64, [G]),
-            "orc_num_cells_internal": (i64, [G]),
-            "orc_map_internal_to_local": (i64, [G, i64]),
-            "orc_num_faces": (i64, [G, i32]),
-            "orc_cell_center": (None, [G, vp, vp]),
-            "orc_grad_faces": (ctypes.c_int, [G, vp, vp, vp, vp]),
-            "orc_div_faces": (ctypes.c_int, [G, vp, vp, vp, vp]),
-            "orc_tvd_face_flux": (ctypes.c_int, [G, i32, vp, vp, vp, vp, vp, vp, vp]),
-            "orc_tvd_explicit_step": (ctypes.c_int, [G, i32, vp, vp, vp, vp, dbl, vp]),
-            "orc_tvd_matrix_cell": (ctypes.c_int, [G, i32, vp, vp, vp, vp, i64, i32, vp, vp, vp]),
-            "orc_ghost_fill": (ctypes.c_int, [G, vp, vp, vp]),
-            "orc_assemble": (i64, [G, T, vp, vp, vp, vp, i64, vp, vp]),
-            "orc_spmv": (None, [i64, vp, vp, vp, vp, vp]),
-            "orc_dot": (dbl, [i64, vp, vp]),
-            "orc_bicgstab2": (ctypes.c_int, [i64, vp, vp, vp, vp, vp, dbl, i32,
-                                             ctypes.POINTER(i32), ctypes.POINTER(dbl), vp, i32]),
-            "orc_copy_to_field": (None, [G, vp, vp]),
-            "orc_set_threads": (None, [i32]),
-            "orc_get_threads": (i32, []),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _load(LIB)
     return _lib
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    G = ctypes.POINTER(OGrid)
+    T = ctypes.POINTER(OTerms)
+    sig = {
+        "orc_limiter": (dbl, [i32, dbl]),
+        "orc_diffusion_n0d1": (dbl, [dbl, dbl, i32]),
+        "orc_num_cells_padded": (i64, [G]),
+        "orc_num_cells_internal": (i64, [G]),
+        "orc_map_internal_to_local": (i64, [G, i64]),
+        "orc_num_faces": (i64, [G, i32]),
+        "orc_cell_center": (None, [G, vp, vp]),
+        "orc_grad_faces": (ctypes.c_int, [G, vp, vp, vp, vp]),
+        "orc_div_faces": (ctypes.c_int, [G, vp, vp, vp, vp]),
+        "orc_tvd_face_flux": (ctypes.c_int, [G, i32, vp, vp, vp, vp, vp, vp, vp]),
+        "orc_tvd_explicit_step": (ctypes.c_int, [G, i32, vp, vp, vp, vp, dbl, vp]),
+        "orc_tvd_matrix_cell": (ctypes.c_int, [G, i32, vp, vp, vp, vp, i64, i32, vp, vp, vp]),
+        "orc_ghost_fill": (ctypes.c_int, [G, vp, vp, vp]),
+        "orc_assemble": (i64, [G, T, vp, vp, vp, vp, i64, vp, vp]),
+        "orc_spmv": (None, [i64, vp, vp, vp, vp, vp]),
+        "orc_dot": (dbl, [i64, vp, vp]),
+        "orc_bicgstab2": (ctypes.c_int, [i64, vp, vp, vp, vp, vp, dbl, i32,
+                                         ctypes.POINTER(i32), ctypes.POINTER(dbl), vp, i32]),
+        "orc_copy_to_field": (None, [G, vp, vp]),
+        "orc_set_threads": (None, [i32]),
+        "orc_get_threads": (i32, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
 
 
 def ref():

@@ -16,6 +16,14 @@
  *   ghost fill (periodic) ............ test_HaloBuffer.cpp value rule
  *   BiCGStab2 ........................ closed-form solves (test_TrilinosSolver.cpp,
  *                                      test_integration_Cartesian_Trilinos.cpp), dense LU
+ *   harmonic-mean face diffusivity ... series-conductance steady profile (closed form)
+ *   2x2 reaction block, source ....... exact per-cell implicit-Euler step, conservation
+ *   explicit TVD step, multi-D TVD ... conservation, quadratic-field closed form of r,
+ *                                      UPWIND step in numpy, matrix form = flux divergence
+ *   convection sign .................. steady upwind advection-diffusion geometric series
+ *   ghost modes 2-4 .................. brute force + face mean / zero-jump properties
+ *   (tests/test_oracle_pins.py also plants each targeted mistake in a mutant build and
+ *    requires its pin to fail)
  */
 #include "oracle.h"
 
