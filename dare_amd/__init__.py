@@ -185,6 +185,7 @@ def lib():
         "dare_ghost_fill": (i32, [vp, G, vp, vp, vp]),
         "dare_grad_faces": (i32, [vp, G, vp, vp, vp, vp]),
         "dare_div_faces": (i32, [vp, G, vp, vp, vp, vp]),
+        "dare_interpolate_to_faces": (i32, [vp, G, vp, vp, vp, vp, vp, vp]),
         "dare_tvd_face_flux": (i32, [vp, G, i32, vp, vp, vp, vp, vp, vp, vp]),
         "dare_tvd_explicit_step": (i32, [vp, G, i32, vp, vp, vp, vp, dbl, vp]),
         "dare_csr_create": (i32, [vp, ctypes.POINTER(vp)]),
@@ -318,6 +319,14 @@ def div_faces(ctx, grid, faces, out):
     f = list(faces) + [None] * (3 - len(faces))
     check(lib().dare_div_faces(ctx.h, ctypes.byref(grid), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]),
                                _ptr(out)), "dare_div_faces")
+
+
+def interpolate_to_faces(ctx, grid, ucell, faces):
+    """Cell-centred velocity fields (one padded tensor per axis) -> face arrays."""
+    u = list(ucell) + [None] * (3 - len(ucell))
+    f = list(faces) + [None] * (3 - len(faces))
+    check(lib().dare_interpolate_to_faces(ctx.h, ctypes.byref(grid), _ptr(u[0]), _ptr(u[1]), _ptr(u[2]),
+                                          _ptr(f[0]), _ptr(f[1]), _ptr(f[2])), "dare_interpolate_to_faces")
 
 
 def tvd_face_flux(ctx, grid, limiter, phi, vel, faces):

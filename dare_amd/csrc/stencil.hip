@@ -196,6 +196,37 @@ __global__ __launch_bounds__(256) void k_grad_faces(Geo G, FaceGeo F, const doub
     }
 }
 
+// ------------------------------------------------------------ cell-centred velocity -> faces
+// InterpolateToFace of a cell-centred one-component field (Interpolation_Cartesian.h:203-219,
+// 298-354), as the TVD constructor evaluates it for every face (TVD_Cartesian.inl:27-38): the
+// neighbour's value plus the cell's, times Divisor<2> = 0.5. Face m along axis d is shared by
+// cells m-1 and m; both orders of the sum give the same bits, so each face is written once.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_interp_faces(Geo G, FaceGeo F, const double* __restrict__ u0,
+                                                      const double* __restrict__ u1, const double* __restrict__ u2,
+                                                      double* __restrict__ f0, double* __restrict__ f1,
+                                                      double* __restrict__ f2) {
+    int q[3];
+    if (!cell_coord<DIM>(G, q)) return;
+    const double* in[3] = {u0, u1, u2};
+    double* out[3] = {f0, f1, f2};
+    const long long lp = pad_ord<DIM>(G, q);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const long long s = G.sl[d];
+        const long long fo = face_ord<DIM>(F, d, q);
+        const double uc = in[d][lp];
+        double v = in[d][lp - s];
+        v += uc;
+        out[d][fo] = v * 0.5;
+        if (q[d] == G.n[d] - 1) {
+            double w = uc;
+            w += in[d][lp + s];
+            out[d][fo + F.fs[d][d]] = w * 0.5;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------- value divergence
 // Divergence_Cartesian.inl:211-235: A0 E - A0 W + A1 N - A1 S + A2 T - A2 B, in that order.
 template <int DIM>
@@ -819,6 +850,24 @@ int dare_grad_faces(dare_ctx* ctx, const dare_grid* g, const double* phi, double
     const FaceGeo F = make_face_geo(G);
     const Launch L = cell_launch(G);
     DARE_DISPATCH_DIM(G, k_grad_faces, L, G, F, phi, fx, fy, fz);
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+int dare_interpolate_to_faces(dare_ctx* ctx, const dare_grid* g, const double* ux, const double* uy,
+                              const double* uz, double* fx, double* fy, double* fz) {
+    DARE_CHECK_ARG(ctx && g && ux && fx, "NULL argument");
+    dare_grid g1 = *g;
+    g1.ncomp = 1;              // velocity fields carry one component
+    Geo G;
+    int rc = make_geo(&g1, &G);
+    if (rc) return rc;
+    DARE_CHECK_ARG(G.g >= 1, "face interpolation needs >= 1 ghost layer");
+    DARE_CHECK_ARG(G.dim < 2 || (uy && fy), "y arrays are NULL");
+    DARE_CHECK_ARG(G.dim < 3 || (uz && fz), "z arrays are NULL");
+    const FaceGeo F = make_face_geo(G);
+    const Launch L = cell_launch(G);
+    DARE_DISPATCH_DIM(G, k_interp_faces, L, G, F, ux, uy, uz, fx, fy, fz);
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }

@@ -148,6 +148,15 @@ int dare_grad_faces(dare_ctx* ctx, const dare_grid* g, const double* phi,
  * (Grid/Cartesian/Divergence_Cartesian.inl:211-235). Writes internal cells of out. */
 int dare_div_faces(dare_ctx* ctx, const dare_grid* g, const double* fx, const double* fy,
                    const double* fz, double* out);
+/* Face velocities of the TVD operator from cell-centred velocity fields (padded layout, one
+ * component each, ghosts filled by the caller): face m along axis d takes (u[m-1] + u[m]) * 0.5.
+ * Replaces the cell-centred source of TVD(grid, ordinal, Vector<Dim, const GridVector*>)
+ * (TVD_Cartesian.inl:27-38) -> math::InterpolateToFace (Grid/Cartesian/Interpolation_Cartesian.h:
+ * 203-219,298-354). The face arrays feed dare_tvd_* and dare_terms.vel; a constant velocity
+ * (TVD(grid, ordinal, Vector<Dim, SC>), TVD_Cartesian.inl:41-53) is face arrays filled with its
+ * components (dare_fill_f64). g->ncomp is ignored. */
+int dare_interpolate_to_faces(dare_ctx* ctx, const dare_grid* g, const double* ux, const double* uy,
+                              const double* uz, double* fx, double* fy, double* fz);
 /* TVD face flux phi_f u_f. Replaces TVD::Interpolate(GridVector) times the face velocity
  * (TVD_Cartesian.inl:27-38,122-183; Stencils_Cartesian.inl:652-681). */
 int dare_tvd_face_flux(dare_ctx* ctx, const dare_grid* g, int32_t limiter, const double* phi,

@@ -93,6 +93,7 @@ def _load(path):
         "orc_cell_center": (None, [G, vp, vp]),
         "orc_grad_faces": (ctypes.c_int, [G, vp, vp, vp, vp]),
         "orc_div_faces": (ctypes.c_int, [G, vp, vp, vp, vp]),
+        "orc_interpolate_to_faces": (ctypes.c_int, [G, vp, vp, vp, vp, vp, vp]),
         "orc_tvd_face_flux": (ctypes.c_int, [G, i32, vp, vp, vp, vp, vp, vp, vp]),
         "orc_tvd_explicit_step": (ctypes.c_int, [G, i32, vp, vp, vp, vp, dbl, vp]),
         "orc_tvd_matrix_cell": (ctypes.c_int, [G, i32, vp, vp, vp, vp, i64, i32, vp, vp, vp]),
@@ -180,6 +181,19 @@ def div_faces(g, faces):
     rc = lib().orc_div_faces(ctypes.byref(g), _p(f[0]), _p(f[1]), _p(f[2]), _p(out))
     assert rc == 0
     return out
+
+
+def interpolate_to_faces(g, ucell):
+    """Cell-centred velocity fields (one padded array per axis) -> face arrays."""
+    f = faces_alloc_1(g) + [None] * (3 - g.dim)
+    u = list(ucell) + [None] * (3 - len(ucell))
+    rc = lib().orc_interpolate_to_faces(ctypes.byref(g), _p(u[0]), _p(u[1]), _p(u[2]), _p(f[0]), _p(f[1]), _p(f[2]))
+    assert rc == 0
+    return f[:g.dim]
+
+
+def faces_alloc_1(g):
+    return [np.zeros(lib().orc_num_faces(ctypes.byref(g), ax)) for ax in range(g.dim)]
 
 
 def tvd_face_flux(g, lim, phi, vel):

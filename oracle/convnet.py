@@ -237,3 +237,19 @@ def loss_and_dout(out, phi_int, x_true_int, s2, exact=False):
     n = d.size
     g = 2.0 * d / n
     return float(np.mean(d * d)), (g if exact else bf16(g))
+
+
+def adam(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1) in float32, in the order the device
+    kernel evaluates it: m = b1 m + (1 - b1) g; v = b2 v + (1 - b2) g g;
+    w -= lr (m / c1) / (sqrt(v / c2) + eps) with c1 = 1 - b1^t, c2 = 1 - b2^t.
+    Returns the new (w, m, v)."""
+    f = np.float32
+    w, g, m, v = (np.asarray(a, np.float32) for a in (w, g, m, v))
+    b1, b2, lr, eps = f(b1), f(b2), f(lr), f(eps)
+    c1 = f(1) - np.power(b1, f(t), dtype=np.float32)
+    c2 = f(1) - np.power(b2, f(t), dtype=np.float32)
+    m = b1 * m + (f(1) - b1) * g
+    v = b2 * v + (f(1) - b2) * g * g
+    w = w - lr * (m / c1) / (np.sqrt(v / c2) + eps)
+    return w, m, v

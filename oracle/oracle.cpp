@@ -694,6 +694,41 @@ int orc_div_faces(const orc_grid* gr, const double* fx, const double* fy, const 
     return 0;
 }
 
+// Face velocities from cell-centred velocity fields (padded layout, one component each):
+// math::InterpolateToFace for a cell-centred source and target (Interpolation_Cartesian.h:
+// 203-219,298-354) adds the neighbour's value, then the cell's, and multiplies by
+// Divisor<2> = 0.5 (Math/Divisors.h:59-79); the TVD constructor of TVD_Cartesian.inl:27-38
+// evaluates it for every face. Face m along axis d lies between cells m-1 and m, and both
+// cells see (u[m-1] + u[m]) * 0.5 (the sum is commutative), so one value per face.
+int orc_interpolate_to_faces(const orc_grid* gr, const double* ux, const double* uy, const double* uz,
+                             double* fx, double* fy, double* fz) {
+    Geo G;
+    if (!make_geo(gr, &G) || G.g < 1) return -1;
+    const double* in[3] = {ux, uy, uz};
+    double* out[3] = {fx, fy, fz};
+    for (int d = 0; d < G.dim; d++) {
+        if (!in[d] || !out[d]) return -1;
+        Ind q;
+        int64_t nf[3] = {1, 1, 1};
+        for (int e = 0; e < G.dim; e++) nf[e] = G.n[e] + (e == d ? 1 : 0);
+        const int64_t total = nf[0] * nf[1] * nf[2];
+        for (int64_t f = 0; f < total; f++) {
+            int64_t r = f;
+            for (int e = G.dim - 1; e >= 0; e--) {
+                q.v[e] = r % nf[e];
+                r /= nf[e];
+            }
+            Ind hi = internal_to_padded(G, q);     // cell m (padded)
+            Ind lo = hi;
+            lo.v[d]--;                             // cell m - 1
+            double v = in[d][local_ord(G, lo)];
+            v += in[d][local_ord(G, hi)];
+            out[d][face_index(G, d, q)] = v * 0.5;
+        }
+    }
+    return 0;
+}
+
 int orc_tvd_face_flux(const orc_grid* gr, int32_t lim, const double* phi,
                       const double* ux, const double* uy, const double* uz,
                       double* fx, double* fy, double* fz) {

@@ -62,6 +62,11 @@ void orc_cell_center(const orc_grid* g, const int32_t* ind, double* pos);
 int orc_grad_faces(const orc_grid* g, const double* phi, double* fx, double* fy, double* fz);
 /* Value divergence of face arrays into padded cell layout. Divergence_Cartesian.inl:211-235. */
 int orc_div_faces(const orc_grid* g, const double* fx, const double* fy, const double* fz, double* out);
+/* Cell-centred velocity fields (padded, 1 component each) -> face arrays, the cell-centred
+ * source of the TVD constructor (TVD_Cartesian.inl:27-38 -> InterpolateToFace,
+ * Interpolation_Cartesian.h:298-354): face value (u[m-1] + u[m]) * 0.5. */
+int orc_interpolate_to_faces(const orc_grid* g, const double* ux, const double* uy, const double* uz,
+                             double* fx, double* fy, double* fz);
 /* TVD face flux phi_f * u_f per face. TVD_Cartesian.inl:27-38,122-183 + Stencils_Cartesian.inl:652-681. */
 int orc_tvd_face_flux(const orc_grid* g, int32_t limiter, const double* phi,
                       const double* ux, const double* uy, const double* uz,

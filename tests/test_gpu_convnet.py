@@ -231,3 +231,30 @@ def test_save_load_roundtrip(ctx, tmp_path):
     assert np.array_equal(net2.params(), w)
     net.close()
     net2.close()
+
+
+def test_adam_step_matches_oracle(ctx, rng):
+    """dare_conv_net_step against the oracle's float32 Adam (oracle/convnet.py adam) on two
+    consecutive steps with the gradients the backward pass produced: the FP32 master weights
+    agree to one float32 rounding."""
+    import torch
+    net, g, dev, (phi, phim, rhs, ext), w, extras = setup(ctx, rng)
+    base = dev[0].view(*[m + 4 for m in N])[2:-2, 2:-2, 2:-2].contiguous().view(-1)
+    x_true = torch.from_numpy(rng.normal(size=int(np.prod(N)))).cuda()
+    w0 = net.params()
+    m = np.zeros_like(w0)
+    v = np.zeros_like(w0)
+    for t in (1, 2):
+        net.pack(g, dev[0], dev[1], dev[2], 0.7, extras)
+        net.forward()
+        net.loss(g, base, x_true)
+        net.backward()
+        gr = net.grads()
+        net.step(lr=2e-3, beta1=0.9, beta2=0.999, eps=1e-8)
+        w1 = net.params()
+        we, m, v = cn.adam(w0, gr, m, v, t, 2e-3, 0.9, 0.999, 1e-8)
+        tol = 2 * np.finfo(np.float32).eps * (np.abs(we.astype(np.float64)) + 2e-3)
+        assert np.all(np.abs(w1.astype(np.float64) - we.astype(np.float64)) <= tol), t
+        assert np.any(w1 != w0)
+        w0 = w1
+    net.close()

@@ -104,3 +104,25 @@ def test_bf16_rounding_is_round_to_nearest_even():
     # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: even mantissa -> 1; 1 + 3*2^-8 -> 1 + 2^-6
     assert got[0] == 1.0 and got[1] == 1.0 and got[2] == 1.0 + 2 ** -6 and got[3] == 1.0
     assert got[4] == -2.5 and got[5] == 0.0
+
+
+def test_adam_oracle_first_step_and_moments():
+    """The first Adam step moves every weight by lr g / (|g| + eps) (bias-corrected moments
+    are g and g^2), and a constant gradient keeps every later step at that size; a zero
+    gradient leaves the weight unchanged."""
+    rng = np.random.default_rng(3)
+    w0 = rng.normal(size=1000).astype(np.float32)
+    g = rng.normal(size=1000).astype(np.float32)
+    g[:10] = 0.0
+    lr = 1e-3
+    w1, m, v = cn.adam(w0, g, np.zeros_like(g), np.zeros_like(g), 1, lr)
+    step = w0.astype(np.float64) - w1.astype(np.float64)
+    exp = lr * g / (np.abs(g) + 1e-8)
+    ulp = 2 * np.finfo(np.float32).eps * (np.abs(w0) + lr)     # rounding of the float32 update
+    assert np.all(np.abs(step - exp) <= ulp)
+    assert np.all(w1[:10] == w0[:10])
+    w, mm, vv = w1, m, v
+    for t in range(2, 6):
+        w_new, mm, vv = cn.adam(w, g, mm, vv, t, lr)
+        assert np.all(np.abs(w.astype(np.float64) - w_new.astype(np.float64) - exp) <= ulp + 1e-5 * lr)
+        w = w_new

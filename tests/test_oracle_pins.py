@@ -456,3 +456,55 @@ def test_pins_catch_planted_mistakes(mutant, mutant_libs):
             except (AssertionError, np.linalg.LinAlgError):
                 err = np.inf          # the mutant broke the oracle's own consistency checks
             assert err > 1e3 * TOL, (mutant, name, err)
+
+
+# ----------------------------------------------------------------------------- face interpolation
+def test_interpolate_to_scalar_face_kat():
+    """InterpolateToScalarFaceTest (Grid/test/test_integration_Cartesian_Interpolation.cpp:
+    1075-1260) for a cell-centred source and target: at cell (2, 3, 4) the WEST face value is
+    0.5 (f(ind) + f(ind - e_i)) and the TOP face value 0.5 (f(ind) + f(ind + e_k)), within
+    1e3 eps, with f drawn as 1e-4 x an integer in [-10000, 10000]. Every other face too, and
+    bit for bit against the reference's order (neighbour + cell) x Divisor<2>."""
+    rng = np.random.default_rng(14)
+    g = orc.grid([5, 6, 7], ghost=2)
+    shape = padded_shape(g)
+    u = [1e-4 * rng.integers(-10000, 10001, size=int(np.prod(shape))).astype(float) for _ in range(3)]
+    f = orc.interpolate_to_faces(g, u)
+    eps = np.finfo(float).eps
+    ind = (2, 3, 4)                                      # internal index
+    for d, face_hi in ((0, False), (2, True)):           # WEST, TOP
+        U = u[d].reshape(shape)
+        p = tuple(i + 2 for i in ind)
+        nb = list(p)
+        nb[d] += 1 if face_hi else -1
+        v_ex = 0.5 * (U[p] + U[tuple(nb)])
+        q = list(ind)
+        if face_hi:
+            q[d] += 1
+        fshape = [g.n[a] + (1 if a == d else 0) for a in range(3)]
+        v = f[d].reshape(fshape)[tuple(q)]
+        assert abs(v - v_ex) <= 1e3 * eps * abs(v_ex)
+    for d in range(3):
+        U = u[d].reshape(shape)
+        fshape = [g.n[a] + (1 if a == d else 0) for a in range(3)]
+        F = f[d].reshape(fshape)
+        for q in np.ndindex(*fshape):
+            hi = tuple(q[a] + 2 for a in range(3))
+            lo = list(hi)
+            lo[d] -= 1
+            assert F[q] == (U[tuple(lo)] + U[hi]) * 0.5
+
+
+def test_interpolated_linear_velocity_is_exact_at_faces():
+    """A velocity linear in the coordinate along its axis is reproduced exactly (to rounding)
+    at the face centres: the two-point mean of a linear function is its midpoint value."""
+    g = orc.grid([6, 5, 4], size=[1.0, 0.5, 2.0], ghost=2)
+    xc = orc.cell_centers(g)
+    u = [(0.3 + 1.7 * xc[..., d]).ravel().copy() for d in range(3)]
+    f = orc.interpolate_to_faces(g, u)
+    for d in range(3):
+        dx = g.size[d] / g.n[d]
+        fshape = [g.n[a] + (1 if a == d else 0) for a in range(3)]
+        xf = np.arange(fshape[d]) * dx
+        exp = np.broadcast_to((0.3 + 1.7 * xf).reshape([-1 if a == d else 1 for a in range(3)]), fshape)
+        assert np.max(np.abs(f[d].reshape(fshape) - exp)) <= 1e-14 * np.max(np.abs(exp))
