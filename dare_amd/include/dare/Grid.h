@@ -11,6 +11,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "Vector.h"
 #include "dare_hip.h"
 
 namespace dare::Grid {
@@ -18,6 +19,12 @@ namespace dare::Grid {
 enum class CartesianNeighbor : char {   // CartesianMesh.h:67-77
     CENTER = 0, WEST = 1, EAST = 2, SOUTH = 3, NORTH = 4, BOTTOM = 5, TOP = 6,
 };
+
+// position helpers of CartesianMesh.h:79-120: stencil slot, face slot (W = 0), outward normal
+inline char ToNum(CartesianNeighbor nb) { return static_cast<char>(nb); }
+inline char ToFace(CartesianNeighbor nb) { return static_cast<char>(static_cast<char>(nb) - 1); }
+inline char ToNormal(CartesianNeighbor nb) { return (static_cast<char>(nb) % 2 == 1) ? -1 : 1; }
+inline CartesianNeighbor ToCartesianNeighbor(std::size_t id) { return static_cast<CartesianNeighbor>(id); }
 
 template <std::size_t Dim>
 class CartesianRepresentation;
@@ -28,13 +35,17 @@ public:
     using LocalOrdinalType = int32_t;
     using GlobalOrdinalType = int64_t;
     using ScalarType = double;
-    using Index = std::array<int32_t, Dim>;
-    using IndexGlobal = std::array<int64_t, Dim>;
-    using VecSC = std::array<double, Dim>;
-    using VecLO = std::array<int32_t, Dim>;
+    using Index = utils::Vector<Dim, int32_t>;
+    using IndexGlobal = utils::Vector<Dim, int64_t>;
+    using VecSC = utils::Vector<Dim, double>;
+    using VecLO = utils::Vector<Dim, int32_t>;
+    using VecGO = utils::Vector<Dim, int64_t>;
     using Representation = CartesianRepresentation<Dim>;
     using NeighborID = CartesianNeighbor;
     using Options = VecLO;    // per axis: 0 cell centred, != 0 staggered
+    static constexpr std::size_t DIMENSION = Dim;
+    static constexpr std::size_t STENCIL_SIZE = Dim * 2 + 1;   // CartesianMesh.h:202-203
+    static constexpr std::size_t NUM_FACES = Dim * 2;
 
     Cartesian(const IndexGlobal& res, const VecSC& size, int32_t num_ghost, const VecLO& periodic = VecLO{})
         : res_(res), size_(size), num_ghost_(num_ghost), periodic_(periodic) {
@@ -80,7 +91,9 @@ public:
     using GridType = Cartesian<Dim>;
     using LO = int32_t;
     using GO = int64_t;
+    using SC = double;
     using Index = typename GridType::Index;
+    using IndexGlobal = typename GridType::IndexGlobal;
     using VecSC = typename GridType::VecSC;
     using Options = typename GridType::Options;
 
@@ -159,6 +172,10 @@ public:
         return ind;
     }
     GO MapLocalToGlobalInternal(LO n) const { return n; }          // one rank: global == local
+    // CartesianRepresentation.inl:617-623: index + the rank's cell offset, which is 0 on one rank
+    IndexGlobal MapLocalToGlobal(const Index& ind) const { return IndexGlobal(ind); }
+    Index MapGlobalToLocal(const IndexGlobal& ind) const { return Index(ind); }
+    Index GetOffsetCells() const { return Index{}; }
     Index MapOrdinalToIndexGlobalInternal(GO n) const { return MapOrdinalToIndexLocalInternal(static_cast<LO>(n)); }
     LO MapGlobalToLocalInternal(GO n) const { return static_cast<LO>(n); }
     // MapIndexToOrdinalGlobalInternal with the periodic wrap of :418-450

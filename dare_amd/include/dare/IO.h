@@ -81,7 +81,7 @@ class VTKWriter {
 public:
     using Representation = typename Grid::Representation;
     using LO = typename Grid::LocalOrdinalType;
-    static constexpr std::size_t Dim = std::tuple_size<typename Grid::Index>::value;
+    static constexpr std::size_t Dim = Grid::DIMENSION;
 
     // time < 0: no TimeValue entry (VTKWriter.inl:143-152)
     VTKWriter(double time, int step) : time_(time), step_(step) {}

@@ -21,6 +21,11 @@
 #include <vector>
 
 #include "Data.h"
+#include "Stencils.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace dare {
 
@@ -66,7 +71,7 @@ public:
     using Index = typename Grid::Index;
     using O = int64_t;
     using SC = double;
-    static constexpr std::size_t Dim = std::tuple_size<Index>::value;
+    static constexpr std::size_t Dim = Grid::DIMENSION;
 
     MatrixBlock() = default;
     void Initialize(const GridRepresentation* g, O node) {
@@ -112,6 +117,41 @@ public:
     SC& GetInitialGuess(std::size_t n) { return guess_[n]; }
     SC GetInitialGuess(std::size_t n) const { return guess_[n]; }
     void SetInitialGuess(std::size_t n, SC v) { guess_[n] = v; }
+
+    // ---- stencils (MatrixBlock_Cartesian.inl:487-593): every position of the stencil is
+    // written through Get (which marks it set), centre first, then W, E, S, N, B, T
+    using CenterStencil = Data::CenterMatrixStencil<Grid, SC, N>;
+    using CenterSlice = Data::CenterMatrixStencil<Grid, SC, 1>;
+    MatrixBlock& Set(const CenterStencil& s) {
+        for (std::size_t n = 0; n < N; n++) {
+            for (std::size_t p = 0; p < 2 * Dim + 1; p++) Get(n, static_cast<CartesianNeighbor>(p)) = s.GetData()[n][p];
+            GetRhs(n) = s.GetRHS()[n];
+        }
+        return *this;
+    }
+    MatrixBlock& Set(std::size_t n, const CenterSlice& s) {
+        for (std::size_t p = 0; p < 2 * Dim + 1; p++) Get(n, static_cast<CartesianNeighbor>(p)) = s.GetData()[0][p];
+        GetRhs(n) = s.GetRHS()[0];
+        return *this;
+    }
+    MatrixBlock& Add(const CenterStencil& s) {
+        for (std::size_t n = 0; n < N; n++) {
+            for (std::size_t p = 0; p < 2 * Dim + 1; p++) Get(n, static_cast<CartesianNeighbor>(p)) += s.GetData()[n][p];
+            GetRhs(n) += s.GetRHS()[n];
+        }
+        return *this;
+    }
+    MatrixBlock& Add(std::size_t n, const CenterSlice& s) {
+        for (std::size_t p = 0; p < 2 * Dim + 1; p++) Get(n, static_cast<CartesianNeighbor>(p)) += s.GetData()[0][p];
+        GetRhs(n) += s.GetRHS()[0];
+        return *this;
+    }
+    // Subtract(s) = Add(-1 * s): coefficients and right-hand side negated first (:576-593)
+    MatrixBlock& Subtract(const CenterStencil& s) { return Add(-1. * s); }
+    MatrixBlock& Subtract(std::size_t n, const CenterSlice& s) { return Add(n, -1. * s); }
+    MatrixBlock& operator=(const CenterStencil& s) { return Set(s); }
+    MatrixBlock& operator+=(const CenterStencil& s) { return Add(s); }
+    MatrixBlock& operator-=(const CenterStencil& s) { return Add(-1. * s); }
 
     // MatrixBlockBase: explicit entries
     void Resize(std::size_t n, std::size_t size) {
@@ -313,7 +353,7 @@ public:
     using MatrixType = DeviceCsr;
     using VecType = DeviceVector;
 
-    explicit Trilinos(std::shared_ptr<Device> dev = Device::Default()) : dev_(std::move(dev)) {}
+    explicit Trilinos(std::shared_ptr<Device> dev = nullptr) : dev_(std::move(dev)) {}
 
     // device fast path (the functor is `terms`); structure built once, values re-filled
     // in place on later calls (BuildReplace semantics, Trilinos.inl:389-457)
@@ -323,56 +363,115 @@ public:
         const dare_grid g = field.Descriptor();
         const std::size_t nrows = static_cast<std::size_t>(grep.GetNumberLocalCellsInternal()) * N;
         if (!A_ || !structured_ || rebuild || nrows != x_->getLocalLength()) {
-            A_ = std::make_shared<DeviceCsr>(dev_);
-            Check(dare_csr_build_structure(dev_->Get(), A_->Get(), &g, &terms.t), "dare_csr_build_structure");
-            x_ = std::make_shared<DeviceVector>(dev_, nrows);
-            B_ = std::make_shared<DeviceVector>(dev_, nrows);
+            A_ = std::make_shared<DeviceCsr>(dev());
+            Check(dare_csr_build_structure(dev()->Get(), A_->Get(), &g, &terms.t), "dare_csr_build_structure");
+            x_ = std::make_shared<DeviceVector>(dev(), nrows);
+            B_ = std::make_shared<DeviceVector>(dev(), nrows);
             structured_ = true;
         }
-        Check(dare_csr_fill_values(dev_->Get(), A_->Get(), &g, &terms.t, field.GetDeviceView(), B_->data(),
+        Check(dare_csr_fill_values(dev()->Get(), A_->Get(), &g, &terms.t, field.GetDeviceView(), B_->data(),
                                    x_->data()),
               "dare_csr_fill_values");
     }
 
-    // generic path: any functor on a host MatrixBlock (Trilinos.inl:287-385)
+    // The rows of a generic functor, assembled on the host: per-cell MatrixBlock, Finalize, rows
+    // sorted with equal columns summed (what fillComplete leaves), x0 from the field
+    // (Trilinos.inl:287-385). The functor runs concurrently on OpenMP threads, as in the
+    // reference (Trilinos.inl:308), so it must be thread-safe; each thread owns a contiguous
+    // range of cells and the rows are joined in cell order, so the result does not depend on
+    // the thread count.
+    struct HostRows {
+        std::vector<int32_t> row_ptr, col;
+        std::vector<double> val, rhs, x0;
+    };
+    template <typename Grid, std::size_t N, typename Lambda>
+    static HostRows AssembleHost(const typename Grid::Representation& grep, Data::GridVector<Grid, SC, N>& field,
+                                 Lambda functor) {
+        field.SyncToHost();
+        const int32_t ncell = grep.GetNumberLocalCellsInternal();
+        const int64_t nrows = static_cast<int64_t>(ncell) * N;
+        int nth = 1;
+#ifdef _OPENMP
+        nth = omp_get_max_threads();
+        if (nth > ncell) nth = ncell > 0 ? ncell : 1;
+#endif
+        struct Part {
+            std::vector<int32_t> len, col;
+            std::vector<double> val;
+            std::string error;
+        };
+        std::vector<Part> parts(nth);
+        HostRows out;
+        out.rhs.resize(nrows);
+        out.x0.resize(nrows);
+#pragma omp parallel num_threads(nth)
+        {
+            int t = 0;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            Part& P = parts[t];
+            const int32_t lo = static_cast<int32_t>(static_cast<int64_t>(ncell) * t / nth);
+            const int32_t hi = static_cast<int32_t>(static_cast<int64_t>(ncell) * (t + 1) / nth);
+            try {
+                MatrixBlock<Grid, N> mb;
+                std::vector<std::pair<int64_t, double>> row;
+                for (int32_t node = lo; node < hi; node++) {
+                    mb.Initialize(&grep, node);
+                    const int32_t full = grep.MapInternalToLocal(node);
+                    for (std::size_t n = 0; n < N; n++) mb.SetInitialGuess(n, static_cast<const Data::GridVector<Grid, SC, N>&>(field).At(full, n));
+                    functor(&mb);
+                    mb.Finalize();
+                    for (std::size_t n = 0; n < N; n++) {
+                        const std::size_t r = static_cast<std::size_t>(node) * N + n;
+                        mb.SortedRow(n, &row);
+                        for (const auto& e : row) {
+                            if (e.first < 0 || e.first >= nrows)
+                                throw std::runtime_error("functor produced a column outside the matrix");
+                            P.col.push_back(static_cast<int32_t>(e.first));
+                            P.val.push_back(e.second);
+                        }
+                        P.len.push_back(static_cast<int32_t>(row.size()));
+                        out.rhs[r] = mb.GetRhs(n);
+                        out.x0[r] = mb.GetInitialGuess(n);
+                    }
+                }
+            } catch (const std::exception& e) {
+                P.error = e.what();
+            }
+        }
+        std::size_t nnz = 0;
+        for (const Part& P : parts) {
+            if (!P.error.empty()) throw std::runtime_error(P.error);
+            nnz += P.col.size();
+        }
+        if (nnz >= (std::size_t(1) << 31)) throw std::runtime_error("nnz exceeds 32-bit row pointers");
+        out.row_ptr.reserve(nrows + 1);
+        out.row_ptr.push_back(0);
+        out.col.reserve(nnz);
+        out.val.reserve(nnz);
+        for (const Part& P : parts) {
+            for (int32_t l : P.len) out.row_ptr.push_back(out.row_ptr.back() + l);
+            out.col.insert(out.col.end(), P.col.begin(), P.col.end());
+            out.val.insert(out.val.end(), P.val.begin(), P.val.end());
+        }
+        return out;
+    }
+
+    // generic path: any functor on a host MatrixBlock, uploaded to the device (Trilinos.inl:287-385)
     template <typename Grid, std::size_t N, typename Lambda>
     void Build(const typename Grid::Representation& grep, Data::GridVector<Grid, SC, N>& field, Lambda functor,
                bool = false) {
-        field.SyncToHost();
-        const int32_t ncell = grep.GetNumberLocalCellsInternal();
-        const std::size_t nrows = static_cast<std::size_t>(ncell) * N;
-        std::vector<int32_t> rp(nrows + 1, 0), col;
-        std::vector<double> val, rhs(nrows), x0(nrows);
-        std::vector<std::pair<int64_t, double>> row;
-        MatrixBlock<Grid, N> mb;
-        for (int32_t node = 0; node < ncell; node++) {
-            mb.Initialize(&grep, node);
-            const int32_t full = grep.MapInternalToLocal(node);
-            for (std::size_t n = 0; n < N; n++) mb.SetInitialGuess(n, field.At(full, n));
-            functor(&mb);
-            mb.Finalize();
-            for (std::size_t n = 0; n < N; n++) {
-                const std::size_t r = static_cast<std::size_t>(node) * N + n;
-                mb.SortedRow(n, &row);
-                for (const auto& e : row) {
-                    if (e.first < 0 || e.first >= static_cast<int64_t>(nrows))
-                        throw std::runtime_error("functor produced a column outside the matrix");
-                    col.push_back(static_cast<int32_t>(e.first));
-                    val.push_back(e.second);
-                }
-                rp[r + 1] = static_cast<int32_t>(col.size());
-                rhs[r] = mb.GetRhs(n);
-                x0[r] = mb.GetInitialGuess(n);
-            }
-        }
-        A_ = std::make_shared<DeviceCsr>(dev_);
-        Check(dare_csr_upload_host(dev_->Get(), A_->Get(), static_cast<int64_t>(nrows),
-                                   static_cast<int64_t>(col.size()), rp.data(), col.data(), val.data()),
+        HostRows h = AssembleHost(grep, field, functor);
+        const std::size_t nrows = h.rhs.size();
+        A_ = std::make_shared<DeviceCsr>(dev());
+        Check(dare_csr_upload_host(dev()->Get(), A_->Get(), static_cast<int64_t>(nrows),
+                                   static_cast<int64_t>(h.col.size()), h.row_ptr.data(), h.col.data(), h.val.data()),
               "dare_csr_upload_host");
-        x_ = std::make_shared<DeviceVector>(dev_, nrows);
-        B_ = std::make_shared<DeviceVector>(dev_, nrows);
-        x_->Buffer().Upload(x0.data());
-        B_->Buffer().Upload(rhs.data());
+        x_ = std::make_shared<DeviceVector>(dev(), nrows);
+        B_ = std::make_shared<DeviceVector>(dev(), nrows);
+        x_->Buffer().Upload(h.x0.data());
+        B_->Buffer().Upload(h.rhs.data());
         structured_ = false;
     }
 
@@ -383,14 +482,18 @@ public:
 
     // Trilinos.inl:168-177: x[node*N + n] -> field.At(MapInternalToLocal(node), n), on the device
     template <typename Grid, std::size_t N>
-    void CopyTo(Data::GridVector<Grid, SC, N>* gvec) const {
+    void CopyTo(Data::GridVector<Grid, SC, N>* gvec) {
         const dare_grid g = gvec->Descriptor();
-        Check(dare_copy_to_field(dev_->Get(), &g, x_->data(), gvec->GetDeviceView()), "dare_copy_to_field");
-        dev_->Synchronize();
+        Check(dare_copy_to_field(dev()->Get(), &g, x_->data(), gvec->GetDeviceView()), "dare_copy_to_field");
+        dev()->Synchronize();
         gvec->SyncToHost();
     }
 
 private:
+    const std::shared_ptr<Device>& dev() {
+        if (!dev_) dev_ = Device::Default();
+        return dev_;
+    }
     std::shared_ptr<Device> dev_;
     std::shared_ptr<MatrixType> A_;
     std::shared_ptr<VecType> x_, B_;
