@@ -15,6 +15,7 @@ of the reference algorithm) run on the host cores for one full 256^3 implicit st
 """
 import argparse
 import json
+import subprocess
 import os
 import sys
 import time
@@ -147,26 +148,38 @@ def main():
             traffic = json.load(open(tpath)).get("bytes_per_launch")
         except Exception:
             traffic = None
-    # attainable HBM bandwidth on this device: a device-to-device copy of 1 GiB, same timing
+    # attainable HBM bandwidth on this device, same timing, from the library's own streaming
+    # probe (dare_stream_probe): a 1 GiB copy, and a read-heavy mix (8 reads per write, 1 GiB
+    # read) like the SpMV's 9:1; the 1 GiB torch copy and fill stay as points of comparison
     src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    src.fill_(1.0)
     dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    copy_gbs = 2 * src.numel() * 8 / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
-    # and the write-only rate (1 GiB fill), the ceiling of write-heavy kernels like the gradient
-    e0.record()
-    for _ in range(reps):
-        dst.fill_(1.0)
-    e1.record()
-    torch.cuda.synchronize()
-    write_gbs = dst.numel() * 8 / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
-    del src, dst
+    dst8 = torch.empty(src.numel() // 8, dtype=torch.float64, device="cuda")
+
+    def probe(fn, nbytes):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return nbytes / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
+
+    stream_copy_gbs = probe(lambda: da.stream_probe(ctx, src, dst, ratio=1), 2 * src.numel() * 8)
+    stream_read8_gbs = probe(lambda: da.stream_probe(ctx, src, dst8, ratio=8), (src.numel() + dst8.numel()) * 8)
+    copy_gbs = probe(lambda: dst.copy_(src), 2 * src.numel() * 8)
+    write_gbs = probe(lambda: dst.fill_(1.0), dst.numel() * 8)
+    del src, dst, dst8
+
+    # the contract-format SpMV (dare_spmv on a CSR without a diagonal view: k_apply_csr), on the
+    # same 256^3 operator uploaded as a plain CSR: 12 nnz + 4 (n + 1) + 8 n (x) + 8 n (y) bytes
+    rp_h, col_h, val_h = A.download()
+    C = da.Csr(ctx)
+    da.upload_host_csr(ctx, C, rp_h, col_h, val_h)
+    del rp_h, col_h, val_h
+    yc = torch.empty_like(x)
 
     # the stencil kernels, same method: TVD step (phi + 3 face velocities in, phi out), face
     # gradient (phi in, 3 face arrays out), value divergence (3 face arrays in, cells out)
@@ -188,6 +201,9 @@ def main():
     gen = torch.Generator(device="cuda")
     gen.manual_seed(2024)
     phi_u = torch.rand(W.phi.numel(), generator=gen, dtype=torch.float64, device="cuda")
+    csr_s = timed(lambda: da.spmv(ctx, C, x, yc))
+    C.close()
+    del yc
     tvd_s = timed(lambda: da.tvd_explicit_step(ctx, g, da.VANLEER, phi_u, W.vel, W.dt, W.phi_work))
     tvd_bytes = 8 * (2 * ncell + nface)
     faces = [torch.empty(v.numel(), dtype=torch.float64, device="cuda") for v in W.vel]
@@ -195,9 +211,11 @@ def main():
     div_s = timed(lambda: da.div_faces(ctx, g, faces, W.phi_work))
     del faces, phi_u
     stencils = {}
+    csr_bytes = 12 * nnz + 4 * (nrows + 1) + 16 * nrows
     for name, kern, sec, nbytes in (("tvd_step", "k_tvd_step3", tvd_s, tvd_bytes),
-                                    ("gradient", "k_grad_faces", grad_s, 8 * (ncell + nface)),
-                                    ("divergence", "k_div_faces", div_s, 8 * (nface + ncell))):
+                                    ("gradient", "k_grad3_pair", grad_s, 8 * (ncell + nface)),
+                                    ("divergence", "k_div3", div_s, 8 * (nface + ncell)),
+                                    ("csr_spmv", "k_apply_csr<double,double,EpiStore>", csr_s, csr_bytes)):
         gbs = nbytes / sec / 1e9
         stencils[name] = {"kernel": kern, "bytes_per_cell": round(nbytes / ncell, 3), "achieved": round(gbs, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
@@ -248,9 +266,10 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": spmv_bytes, "bytes_per_row": 80, "avg_launch_us": round(spmv_s * 1e6, 2),
-            "attainable_copy_GBs": round(copy_gbs, 1),
-            "attainable_write_GBs": round(write_gbs, 1),
-            "frac_of_attainable": round(achieved / copy_gbs, 4),
+            "attainable": {"probe": "dare_stream_probe (own kernel, 1 GiB read)",
+                           "copy_GBs": round(stream_copy_gbs, 1), "read8_write1_GBs": round(stream_read8_gbs, 1),
+                           "torch_copy_GBs": round(copy_gbs, 1), "torch_fill_GBs": round(write_gbs, 1)},
+            "frac_of_attainable": round(achieved / max(stream_read8_gbs, stream_copy_gbs), 4),
         },
         "stencil_rooflines": stencils,
         "cpu_baseline": cpu,
@@ -297,10 +316,13 @@ def target_leg(ctx, n, steps, inner_tol=1e-4):
         net = WarmStartNet(ctx, [n] * 3, in_channels=meta["in_channels"], params=w)
     # (name, mixed, precond, guess)
     forms = [("fp64", False, 0, "previous"), ("fp64+jacobi", False, 1, "previous"),
+             ("fp64+mg", False, 2, "previous"),
              ("mixed", True, 0, "previous"), ("mixed+jacobi", True, 1, "previous"),
-             ("mixed+jacobi+extrapolated", True, 1, "extrapolated")]
+             ("mixed+mg", True, 2, "previous"),
+             ("mixed+jacobi+extrapolated", True, 1, "extrapolated"), ("mixed+mg+extrapolated", True, 2, "extrapolated")]
     if net:
-        forms += [("mixed+learned", True, 0, "learned"), ("mixed+jacobi+learned", True, 1, "learned")]
+        forms += [("mixed+learned", True, 0, "learned"), ("mixed+jacobi+learned", True, 1, "learned"),
+                  ("mixed+mg+learned", True, 2, "learned")]
     st = {f[0]: {"ms": [], "passes": [], "refine": [], "rel": [], "guess_ms": [], "rel0": []} for f in forms}
     xg = torch.empty_like(W.x)
     for k in range(steps + 1):
@@ -383,7 +405,8 @@ def precision_leg(ctx, n=384):
     xr = x0.clone()
     da.bicgstab2(ctx, A, rhs, xr, tol=1e-13, tol_kind=1, max_it=5000, precond=1)
     out = {"workload": f"{n}^3 variable-coefficient diffusion step, to true rel residual 1e-10"}
-    for name, mixed, pre in (("fp64", False, 0), ("mixed", True, 0), ("mixed+jacobi", True, 1)):
+    for name, mixed, pre in (("fp64", False, 0), ("mixed", True, 0), ("mixed+jacobi", True, 1), ("fp64+mg", False, 2),
+                             ("mixed+mg", True, 2)):
         ts = []
         for rep in range(2):
             x = x0.clone()
@@ -400,7 +423,7 @@ def precision_leg(ctx, n=384):
         out[name] = {"ms": round(1e3 * min(ts), 2), "passes": r["iterations"], "refinements": r["refinements"],
                      "converged": r["converged"], "true_rel_residual": _true_rel(ctx, A, rhs, x),
                      "rel_error_vs_fp64_reference": err}
-    for name in ("mixed", "mixed+jacobi"):
+    for name in ("mixed", "mixed+jacobi", "fp64+mg", "mixed+mg"):
         out[name]["speedup_vs_fp64"] = round(out["fp64"]["ms"] / out[name]["ms"], 3)
     A.close()
     return out
@@ -488,45 +511,66 @@ def learned_leg(ctx, seeds=2, steps=30):
     return {"workload": f"128^3 advection-diffusion, {seeds} held-out seeds x {steps} steps", **ev}
 
 
+def _host_info():
+    """lscpu's description of the host (model name, sockets, cores, threads), for the record."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keep = {"Model name": "model", "Vendor ID": "vendor", "Socket(s)": "sockets",
+                "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core",
+                "CPU(s)": "cpus", "NUMA node(s)": "numa_nodes", "CPU max MHz": "max_mhz"}
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() in keep:
+                info[keep[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def cpu_baseline(args, W, ctx):
     """The oracle (C++ restatement of the reference algorithm) on the host: one full implicit
-    step of the same workload (assembly + BiCGStab2 to 1e-8 relative)."""
+    step of the same workload (assembly + BiCGStab2 to 1e-8 relative). One warm-up step, then
+    the median of five timed steps on all host threads (up to 16, the box's share), and the
+    median of three on one thread."""
     import numpy as np
     import torch
     import oracle as orc
     import dare_amd as da
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    orc.set_threads(threads)
-    log(f"[bench] cpu baseline: oracle, {threads} threads, {args.n}^3 implicit step")
+    log(f"[bench] cpu baseline: oracle, {threads} threads, {args.n}^3 implicit step, 1 + 5 runs")
     da.ghost_fill(ctx, W.grid, W.phi, W.ghost_modes, W.ghost_values)
     da.tvd_explicit_step(ctx, W.grid, da.VANLEER, W.phi, W.vel, W.dt, W.phi_work)
     torch.cuda.synchronize()
     phi = W.phi_work.cpu().numpy()
     go = orc.grid([args.n] * 3, ghost=2)
     t = orc.terms(dt=W.dt, diff=W.D, bc_type=[1] * 6, bc_value=[0.0] * 6)
-    t0 = time.perf_counter()
-    rp, col, val, rhs, x0 = orc.assemble(go, t, phi)
-    t1 = time.perf_counter()
-    x, ok, it, res, _ = orc.bicgstab2(rp, col, val, rhs, x0, TOL * float(np.linalg.norm(rhs)), 1000)
-    t2 = time.perf_counter()
-    # the same step on one thread (the survey asks for both), and what the host is
-    orc.set_threads(1)
-    t3 = time.perf_counter()
-    rp, col, val, rhs, x0 = orc.assemble(go, t, phi)
-    orc.bicgstab2(rp, col, val, rhs, x0, TOL * float(np.linalg.norm(rhs)), 1000)
-    t4 = time.perf_counter()
+
+    def one_step():
+        t0 = time.perf_counter()
+        rp, col, val, rhs, x0 = orc.assemble(go, t, phi)
+        t1 = time.perf_counter()
+        x, ok, it, res, _ = orc.bicgstab2(rp, col, val, rhs, x0, TOL * float(np.linalg.norm(rhs)), 1000)
+        t2 = time.perf_counter()
+        return t2 - t0, t1 - t0, t2 - t1, it, ok
+
     orc.set_threads(threads)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
-    return {"value": round(1e3 * (t2 - t0), 1), "unit": "ms", "cores": threads, "kind": "port",
-            "sample": f"one full {args.n}^3 implicit step: assembly {1e3 * (t1 - t0):.0f} ms + "
-                      f"BiCGStab2 {1e3 * (t2 - t1):.0f} ms ({it} passes, converged={ok})",
-            "single_thread_ms": round(1e3 * (t4 - t3), 1),
-            "host": {"model": model, "nproc": os.cpu_count()}}
+    one_step()                                   # warm-up
+    runs = [one_step() for _ in range(5)]
+    tot = sorted(r[0] for r in runs)
+    med = runs[[r[0] for r in runs].index(tot[2])]
+    orc.set_threads(1)
+    log("[bench] cpu baseline: one thread, 3 runs")
+    single = sorted(one_step()[0] for _ in range(3))
+    orc.set_threads(threads)
+    return {"value": round(1e3 * tot[2], 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"one full {args.n}^3 implicit step (the median of 5 after 1 warm-up): assembly "
+                      f"{1e3 * med[1]:.0f} ms + BiCGStab2 {1e3 * med[2]:.0f} ms ({med[3]} passes, converged={med[4]})",
+            "runs_ms": [round(1e3 * v, 1) for v in tot],
+            "spread_ms": round(1e3 * (tot[-1] - tot[0]), 1),
+            "single_thread_ms": round(1e3 * single[1], 1),
+            "single_thread_runs_ms": [round(1e3 * v, 1) for v in single],
+            "host": _host_info()}
 
 
 if __name__ == "__main__":

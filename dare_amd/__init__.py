@@ -18,6 +18,7 @@ CDS, UPWIND, VANALBADA, MINMOD, VANLEER = 0, 1, 2, 3, 4
 LIMITERS = {"CDS": CDS, "UPWIND": UPWIND, "VANALBADA": VANALBADA, "MINMOD": MINMOD, "VANLEER": VANLEER}
 GHOST_LEAVE, GHOST_PERIODIC, GHOST_DIRICHLET, GHOST_NEUMANN, GHOST_CONSTANT = 0, 1, 2, 3, 4
 BC_NONE, BC_DIRICHLET, BC_NEUMANN, BC_FIXED = 0, 1, 2, 3
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
 
 DARE_OK = 0
 DARE_ERR_BREAKDOWN = -5
@@ -182,6 +183,7 @@ def lib():
         "dare_memcpy_d2h": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "dare_memcpy_d2d": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "dare_fill_f64": (i32, [vp, vp, i64, dbl]),
+        "dare_stream_probe": (i32, [vp, i64, vp, vp, i32, i32, i32]),
         "dare_ghost_fill": (i32, [vp, G, vp, vp, vp]),
         "dare_grad_faces": (i32, [vp, G, vp, vp, vp, vp]),
         "dare_div_faces": (i32, [vp, G, vp, vp, vp, vp]),
@@ -302,6 +304,15 @@ class Csr:
 
 
 # ---------------------------------------------------------------- thin wrappers (torch tensors)
+
+def stream_probe(ctx, src, dst, ratio=1, grid=0, layout=0):
+    """dst[j] = sum of `ratio` elements of src (src holds ratio * dst.numel()), through the
+    library's streaming kernel: the attainable-bandwidth probe (ratio 1 = copy; layout 0 XCD
+    slabs, 1 grid stride, 2 XCD slabs over `ratio` separate arrays)."""
+    assert src.numel() == ratio * dst.numel()
+    check(lib().dare_stream_probe(ctx.h, dst.numel(), _ptr(src), _ptr(dst), ratio, grid, layout),
+          "dare_stream_probe")
+
 
 def ghost_fill(ctx, grid, phi, modes, values=None):
     m = (ctypes.c_int32 * 6)(*list(modes) + [0] * (6 - len(modes)))

@@ -597,6 +597,7 @@ int dare_csr_destroy(dare_csr* A) {
     if (A->asym32) hipFree(A->asym32);
     if (A->dinv) hipFree(A->dinv);
     if (A->dia_pre) hipFree(A->dia_pre);
+    mg_free(A->mg);
     delete A;
     return DARE_OK;
 }

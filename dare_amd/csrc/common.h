@@ -127,6 +127,7 @@ struct dare_csr {
     int bc_type[6] = {0, 0, 0, 0, 0, 0};
     int cross = 0;
     int vals_stale = 0;          // 1: CSR values not yet written for the last fill
+    void* mg = nullptr;          // multigrid hierarchy of the right preconditioner (precond = 2)
 };
 
 namespace dare_hip {
@@ -136,6 +137,9 @@ int make_geo(const dare_grid* gr, Geo* G);
 // Looks up, with `device` current, the address of the allocation-free solve's static scratch
 // on that device (once per device; dare_ctx_create calls it).
 int noalloc_scratch_init(int device);
+
+// Releases the multigrid hierarchy a solve attached to a matrix (dare_csr::mg).
+void mg_free(void* mg);
 
 // Writes the CSR values of a matrix whose last fill only wrote the diagonal view.
 int csr_ensure_values(dare_csr* A, hipStream_t stream);

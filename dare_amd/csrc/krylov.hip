@@ -57,6 +57,14 @@ struct SolveOp {
     }
 };
 
+}  // namespace
+}  // namespace dare_hip
+
+#include "mg.h"
+
+namespace dare_hip {
+namespace {
+
 // ---- per-context solver workspace (device vectors + state), owned by the context
 struct Workspace {
     void* buf;
@@ -99,10 +107,15 @@ static_assert(sizeof(KState) <= 64 * sizeof(double), "pinned readback block too 
 
 // Host-driven BiCGStab2 over device-resident scalars; XT is the type of every vector, NT the
 // choice of non-temporal stores (vectors larger than the caches).
+// With a multigrid hierarchy mg the method runs right-preconditioned (BiCGStab2.inl:353-365):
+// every operator application is A M^-1, the iterate y starts at zero and accumulates in y_buf
+// (or in x itself when x starts at zero), and x += M^-1 y at the end. The residual r stays the
+// residual of A x = b, so the stopping test keeps its meaning.
 template <typename VT, typename XT, bool NT>
 int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
                      double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                     bool* converged, XT* work, int min_passes, bool x_zero, KState* st, ReduceWs ws) {
+                     bool* converged, XT* work, int min_passes, bool x_zero, KState* st, ReduceWs ws,
+                     const MgHierarchy* mg = nullptr, XT* y_buf = nullptr) {
     const long long n = A.n;
     XT* r = work;
     XT* rh = r + n;
@@ -115,11 +128,14 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
     hipStream_t sm = ctx->stream;
     *hist_len = 0;
     *status = 0;
+    XT* y = (mg && !x_zero) ? y_buf : x;       // the iterate the passes update
+    auto pre = [&](const XT* in) -> const XT* { return mg ? mg_apply<VT, XT>(sm, *mg, in) : in; };
     if (x_zero) {   // x = 0 written here: no operator sweep needed for r = b - A x
         hipLaunchKernelGGL((k_init_zero<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, b, x, r, rh, u, v, w, ws, st);
         launch_finish<2>(sm, ws, gr, st, TAIL_INIT, 0);
     } else {
         A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);     // (0)-(2)
+        if (mg) hipLaunchKernelGGL((k_zero<XT>), dim3(gr), dim3(kThreads), 0, sm, n, y);
     }
     DARE_LAUNCH_CHECK();
     KState h;
@@ -141,14 +157,15 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
             break;
         }
         hipLaunchKernelGGL((k_upd_u<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, u, st);
-        A.launch(sm, u, EpiDot<XT, NT>{v, rh}, ws, st, TAIL_GAMMA, 1);
-        hipLaunchKernelGGL((k_upd_rx<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, x, u, st);
-        A.launch(sm, r, EpiDot<XT, NT>{s, rh}, ws, st, TAIL_RHO, 1);
+        A.launch(sm, pre(u), EpiDot<XT, NT>{v, rh}, ws, st, TAIL_GAMMA, 1);
+        hipLaunchKernelGGL((k_upd_rx<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, y, u, st);
+        A.launch(sm, pre(r), EpiDot<XT, NT>{s, rh}, ws, st, TAIL_RHO, 1);
         hipLaunchKernelGGL((k_upd_v<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, s, v, st);
-        A.launch(sm, v, EpiDot<XT, NT>{w, rh}, ws, st, TAIL_GAMMA, 1);
+        A.launch(sm, pre(v), EpiDot<XT, NT>{w, rh}, ws, st, TAIL_GAMMA, 1);
         hipLaunchKernelGGL((k_upd_urs<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, u, r, v, s, w, st);
-        A.launch(sm, s, EpiGcr<XT, NT>{t, r}, ws, st, TAIL_GCR, 1);
-        hipLaunchKernelGGL((k_end<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, x, r, s, t, u, v, w, rh, ws, st);
+        if (mg) A.launch(sm, pre(s), EpiGcrP<XT, NT>{t, r, s}, ws, st, TAIL_GCR, 1);
+        else A.launch(sm, s, EpiGcr<XT, NT>{t, r}, ws, st, TAIL_GCR, 1);
+        hipLaunchKernelGGL((k_end<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, y, r, s, t, u, v, w, rh, ws, st);
         launch_finish<2>(sm, ws, gr, st, TAIL_END, 1);
         DARE_LAUNCH_CHECK();
         rc = read_state(ctx, st, &h);
@@ -156,8 +173,16 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
         if (h.status) {
             *status = h.status;
             if (h.status == 21) {
-                // tau == 0: true residual check (BiCGStab2.inl:514-536)
-                A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);
+                // tau == 0: true residual check (BiCGStab2.inl:514-536); preconditioned, the
+                // iterate is x + M^-1 y, formed in t (unused from here on)
+                const XT* xt = x;
+                if (mg) {
+                    const XT* z = pre(y);
+                    hipLaunchKernelGGL(k_mg_combine<XT>, dim3(gr), dim3(kThreads), 0, sm, n,
+                                       static_cast<const XT*>(x_zero ? nullptr : x), z, t);
+                    xt = t;
+                }
+                A.launch(sm, xt, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);
                 rc = read_state(ctx, st, &h);
                 if (rc) return rc;
                 conv = h.rnorm;
@@ -165,6 +190,11 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
                     done_ok = true;
                     *status = 0;
                 }
+                if (mg) hipMemcpyAsync(x, t, sizeof(XT) * n, hipMemcpyDeviceToDevice, sm);
+                *iters = k;
+                *rnorm = conv;
+                *converged = done_ok;
+                return DARE_OK;
             }
             break;
         }
@@ -172,6 +202,12 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
         if (hist && *hist_len < hist_cap) hist[(*hist_len)++] = conv;
         if (conv <= tol_abs && k >= min_passes) break;   // reference: k > 1 (min_passes 2)
         ++k;
+    }
+    if (mg) {   // x = x0 + M^-1 y (x0 = 0: x holds y)
+        const XT* z = pre(y);
+        hipLaunchKernelGGL(k_mg_combine<XT>, dim3(gr), dim3(kThreads), 0, sm, n,
+                           static_cast<const XT*>(x_zero ? nullptr : x), z, x);
+        DARE_LAUNCH_CHECK();
     }
     *iters = k;
     *rnorm = conv;
@@ -184,12 +220,13 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
 template <typename VT, typename XT>
 int bicgstab2_run(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, double tol_abs, int max_it,
                   double* hist, int hist_cap, int* hist_len, int* iters, double* rnorm, int* status,
-                  bool* converged, XT* work, int min_passes, bool x_zero, KState* st, ReduceWs ws) {
+                  bool* converged, XT* work, int min_passes, bool x_zero, KState* st, ReduceWs ws,
+                  const MgHierarchy* mg = nullptr, XT* y_buf = nullptr) {
     if (A.n >= kStreamRows)
         return bicgstab2_run_nt<VT, XT, true>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
-                                              status, converged, work, min_passes, x_zero, st, ws);
+                                              status, converged, work, min_passes, x_zero, st, ws, mg, y_buf);
     return bicgstab2_run_nt<VT, XT, false>(ctx, A, b, x, tol_abs, max_it, hist, hist_cap, hist_len, iters, rnorm,
-                                           status, converged, work, min_passes, x_zero, st, ws);
+                                           status, converged, work, min_passes, x_zero, st, ws, mg, y_buf);
 }
 
 static int dot_dev_on(dare_ctx* ctx, long long n, const double* a, const double* b, double* out, KState* st,
@@ -335,6 +372,20 @@ int jacobi_csr_values(dare_ctx* ctx, dare_csr* A, OT* out) {
     return DARE_OK;
 }
 
+// the multigrid hierarchy of A for a solve whose operator is op (slots D0 of type VT)
+template <typename VT, typename XT>
+int mg_prepare(dare_ctx* ctx, dare_csr* A, const SolveOp<VT>& op, const VT* D0, MgHierarchy** out) {
+    if (!op.dia || !A->has_dia) {
+        set_error("the multigrid preconditioner needs a device-assembled 3-D operator (diagonal view)");
+        return DARE_ERR_UNSUPPORTED;
+    }
+    if (!A->mg) A->mg = new MgHierarchy();
+    MgHierarchy* H = static_cast<MgHierarchy*>(A->mg);
+    const int rc = mg_setup<VT, XT>(ctx, H, A->dia_view, D0);
+    *out = H;
+    return rc;
+}
+
 static SolveOp<double> op64(const dare_csr* A, bool allow_dia) {
     SolveOp<double> op;
     op.n = A->nrows;
@@ -400,16 +451,32 @@ static double* noalloc_scratch(int device) {
 
 }  // namespace dare_hip
 
+namespace dare_hip {
+void mg_free(void* p) {
+    if (!p) return;
+    MgHierarchy* H = static_cast<MgHierarchy*>(p);
+    if (H->coarse) hipFree(H->coarse);
+    if (H->fine) hipFree(H->fine);
+    delete H;
+}
+}  // namespace dare_hip
+
 using namespace dare_hip;
 
 extern "C" {
 
 int dare_spmv(dare_ctx* ctx, const dare_csr* A, const double* x, double* y) {
     DARE_CHECK_ARG(ctx && A && x && y, "NULL argument");
-    int rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
-    if (rc) return rc;
-    // the Tpetra apply replacement: CSR, per-row sum in sorted column order
-    SolveOp<double> op = op64(A, false);
+    // the Tpetra apply replacement: per-row sum in sorted column order. Without a periodic
+    // axis no row wraps, and the diagonal view's row order (W,S,B,[X],C,[X],T,N,E) is the
+    // sorted column order, so a device-assembled matrix is applied from its diagonal view
+    // (80 B/row, no CSR values to materialize) with bit-identical results; otherwise the CSR.
+    const bool wrap = A->has_dia && (A->dia_view.per[0] || A->dia_view.per[1] || A->dia_view.per[2]);
+    SolveOp<double> op = op64(A, !wrap);
+    if (!op.dia) {
+        int rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
+        if (rc) return rc;
+    }
     if (A->nrows >= kStreamRows) op.launch(ctx->stream, x, EpiStore<double, true>{y}, ctx->ws, nullptr, -1, 0);
     else op.launch(ctx->stream, x, EpiStore<double, false>{y}, ctx->ws, nullptr, -1, 0);
     DARE_LAUNCH_CHECK();
@@ -477,8 +544,10 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     DARE_CHECK_ARG(ctx && A && b && x && p && res, "NULL argument");
     DARE_CHECK_ARG(A->nrows > 0 && A->row_ptr && A->col && A->val, "empty matrix");
     const long long n = A->nrows;
+    DARE_CHECK_ARG(p->precond >= 0 && p->precond <= 2, "precond must be 0 (none), 1 (Jacobi) or 2 (multigrid)");
+    const bool use_mg = p->precond == DARE_PRECOND_MG;
     Workspace W;
-    int rc = ws_reserve(ctx, sizeof(double) * 7 * n, &W);
+    int rc = ws_reserve(ctx, sizeof(double) * (use_mg ? 8 : 7) * n, &W);   // + the preconditioned iterate
     if (rc) return rc;
     double bnorm2 = 0.;
     rc = dot_dev(ctx, n, b, b, &bnorm2);
@@ -487,7 +556,11 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
     SolveOp<double> op = op64(A, true);
     const bool pre = p->precond == 1;
-    DARE_CHECK_ARG(p->precond == 0 || p->precond == 1, "precond must be 0 (none) or 1 (Jacobi)");
+    MgHierarchy* mg = nullptr;
+    if (use_mg) {
+        rc = mg_prepare<double, double>(ctx, const_cast<dare_csr*>(A), op, A->dia, &mg);
+        if (rc) return rc;
+    }
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
         if (rc) return rc;
@@ -517,7 +590,8 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     double rn = 0.;
     bool conv = false;
     rc = bicgstab2_run<double, double>(ctx, op, b, x, tol_abs, p->max_it, p->history, p->history_cap, &hl, &iters,
-                                       &rn, &status, &conv, static_cast<double*>(W.buf), 2, false, W.st, ctx->ws);
+                                       &rn, &status, &conv, static_cast<double*>(W.buf), 2, false, W.st, ctx->ws, mg,
+                                       static_cast<double*>(W.buf) + 7 * n);
     if (rc) return rc;
     if (pre) {
         hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 1);  // x = M^-1 y
@@ -592,8 +666,9 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     // FP32 copy of the operator (values change every step: converted on every call)
     SolveOp<float> op32;
     op32.n = n;
-    DARE_CHECK_ARG(p->precond == 0 || p->precond == 1, "precond must be 0 (none) or 1 (Jacobi)");
+    DARE_CHECK_ARG(p->precond >= 0 && p->precond <= 2, "precond must be 0 (none), 1 (Jacobi) or 2 (multigrid)");
     const bool pre = p->precond == 1;
+    const bool use_mg = p->precond == DARE_PRECOND_MG;
     if (A->has_dia && kXcd * make_dia<double>(A->dia_view, A->dia).per_xcd <= kReduceBlocks) {
         const int slots = A->dia_view.cross ? 8 : 7;
         const int64_t need = static_cast<int64_t>(slots) * n;
@@ -666,6 +741,11 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     }
     op32.csr = CsrOp<float>{A->row_ptr, A->col, A->val32, n};
     DARE_LAUNCH_CHECK();
+    MgHierarchy* mg = nullptr;
+    if (use_mg) {
+        int mrc = mg_prepare<float, float>(ctx, A, op32, static_cast<const float*>(A->dia32), &mg);
+        if (mrc) return mrc;
+    }
     Workspace W;
     // 7 FP32 work vectors + FP32 residual + FP32 correction
     int rc = ws_reserve(ctx, sizeof(float) * 9 * n, &W);
@@ -710,7 +790,7 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         bool iconv = false;
         rc = bicgstab2_run<float, float>(ctx, op32, r32, d32, inner_threshold(inner_tol, rn, tol_abs), inner_max,
                                          nullptr, 0, &dummy_len, &it, &irn, &ist, &iconv, work, 2,
-                                         true, W.st, ctx->ws);   // d32 = 0 to start
+                                         true, W.st, ctx->ws, mg);   // d32 = 0 to start
         if (rc) return rc;
         total_it += it;
         if (pre)    // the inner solution is y = M d

@@ -292,6 +292,29 @@ struct EpiGcr {         // t = A s; <r,s>, <s,s>, <s,t>, <t,t>, <r,t>   (18)-(20
     }
 };
 
+// the same step when a right preconditioner sits between s and the operator: t = A (M^-1 s),
+// with s read from its own vector (the operand of the traversal is M^-1 s)
+template <typename XT, bool NT = false>
+struct EpiGcrP {
+    static constexpr int NDOT = 5;
+    static constexpr bool kNT = NT;
+    XT* t;
+    const XT* r;
+    const XT* s;
+    __device__ double load(long long i) const { return static_cast<double>(r[i]); }
+    __device__ void operator()(long long i, double y, double, double rd, double* acc) const {
+        const XT ti = static_cast<XT>(y);
+        put<NT>(t + i, ti);
+        const double td = static_cast<double>(ti);
+        const double sc = static_cast<double>(s[i]);
+        acc[0] += rd * sc;
+        acc[1] += sc * sc;
+        acc[2] += sc * td;
+        acc[3] += td * td;
+        acc[4] += rd * td;
+    }
+};
+
 template <bool NT = false>
 struct EpiResid {       // refinement: r = b - A x in FP64, r32 = float(r), <r,r>
     static constexpr int NDOT = 1;
@@ -395,34 +418,75 @@ __device__ __forceinline__ void epi_quad(const EpiInit<XT, NT>& e, long long i, 
 }
 
 // ------------------------------------------------------------------------------ traversals
-// Generic CSR: one row per lane; the row's entries are fetched four at a time with clamped
-// indices (all loads in flight before the multiply-adds), summed in column order.
+// Generic CSR, one wave per 64 consecutive rows. The wave walks its rows' run of the CSR
+// arrays -- one contiguous range -- with coalesced loads (lane l takes entries l, l + 64, ...),
+// forms each product val * x[col] (rounded to double, as the row loop rounds it) and stages it
+// in its own slice of LDS; then lane l sums row r0 + l from LDS in column order, starting from
+// +0.0: the sequential row sum of the reference's row loop, so y is bit-identical to it. A wave
+// whose rows hold more than kCsrWaveNnz entries sums them from global memory instead. Wave
+// tiles are dealt so that each XCD (blocks b = x mod 8) owns one contiguous eighth of the rows.
+// Only the wave that wrote a slice reads it, so the phases need no work-group barrier.
+constexpr int kCsrWaveRows = 64;
+constexpr int kCsrWaveNnz = 768;
+constexpr int kCsrWaves = kThreads / kCsrWaveRows;
+
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename VT, typename XT, class Epi>
 __global__ __launch_bounds__(kThreads) void k_apply_csr(CsrOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
                                                        KState* st, int tail, int check_status) {
+    __shared__ double s_prod[kCsrWaves][kCsrWaveNnz];
     double acc[Epi::NDOT];
 #pragma unroll
     for (int j = 0; j < Epi::NDOT; j++) acc[j] = 0.;
     const bool run = !(check_status && st->status);
     if (run) {
-        for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < A.n; i += (long long)gridDim.x * kThreads) {
-            const int b = A.rp[i], e = A.rp[i + 1];
-            const double pre = epi.load(i);
-            const double xi = static_cast<double>(x[i]);
+        const int lane = threadIdx.x % kCsrWaveRows, wv = threadIdx.x / kCsrWaveRows;
+        double* prod = s_prod[wv];
+        const long long ntiles = (A.n + kCsrWaveRows - 1) / kCsrWaveRows;
+        const int xcd = static_cast<int>(blockIdx.x % kXcd);
+        const long long t0 = ntiles * xcd / kXcd, t1 = ntiles * (xcd + 1) / kXcd;
+        const long long waves_per_xcd = static_cast<long long>(gridDim.x / kXcd) * kCsrWaves;
+        for (long long tile = t0 + static_cast<long long>(blockIdx.x / kXcd) * kCsrWaves + wv; tile < t1;
+             tile += waves_per_xcd) {
+            const long long r0 = tile * kCsrWaveRows;
+            const long long r1 = (r0 + kCsrWaveRows < A.n) ? r0 + kCsrWaveRows : A.n;
+            const int B = A.rp[r0], E = A.rp[r1];
+            const long long i = r0 + lane;
+            const bool own = i < r1;
+            const int b = own ? A.rp[i] : E, e = own ? A.rp[i + 1] : E;
             double s = 0.;
-            for (int k = b; k < e; k += 4) {
-                double vv[4], xv[4];
+            const int nnz = E - B;
+            if (nnz <= kCsrWaveNnz) {
+                for (int k0 = 0; k0 < nnz; k0 += 8 * kCsrWaveRows) {
+                    int c[8];
+                    double v[8], xv[8];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int kk = (k + u < e) ? k + u : b;
-                    vv[u] = static_cast<double>(A.val[kk]);
-                    xv[u] = static_cast<double>(x[A.col[kk]]);
+                    for (int u = 0; u < 8; u++) {
+                        const int k = k0 + u * kCsrWaveRows + lane;
+                        const int kk = B + (k < nnz ? k : nnz - 1);
+                        c[u] = A.col[kk];
+                        v[u] = static_cast<double>(A.val[kk]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) xv[u] = static_cast<double>(x[c[u]]);
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int k = k0 + u * kCsrWaveRows + lane;
+                        if (k < nnz) prod[k] = v[u] * xv[u];
+                    }
                 }
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (k + u < e) s += vv[u] * xv[u];
+                wave_lds_sync();
+                for (int k = b - B; k < e - B; k++) s += prod[k];
+                wave_lds_sync();   // the slice is rewritten by the wave's next tile
+            } else {
+                for (int k = b; k < e; k++) s += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
             }
-            epi(i, s, xi, pre, acc);
+            if (own) epi(i, s, static_cast<double>(x[i]), epi.load(i), acc);
         }
     }
     if (tail >= 0) grid_reduce<Epi::NDOT>(acc, ws, st, run ? tail : TAIL_NONE);
@@ -1030,10 +1094,10 @@ __global__ void k_clear_state(KState* st) {
     }
 }
 
-inline unsigned grid_rows(long long n) {      // one row per lane per sweep (CSR traversal)
+inline unsigned grid_rows(long long n) {      // CSR traversal: kThreads rows (one per wave and 64 rows) per block and sweep
     long long g = (n + kThreads - 1) / kThreads;
-    if (g > kGridMax) g = kGridMax;
-    if (g < 1) g = 1;
+    if (g > kReduceBlocks) g = kReduceBlocks;
+    g = (g + kXcd - 1) / kXcd * kXcd;             // every XCD gets the same number of blocks
     return static_cast<unsigned>(g);
 }
 

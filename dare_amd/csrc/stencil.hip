@@ -684,6 +684,40 @@ __global__ void k_fill(double* __restrict__ p, long long n, double v) {
     if (i < n) p[i] = v;
 }
 
+// Read + write stream, the attainable-bandwidth probe of the measurements: each lane reads R
+// 16-byte chunks (non-temporal, all in flight before the store) and writes their sum, one
+// 16-byte chunk; a work-group covers 256 consecutive chunks per read, so every access is
+// coalesced. R = 1 is a copy; larger R gives the read-heavy mixes of the solver kernels
+// (the FP64 SpMV moves 9 bytes read per byte written).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+// LAYOUT 0: XCD x (blocks b = x mod 8) owns one contiguous eighth of the chunks and each chunk
+// reads R consecutive 4 KiB pieces of src; 1: the same pieces with the grid striding over all
+// chunks; 2: XCD slabs, piece u of a chunk read from the u-th of R separate arrays (src + u *
+// n_dst: the per-slot arrays of the diagonal view).
+template <int R, int LAYOUT>
+__global__ __launch_bounds__(256) void k_stream_probe(const dvec2* __restrict__ src, dvec2* __restrict__ dst,
+                                                      long long n2_dst) {
+    const long long nblk = n2_dst / 256;
+    long long b0 = blockIdx.x, b1 = nblk, step = gridDim.x;
+    if (LAYOUT != 1) {
+        const int xcd = static_cast<int>(blockIdx.x % 8);
+        b0 = nblk * xcd / 8 + blockIdx.x / 8;
+        b1 = nblk * (xcd + 1) / 8;
+        step = gridDim.x / 8;
+    }
+    for (long long blk = b0; blk < b1; blk += step) {
+        const long long j = blk * 256 + threadIdx.x;
+        dvec2 v[R];
+#pragma unroll
+        for (int u = 0; u < R; u++)
+            v[u] = __builtin_nontemporal_load(LAYOUT == 2 ? src + u * n2_dst + j : src + (blk * R + u) * 256 + threadIdx.x);
+        dvec2 sum = v[0];
+#pragma unroll
+        for (int u = 1; u < R; u++) sum += v[u];
+        __builtin_nontemporal_store(sum, dst + j);
+    }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(256) void k_copy_to_field(Geo G, const double* __restrict__ x, double* __restrict__ phi) {
     int q[3];
@@ -789,6 +823,37 @@ int dare_fill_f64(dare_ctx* ctx, double* p, int64_t n, double v) {
     DARE_CHECK_ARG(ctx && (p || n == 0) && n >= 0, "invalid fill arguments");
     if (n == 0) return DARE_OK;
     hipLaunchKernelGGL(k_fill, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, p, static_cast<long long>(n), v);
+    DARE_LAUNCH_CHECK();
+    return DARE_OK;
+}
+
+int dare_stream_probe(dare_ctx* ctx, int64_t n_dst, const double* src, double* dst, int32_t ratio, int32_t grid,
+                      int32_t layout) {
+    DARE_CHECK_ARG(ctx && src && dst && n_dst >= 0 && n_dst % 512 == 0,
+                   "invalid stream-probe arguments (n_dst a multiple of 512)");
+    DARE_CHECK_ARG(ratio == 1 || ratio == 2 || ratio == 4 || ratio == 8, "ratio must be 1, 2, 4 or 8");
+    DARE_CHECK_ARG(layout >= 0 && layout <= 2, "layout must be 0 (XCD slabs), 1 (grid stride) or 2 (separate arrays)");
+    DARE_CHECK_ARG((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16 == 0,
+                   "stream probe needs 16-byte aligned vectors");
+    if (n_dst == 0) return DARE_OK;
+    const long long n2 = n_dst / 2;
+    long long gr = grid > 0 ? grid : 8 * ctx->num_cu;
+    if (gr > n2 / 256) gr = n2 / 256;
+    if (layout != 1) gr = (gr + 7) / 8 * 8;
+    const dim3 g(static_cast<unsigned>(gr)), b(256);
+    const dvec2* s2 = reinterpret_cast<const dvec2*>(src);
+    dvec2* d2 = reinterpret_cast<dvec2*>(dst);
+#define DARE_PROBE(R)                                                                                   \
+    if (layout == 0) hipLaunchKernelGGL((k_stream_probe<R, 0>), g, b, 0, ctx->stream, s2, d2, n2);     \
+    else if (layout == 1) hipLaunchKernelGGL((k_stream_probe<R, 1>), g, b, 0, ctx->stream, s2, d2, n2); \
+    else hipLaunchKernelGGL((k_stream_probe<R, 2>), g, b, 0, ctx->stream, s2, d2, n2);
+    switch (ratio) {
+    case 1: DARE_PROBE(1) break;
+    case 2: DARE_PROBE(2) break;
+    case 4: DARE_PROBE(4) break;
+    default: DARE_PROBE(8) break;
+    }
+#undef DARE_PROBE
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }

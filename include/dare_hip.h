@@ -55,6 +55,9 @@ extern "C" {
 /* Right preconditioners of the solver (dare_solver_params.precond). */
 #define DARE_PRECOND_NONE 0
 #define DARE_PRECOND_JACOBI 1   /* M = diag(A): Ifpack2 "RELAXATION", Jacobi, one sweep, damping 1 */
+/* aggregation multigrid V-cycle on the diagonal view (one-component 3-D operators): the
+ * stand-in for MueLu "AMG" (MatrixSystem/TrilinosSolver.inl:180-246) */
+#define DARE_PRECOND_MG 2
 
 typedef struct dare_ctx dare_ctx;
 typedef struct dare_csr dare_csr;
@@ -106,7 +109,9 @@ typedef struct {
     int32_t precond;         /* 0: none (the reference default, M == null); 1: Jacobi, the right
                               * preconditioner M = diag(A) (BiCGStab2.inl:353-365 applies M^-1 on
                               * the right); the diagonal-view operator is column-scaled, a
-                              * generic CSR matrix gets a scaled copy of its values */
+                              * generic CSR matrix gets a scaled copy of its values; 2:
+                              * multigrid V-cycle M^-1 applied before every operator
+                              * application (DARE_PRECOND_MG) */
 } dare_solver_params;
 
 typedef struct {
@@ -134,6 +139,16 @@ int dare_memcpy_h2d(dare_ctx* ctx, void* dst, const void* src, size_t bytes);
 int dare_memcpy_d2h(dare_ctx* ctx, void* dst, const void* src, size_t bytes);
 int dare_memcpy_d2d(dare_ctx* ctx, void* dst, const void* src, size_t bytes);
 int dare_fill_f64(dare_ctx* ctx, double* p, int64_t n, double v);
+
+/* Bandwidth probe: dst[j] = sum of `ratio` doubles of src, src holding n_dst * ratio doubles,
+ * every access a coalesced non-temporal 16-byte one (ratio 1, 2, 4 or 8; n_dst a multiple of 512;
+ * 16-byte aligned). ratio 1 is a copy; ratio 8 has the read-heavy mix of the solver's SpMV. The
+ * attainable HBM bandwidth the measurements quote beside each roofline. No reference
+ * counterpart. grid <= 0 picks eight work-groups per CU; layout 0 deals each XCD one contiguous
+ * slab (as the solver's kernels do), 1 strides the whole grid over the vectors, 2 is layout 0
+ * with src read as `ratio` separate arrays of n_dst doubles (the slot-major diagonal view). */
+int dare_stream_probe(dare_ctx* ctx, int64_t n_dst, const double* src, double* dst, int32_t ratio, int32_t grid,
+                      int32_t layout);
 
 /* ---- stencils (device pointers, FP64) ------------------------------------------------- */
 /* Ghost fill. Replaces Field::ExchangeHaloCells -> HaloBuffer::Exchange on one rank

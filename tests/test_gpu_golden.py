@@ -1,7 +1,8 @@
 """The HIP path against the committed golden fixtures (tests/golden/), with no oracle build:
 stencils, ghost fill, face interpolation, TVD and the CSR assembly bit-exact; BiCGStab2 within
 one pass of the fixture's count, the solution within 1e-12 relative and the residual norms
-of passes 1-3 within 1e-10 relative (the dot products are summed in another order)."""
+of passes 1-3 within 1e-10 relative or 1e-13 of ||b|| (the dot products are summed in another
+order)."""
 import os
 import sys
 
@@ -97,6 +98,10 @@ def test_assembly_and_solve_match_golden(ctx, n):
         assert r["converged"] == 1 and abs(r["iterations"] - int(Z[f"{name}_it"][0])) <= 1, (name, r["iterations"])
         xs = Z[f"{name}_x"]
         assert np.max(np.abs(x.cpu().numpy() - xs)) <= 1e-12 * np.max(np.abs(xs)), name
+        # residual norms of passes 1-3, to 1e-10 relative or 1e-13 of ||b|| (a norm far below ||b||
+        # is a cancellation result: its relative rounding differences grow as it shrinks)
         h = np.array(r["history"][:len(Z[f"{name}_hist"])])
-        assert np.all(np.abs(h - Z[f"{name}_hist"]) <= 1e-10 * Z[f"{name}_hist"]), (name, h, Z[f"{name}_hist"])
+        href = Z[f"{name}_hist"]
+        bnorm = float(np.linalg.norm(Z[f"{name}_rhs"]))
+        assert len(h) == len(href) and np.all(np.abs(h - href) <= 1e-10 * href + 1e-13 * bnorm), (name, h, href)
         A.close()

@@ -84,3 +84,27 @@ def test_dot_and_axpby(ctx, rng):
         assert np.array_equal(y.cpu().numpy().view(np.int64), exp.view(np.int64)), n
     with pytest.raises(da.DareError):
         da.check(da.lib().dare_axpby(ctx.h, -1, 1.0, 1, 1.0, 1), "dare_axpby")
+
+
+def test_stream_probe(ctx, rng):
+    """The bandwidth probe computes what it claims (copy, and sums of 8 strided chunks)."""
+    import torch
+    n = 512 * 300
+    src = torch.from_numpy(rng.normal(size=8 * n)).cuda()
+    dst = torch.empty(n, dtype=torch.float64, device="cuda")
+    da.stream_probe(ctx, src[:n], dst, ratio=1)
+    assert torch.equal(dst, src[:n])
+    s = src.cpu().numpy().reshape(-1, 8, 256, 2)          # [block][u][lane][2]
+    exp = s[:, 0].copy()
+    for u in range(1, 8):
+        exp = exp + s[:, u]
+    for layout in (0, 1):
+        dst.zero_()
+        da.stream_probe(ctx, src, dst, ratio=8, layout=layout)
+        assert np.array_equal(dst.cpu().numpy(), exp.reshape(-1)), layout
+    a = src.cpu().numpy().reshape(8, n)                    # layout 2: eight separate arrays
+    exp = a[0].copy()
+    for u in range(1, 8):
+        exp = exp + a[u]
+    da.stream_probe(ctx, src, dst, ratio=8, layout=2, grid=64)
+    assert np.array_equal(dst.cpu().numpy(), exp)
