@@ -308,7 +308,8 @@ class Csr:
 def stream_probe(ctx, src, dst, ratio=1, grid=0, layout=0):
     """dst[j] = sum of `ratio` elements of src (src holds ratio * dst.numel()), through the
     library's streaming kernel: the attainable-bandwidth probe (ratio 1 = copy; layout 0 XCD
-    slabs, 1 grid stride, 2 XCD slabs over `ratio` separate arrays)."""
+    slabs, 1 grid stride, 2 XCD slabs over `ratio` separate arrays, 3 the same read-only: dst[:512 grid]
+    receives per-lane totals)."""
     assert src.numel() == ratio * dst.numel()
     check(lib().dare_stream_probe(ctx.h, dst.numel(), _ptr(src), _ptr(dst), ratio, grid, layout),
           "dare_stream_probe")

@@ -467,6 +467,7 @@ extern "C" {
 
 int dare_spmv(dare_ctx* ctx, const dare_csr* A, const double* x, double* y) {
     DARE_CHECK_ARG(ctx && A && x && y, "NULL argument");
+    if (A->nrows == 0) return DARE_OK;
     // the Tpetra apply replacement: per-row sum in sorted column order. Without a periodic
     // axis no row wraps, and the diagonal view's row order (W,S,B,[X],C,[X],T,N,E) is the
     // sorted column order, so a device-assembled matrix is applied from its diagonal view
@@ -485,6 +486,10 @@ int dare_spmv(dare_ctx* ctx, const dare_csr* A, const double* x, double* y) {
 
 int dare_residual(dare_ctx* ctx, const dare_csr* A, const double* b, const double* x, double* r, double* norm_host) {
     DARE_CHECK_ARG(ctx && A && b && x && r, "NULL argument");
+    if (A->nrows == 0) {
+        if (norm_host) *norm_host = 0.;
+        return DARE_OK;
+    }
     Workspace W;
     int rc = ws_reserve(ctx, 0, &W);
     if (rc) return rc;
@@ -509,6 +514,10 @@ int dare_residual(dare_ctx* ctx, const dare_csr* A, const double* b, const doubl
 int dare_spmv_dot(dare_ctx* ctx, const dare_csr* A, const double* x, double* y, const double* w,
                   double* out_host) {
     DARE_CHECK_ARG(ctx && A && x && y && w, "NULL argument");
+    if (A->nrows == 0) {
+        if (out_host) *out_host = 0.;
+        return DARE_OK;
+    }
     Workspace W;
     int rc = ws_reserve(ctx, 0, &W);
     if (rc) return rc;

@@ -427,13 +427,44 @@ __device__ __forceinline__ void epi_quad(const EpiInit<XT, NT>& e, long long i, 
 // tiles are dealt so that each XCD (blocks b = x mod 8) owns one contiguous eighth of the rows.
 // Only the wave that wrote a slice reads it, so the phases need no work-group barrier.
 constexpr int kCsrWaveRows = 64;
-constexpr int kCsrWaveNnz = 768;
+constexpr int kCsrWaveNnz = 8 * 64;   // eight entries per lane
 constexpr int kCsrWaves = kThreads / kCsrWaveRows;
 
 __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// row bounds of wave tile t: the run [B, E) of the CSR arrays and this lane's row [b, e)
+struct CsrTileRp {
+    int B, E, b, e;
+};
+template <typename VT>
+__device__ inline CsrTileRp csr_tile_rp(const CsrOp<VT>& A, long long t, int lane) {
+    const long long r0 = t * kCsrWaveRows;
+    const long long r1 = (r0 + kCsrWaveRows < A.n) ? r0 + kCsrWaveRows : A.n;
+    const long long i = r0 + lane;
+    CsrTileRp R;
+    R.B = A.rp[r0];
+    R.E = A.rp[r1];
+    R.b = i < r1 ? A.rp[i] : R.E;
+    R.e = i < r1 ? A.rp[i + 1] : R.E;
+    return R;
+}
+
+// columns and values of a staged wave tile: eight entries per lane, entries past the tile's run
+// repeating its last one; streamed with non-temporal loads so that x keeps the caches
+template <typename VT>
+__device__ inline void csr_tile_cv(const CsrOp<VT>& A, const CsrTileRp& R, int lane, int* c, double* v) {
+    const int nnz = R.E - R.B;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int k = u * kCsrWaveRows + lane;
+        const int kk = R.B + (k < nnz ? k : nnz - 1);
+        c[u] = __builtin_nontemporal_load(A.col + kk);
+        v[u] = static_cast<double>(__builtin_nontemporal_load(A.val + kk));
+    }
 }
 
 template <typename VT, typename XT, class Epi>
@@ -450,43 +481,64 @@ __global__ __launch_bounds__(kThreads) void k_apply_csr(CsrOp<VT> A, const XT* _
         const long long ntiles = (A.n + kCsrWaveRows - 1) / kCsrWaveRows;
         const int xcd = static_cast<int>(blockIdx.x % kXcd);
         const long long t0 = ntiles * xcd / kXcd, t1 = ntiles * (xcd + 1) / kXcd;
-        const long long waves_per_xcd = static_cast<long long>(gridDim.x / kXcd) * kCsrWaves;
-        for (long long tile = t0 + static_cast<long long>(blockIdx.x / kXcd) * kCsrWaves + wv; tile < t1;
-             tile += waves_per_xcd) {
-            const long long r0 = tile * kCsrWaveRows;
-            const long long r1 = (r0 + kCsrWaveRows < A.n) ? r0 + kCsrWaveRows : A.n;
-            const int B = A.rp[r0], E = A.rp[r1];
-            const long long i = r0 + lane;
-            const bool own = i < r1;
-            const int b = own ? A.rp[i] : E, e = own ? A.rp[i + 1] : E;
+        const long long stride = static_cast<long long>(gridDim.x / kXcd) * kCsrWaves;
+        long long tile = t0 + static_cast<long long>(blockIdx.x / kXcd) * kCsrWaves + wv;
+        // software pipeline: the next tile's columns and values, and the row bounds of the one
+        // after it, are loaded while this tile's x entries are gathered and summed
+        CsrTileRp R0{}, R1{};
+        int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+        if (tile < t1) {
+            R0 = csr_tile_rp(A, tile, lane);
+            if (R0.E - R0.B > 0 && R0.E - R0.B <= kCsrWaveNnz) csr_tile_cv(A, R0, lane, c, v);
+        }
+        if (tile + stride < t1) R1 = csr_tile_rp(A, tile + stride, lane);
+        for (; tile < t1; tile += stride) {
+            const long long i = tile * kCsrWaveRows + lane;
+            const bool own = i < A.n;
+            const long long ic = own ? i : A.n - 1;      // a valid address for every lane
+            const double xi = static_cast<double>(x[ic]);
+            const double pre = epi.load(ic);
+            const int nnz = R0.E - R0.B, b = R0.b - R0.B, e = R0.e - R0.B;
+            const bool staged = nnz <= kCsrWaveNnz;
+            int cn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            double vn[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+            CsrTileRp R2{};
+            if (tile + stride < t1 && R1.E - R1.B > 0 && R1.E - R1.B <= kCsrWaveNnz) csr_tile_cv(A, R1, lane, cn, vn);
+            if (tile + 2 * stride < t1) R2 = csr_tile_rp(A, tile + 2 * stride, lane);
             double s = 0.;
-            const int nnz = E - B;
-            if (nnz <= kCsrWaveNnz) {
-                for (int k0 = 0; k0 < nnz; k0 += 8 * kCsrWaveRows) {
-                    int c[8];
-                    double v[8], xv[8];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const int k = k0 + u * kCsrWaveRows + lane;
-                        const int kk = B + (k < nnz ? k : nnz - 1);
-                        c[u] = A.col[kk];
-                        v[u] = static_cast<double>(A.val[kk]);
-                    }
+            if (staged) {
+                // every slot of the slice is written (entries past nnz repeat the last one and
+                // are never read), so no lane branches
+                if (nnz > 0) {
+                    double xv[8];
 #pragma unroll
                     for (int u = 0; u < 8; u++) xv[u] = static_cast<double>(x[c[u]]);
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const int k = k0 + u * kCsrWaveRows + lane;
-                        if (k < nnz) prod[k] = v[u] * xv[u];
-                    }
+                    for (int u = 0; u < 8; u++) prod[u * kCsrWaveRows + lane] = v[u] * xv[u];
                 }
                 wave_lds_sync();
-                for (int k = b - B; k < e - B; k++) s += prod[k];
+                for (int k = b; k < e; k += 8) {
+                    double pv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) pv[u] = prod[(k + u < e) ? k + u : b];
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (k + u < e) s += pv[u];
+                }
                 wave_lds_sync();   // the slice is rewritten by the wave's next tile
             } else {
-                for (int k = b; k < e; k++) s += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
+                for (int k = R0.B + b; k < R0.B + e; k++)
+                    s += static_cast<double>(A.val[k]) * static_cast<double>(x[A.col[k]]);
             }
-            if (own) epi(i, s, static_cast<double>(x[i]), epi.load(i), acc);
+            if (own) epi(i, s, xi, pre, acc);
+            R0 = R1;
+            R1 = R2;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                c[u] = cn[u];
+                v[u] = vn[u];
+            }
         }
     }
     if (tail >= 0) grid_reduce<Epi::NDOT>(acc, ws, st, run ? tail : TAIL_NONE);

@@ -693,7 +693,8 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 // LAYOUT 0: XCD x (blocks b = x mod 8) owns one contiguous eighth of the chunks and each chunk
 // reads R consecutive 4 KiB pieces of src; 1: the same pieces with the grid striding over all
 // chunks; 2: XCD slabs, piece u of a chunk read from the u-th of R separate arrays (src + u *
-// n_dst: the per-slot arrays of the diagonal view).
+// n_dst: the per-slot arrays of the diagonal view); 3: layout 2 read-only, each lane adding its
+// chunks' sums and writing the total once, to dst[block * 256 + lane].
 template <int R, int LAYOUT>
 __global__ __launch_bounds__(256) void k_stream_probe(const dvec2* __restrict__ src, dvec2* __restrict__ dst,
                                                       long long n2_dst) {
@@ -705,17 +706,20 @@ __global__ __launch_bounds__(256) void k_stream_probe(const dvec2* __restrict__ 
         b1 = nblk * (xcd + 1) / 8;
         step = gridDim.x / 8;
     }
+    dvec2 total = {0., 0.};
     for (long long blk = b0; blk < b1; blk += step) {
         const long long j = blk * 256 + threadIdx.x;
         dvec2 v[R];
 #pragma unroll
         for (int u = 0; u < R; u++)
-            v[u] = __builtin_nontemporal_load(LAYOUT == 2 ? src + u * n2_dst + j : src + (blk * R + u) * 256 + threadIdx.x);
+            v[u] = __builtin_nontemporal_load(LAYOUT >= 2 ? src + u * n2_dst + j : src + (blk * R + u) * 256 + threadIdx.x);
         dvec2 sum = v[0];
 #pragma unroll
         for (int u = 1; u < R; u++) sum += v[u];
-        __builtin_nontemporal_store(sum, dst + j);
+        if (LAYOUT == 3) total += sum;
+        else __builtin_nontemporal_store(sum, dst + j);
     }
+    if (LAYOUT == 3) dst[static_cast<long long>(blockIdx.x) * 256 + threadIdx.x] = total;
 }
 
 template <int DIM>
@@ -832,7 +836,8 @@ int dare_stream_probe(dare_ctx* ctx, int64_t n_dst, const double* src, double* d
     DARE_CHECK_ARG(ctx && src && dst && n_dst >= 0 && n_dst % 512 == 0,
                    "invalid stream-probe arguments (n_dst a multiple of 512)");
     DARE_CHECK_ARG(ratio == 1 || ratio == 2 || ratio == 4 || ratio == 8, "ratio must be 1, 2, 4 or 8");
-    DARE_CHECK_ARG(layout >= 0 && layout <= 2, "layout must be 0 (XCD slabs), 1 (grid stride) or 2 (separate arrays)");
+    DARE_CHECK_ARG(layout >= 0 && layout <= 3,
+                   "layout must be 0 (XCD slabs), 1 (grid stride), 2 (separate arrays) or 3 (read-only)");
     DARE_CHECK_ARG((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16 == 0,
                    "stream probe needs 16-byte aligned vectors");
     if (n_dst == 0) return DARE_OK;
@@ -840,13 +845,15 @@ int dare_stream_probe(dare_ctx* ctx, int64_t n_dst, const double* src, double* d
     long long gr = grid > 0 ? grid : 8 * ctx->num_cu;
     if (gr > n2 / 256) gr = n2 / 256;
     if (layout != 1) gr = (gr + 7) / 8 * 8;
+    DARE_CHECK_ARG(layout != 3 || 512 * gr <= n_dst, "read-only probe: dst holds 512 doubles per work-group");
     const dim3 g(static_cast<unsigned>(gr)), b(256);
     const dvec2* s2 = reinterpret_cast<const dvec2*>(src);
     dvec2* d2 = reinterpret_cast<dvec2*>(dst);
 #define DARE_PROBE(R)                                                                                   \
     if (layout == 0) hipLaunchKernelGGL((k_stream_probe<R, 0>), g, b, 0, ctx->stream, s2, d2, n2);     \
     else if (layout == 1) hipLaunchKernelGGL((k_stream_probe<R, 1>), g, b, 0, ctx->stream, s2, d2, n2); \
-    else hipLaunchKernelGGL((k_stream_probe<R, 2>), g, b, 0, ctx->stream, s2, d2, n2);
+    else if (layout == 2) hipLaunchKernelGGL((k_stream_probe<R, 2>), g, b, 0, ctx->stream, s2, d2, n2); \
+    else hipLaunchKernelGGL((k_stream_probe<R, 3>), g, b, 0, ctx->stream, s2, d2, n2);
     switch (ratio) {
     case 1: DARE_PROBE(1) break;
     case 2: DARE_PROBE(2) break;

@@ -146,7 +146,9 @@ int dare_fill_f64(dare_ctx* ctx, double* p, int64_t n, double v);
  * attainable HBM bandwidth the measurements quote beside each roofline. No reference
  * counterpart. grid <= 0 picks eight work-groups per CU; layout 0 deals each XCD one contiguous
  * slab (as the solver's kernels do), 1 strides the whole grid over the vectors, 2 is layout 0
- * with src read as `ratio` separate arrays of n_dst doubles (the slot-major diagonal view). */
+ * with src read as `ratio` separate arrays of n_dst doubles (the slot-major diagonal view), 3 is
+ * layout 2 read-only: each lane sums all it reads and writes the total once, as two doubles at
+ * dst[2 (256 b + t)] for lane t of work-group b (dst then needs 512 doubles per work-group). */
 int dare_stream_probe(dare_ctx* ctx, int64_t n_dst, const double* src, double* dst, int32_t ratio, int32_t grid,
                       int32_t layout);
 

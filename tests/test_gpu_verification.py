@@ -108,3 +108,17 @@ def test_stream_probe(ctx, rng):
         exp = exp + a[u]
     da.stream_probe(ctx, src, dst, ratio=8, layout=2, grid=64)
     assert np.array_equal(dst.cpu().numpy(), exp)
+    # layout 3 (read-only): lane t of work-group b writes the total of its chunks at 256 b + t
+    grid = 64
+    dst.fill_(-1.0)
+    da.stream_probe(ctx, src, dst, ratio=8, layout=3, grid=grid)
+    e2 = exp.reshape(-1, 256, 2)                           # [chunk][lane][2]
+    nblk = e2.shape[0]
+    tot = np.zeros((grid, 256, 2))
+    for b in range(grid):
+        x = b % 8
+        for blk in range(nblk * x // 8 + b // 8, nblk * (x + 1) // 8, grid // 8):
+            tot[b] += e2[blk]
+    got = dst.cpu().numpy()
+    assert np.allclose(got[:grid * 512], tot.reshape(-1), rtol=1e-12, atol=0)
+    assert np.all(got[grid * 512:] == -1.0)
