@@ -318,8 +318,7 @@ def target_leg(ctx, n, steps, inner_tol=1e-4):
         wpath = os.path.join(WEIGHTS, "ws_vardiff_128.npz")
     wpath = WEIGHTS_OVERRIDE or wpath
     if os.path.exists(wpath):
-        w, meta = WarmStartNet.load_params(wpath)
-        net = WarmStartNet(ctx, [n] * 3, in_channels=meta["in_channels"], params=w)
+        net = WarmStartNet.load(ctx, wpath, [n] * 3)
     # (name, mixed, precond, guess)
     forms = [("fp64", False, 0, "previous"), ("fp64+jacobi", False, 1, "previous"),
              ("fp64+mg", False, 2, "previous"),
@@ -510,8 +509,7 @@ def learned_leg(ctx, seeds=2, steps=30):
     if not os.path.exists(wpath):
         return None
     log("[bench] learned: held-out 128^3 advection-diffusion steps")
-    w, meta = WarmStartNet.load_params(wpath)
-    net = WarmStartNet(ctx, [128] * 3, in_channels=meta["in_channels"], params=w)
+    net = WarmStartNet.load(ctx, wpath, [128] * 3)
     ev = evaluate(ctx, net, n=128, eval_seeds=range(5000, 5000 + seeds), steps=steps, workload="advdiff")
     net.close()
     return {"workload": f"128^3 advection-diffusion, {seeds} held-out seeds x {steps} steps", **ev}

@@ -845,10 +845,13 @@ __global__ __launch_bounds__(kNetThreads) void k_mask(const uint16_t* __restrict
 // s[2] is also the scale of the output: the guess is phi^n + s[2] * out.
 // The network may see a sub-box (crop) of the grid: n0..n2 are its sizes, o0..o2 its origin
 // among the internal cells, m1, m2 the grid's internal sizes (row order of the solver vectors).
+// With coarsening cf (1 or 2) the network sees the box at 1/cf resolution: network voxel
+// (z, y, x) stands for the cf^3 cells (cf z + a, cf y + b, cf x + c) of the box, c0..c2 = n/cf.
 struct FieldGeo {
     int n0, n1, n2, g;
     int o0, o1, o2, m1, m2;
     long long s0, s1;    // padded strides of i and j
+    int cf, c0, c1, c2;
 };
 
 __device__ inline long long pad_index(const FieldGeo& F, int z, int y, int x) {
@@ -859,6 +862,17 @@ __device__ inline long long row_index(const FieldGeo& F, int z, int y, int x) {
     return (static_cast<long long>(z + F.o0) * F.m1 + (y + F.o1)) * F.m2 + (x + F.o2);
 }
 
+// (z, y, x) of linear index q over an a x b x c box, c fastest, in 32-bit arithmetic (boxes
+// hold fewer than 2^32 cells: check_grid)
+__device__ inline void box_zyx(long long q, int b, int c, int& z, int& y, int& x) {
+    const unsigned u = static_cast<unsigned>(q);
+    const unsigned t = u / static_cast<unsigned>(c);
+    x = static_cast<int>(u - t * static_cast<unsigned>(c));
+    const unsigned zz = t / static_cast<unsigned>(b);
+    y = static_cast<int>(t - zz * static_cast<unsigned>(b));
+    z = static_cast<int>(zz);
+}
+
 __global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double* __restrict__ phi,
                                                        const double* __restrict__ phim, const double* __restrict__ resid,
                                                        double resid_scale, double* __restrict__ part) {
@@ -866,9 +880,8 @@ __global__ __launch_bounds__(kNetThreads) void k_scales(FieldGeo F, const double
     double m0 = 0., m1 = 0., m2 = 0.;
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
     for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int x = static_cast<int>(q % F.n2);
-        const int y = static_cast<int>((q / F.n2) % F.n1);
-        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
+        int z, y, x;
+        box_zyx(q, F.n1, F.n2, z, y, x);
         const double p = phi[pad_index(F, z, y, x)];
         const double pm = phim[pad_index(F, z, y, x)];
         m0 = fmax(m0, fabs(p));
@@ -912,61 +925,101 @@ struct Extras {
     int n;
 };
 
+template <int CF>
 __global__ __launch_bounds__(kNetThreads) void k_pack(FieldGeo F, const double* __restrict__ phi,
                                                      const double* __restrict__ phim, const double* __restrict__ resid,
                                                      double resid_scale, Extras ex, const double* __restrict__ s,
                                                      uint16_t* __restrict__ x0) {
-    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    // channel values of a voxel: their mean over its CF^3 cells (CF = 1: the cell's values)
+    const long long n = static_cast<long long>(F.c0) * F.c1 * F.c2;
     const double i0 = 1. / s[0], i1 = 1. / s[1], i2 = 1. / s[2];
+    const double avg = 1. / (CF * CF * CF);
     for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int x = static_cast<int>(q % F.n2);
-        const int y = static_cast<int>((q / F.n2) % F.n1);
-        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
-        const long long pi = pad_index(F, z, y, x);
-        const double p = phi[pi];
+        int z, y, x;
+        box_zyx(q, F.c1, F.c2, z, y, x);
+        double acc[kInChannels];
+        for (int c = 0; c < kInChannels; c++) acc[c] = 0.;
+        for (int a = 0; a < CF; a++)
+            for (int b = 0; b < CF; b++)
+                for (int d = 0; d < CF; d++) {
+                    const int zz = CF * z + a, yy = CF * y + b, xx = CF * x + d;
+                    const long long pi = pad_index(F, zz, yy, xx);
+                    const double p = phi[pi];
+                    acc[0] += p * i0;
+                    acc[1] += (p - phim[pi]) * i1;
+                    acc[2] += resid[row_index(F, zz, yy, xx)] * resid_scale * i2;
+                    for (int c = 3; c < kInChannels; c++) {
+                        const int e = c - 3;
+                        if (e < ex.n) acc[c] += (ex.f[e][pi] - ex.offset[e]) * ex.scale[e];
+                    }
+                }
         uint16_t ch[kInChannels];
-        ch[0] = f2bf(static_cast<float>(p * i0));
-        ch[1] = f2bf(static_cast<float>((p - phim[pi]) * i1));
-        ch[2] = f2bf(static_cast<float>(resid[row_index(F, z, y, x)] * resid_scale * i2));
-        for (int c = 3; c < kInChannels; c++) {
-            const int e = c - 3;
-            ch[c] = e < ex.n ? f2bf(static_cast<float>((ex.f[e][pi] - ex.offset[e]) * ex.scale[e])) : uint16_t(0);
-        }
+        for (int c = 0; c < kInChannels; c++) ch[c] = f2bf(static_cast<float>(acc[c] * avg));
         for (int c = 0; c < kInChannels; c++) x0[q * kInChannels + c] = ch[c];
     }
 }
 
 // x[row] = base[row] + s2 * out (the guess, solver row order; base = the classical guess
-// whose residual the network saw)
+// whose residual the network saw). CF = 2: out is read at the cell through the trilinear
+// interpolation of the voxel values (cell centres: weights 3/4 and 1/4 of the two nearest
+// voxels per axis, the nearest one repeated at the box faces).
+template <int CF>
 __global__ __launch_bounds__(kNetThreads) void k_unpack(FieldGeo F, const double* __restrict__ base,
                                                        const float* __restrict__ out, const double* __restrict__ s,
                                                        double* __restrict__ xg) {
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
     const double s2 = s[2];
     for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int x = static_cast<int>(q % F.n2);
-        const int y = static_cast<int>((q / F.n2) % F.n1);
-        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
+        int z, y, x;
+        box_zyx(q, F.n1, F.n2, z, y, x);
         const long long r = row_index(F, z, y, x);
-        xg[r] = base[r] + s2 * static_cast<double>(out[q]);
+        double o;
+        if (CF == 1) {
+            o = static_cast<double>(out[q]);
+        } else {
+            const int p[3] = {z, y, x}, nc[3] = {F.c0, F.c1, F.c2};
+            int v0[3], v1[3];
+            for (int d = 0; d < 3; d++) {
+                v0[d] = p[d] >> 1;
+                const int other = v0[d] + ((p[d] & 1) ? 1 : -1);
+                v1[d] = other < 0 ? 0 : (other >= nc[d] ? nc[d] - 1 : other);
+            }
+            o = 0.;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    for (int c = 0; c < 2; c++) {
+                        const double w = (a ? 0.25 : 0.75) * (b ? 0.25 : 0.75) * (c ? 0.25 : 0.75);
+                        const long long vz = a ? v1[0] : v0[0], vy = b ? v1[1] : v0[1], vx = c ? v1[2] : v0[2];
+                        o += w * static_cast<double>(out[(vz * F.c1 + vy) * F.c2 + vx]);
+                    }
+        }
+        xg[r] = base[r] + s2 * o;
     }
 }
 
-// loss = mean((out - t)^2), t = (x_true - base) / s2; dout[v][0] = 2 (out - t) / n, other channels 0.
+// loss = mean((out - t)^2) over the voxels, t = the mean over the voxel's cells of
+// (x_true - base) / s2; dout[v][0] = 2 (out - t) / n, other channels 0.
+template <int CF>
 __global__ __launch_bounds__(kNetThreads) void k_mse(FieldGeo F, const double* __restrict__ base,
                                                     const double* __restrict__ xt, const float* __restrict__ out,
                                                     const double* __restrict__ s, uint16_t* __restrict__ dout,
                                                     double* __restrict__ part) {
     __shared__ double red[kNetThreads / 64];
-    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    const long long n = static_cast<long long>(F.c0) * F.c1 * F.c2;
     const double inv = 1. / s[2];
+    const double avg = 1. / (CF * CF * CF);
     double acc = 0.;
     for (long long q = blockIdx.x * (long long)kNetThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kNetThreads) {
-        const int x = static_cast<int>(q % F.n2);
-        const int y = static_cast<int>((q / F.n2) % F.n1);
-        const int z = static_cast<int>(q / (static_cast<long long>(F.n2) * F.n1));
-        const long long r = row_index(F, z, y, x);
-        const double t = (xt[r] - base[r]) * inv;
+        int z, y, x;
+        box_zyx(q, F.c1, F.c2, z, y, x);
+        double tsum = 0.;
+        for (int a = 0; a < CF; a++)
+            for (int b = 0; b < CF; b++)
+                for (int c = 0; c < CF; c++) {
+                    const long long r = row_index(F, CF * z + a, CF * y + b, CF * x + c);
+                    tsum += (xt[r] - base[r]) * inv;
+                }
+        const double t = tsum * avg;
         const double d = static_cast<double>(out[q]) - t;
         acc += d * d;
         dout[q * kOutWidth] = f2bf(static_cast<float>(2. * d / static_cast<double>(n)));
@@ -1035,6 +1088,7 @@ struct dare_convnet {
     int in_channels = 3;
     int max_dims[3] = {0, 0, 0};
     int dims[3] = {0, 0, 0};              // current full-resolution dims (divisible by 4)
+    int cf = 1;                           // the network sees the grid at 1/cf resolution
     long long max_v0 = 0;
     // parameters
     long long off[kLayers + 1] = {};
@@ -1061,6 +1115,11 @@ struct dare_convnet {
     double* part = nullptr;               // 3 * 1024
     double* loss = nullptr;
     double* h_pinned = nullptr;           // 4
+    // dare_convnet_predict: pack -> forward (inference) -> unpack captured once as a HIP graph
+    // on a private stream and replayed while every argument stays the same
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t predict_exec = nullptr;
+    std::vector<unsigned char> predict_key;
 };
 
 namespace {
@@ -1228,7 +1287,7 @@ int wgrad_any(dare_convnet* net, hipStream_t sm, int layer, const uint16_t* X, c
     return DARE_OK;
 }
 
-FieldGeo field_geo(const dare_grid* g, const int32_t* crop) {
+FieldGeo field_geo(const dare_grid* g, const int32_t* crop, int cf) {
     FieldGeo F;
     F.n0 = crop ? crop[3] : g->n[0];
     F.n1 = crop ? crop[4] : g->n[1];
@@ -1242,6 +1301,10 @@ FieldGeo field_geo(const dare_grid* g, const int32_t* crop) {
     const long long p1 = g->n[1] + 2 * g->ghost, p2 = g->n[2] + 2 * g->ghost;
     F.s1 = p2;
     F.s0 = p1 * p2;
+    F.cf = cf;
+    F.c0 = F.n0 / cf;
+    F.c1 = F.n1 / cf;
+    F.c2 = F.n2 / cf;
     return F;
 }
 
@@ -1253,10 +1316,14 @@ int check_grid(const dare_convnet* net, const dare_grid* g, const int32_t* crop)
         if (crop)
             DARE_CHECK_ARG(crop[d] >= 0 && crop[3 + d] >= 1 && crop[d] + crop[3 + d] <= g->n[d],
                            "crop outside the grid");
-        DARE_CHECK_ARG(box[d] % 4 == 0 && box[d] >= 4, "network box sizes must be multiples of 4");
-        DARE_CHECK_ARG(box[d] <= net->max_dims[d], "box larger than the network was created for");
+        DARE_CHECK_ARG(box[d] % (4 * net->cf) == 0 && box[d] >= 4 * net->cf,
+                       net->cf == 1 ? "network box sizes must be multiples of 4"
+                                    : "network box sizes must be multiples of 8 (coarsening 2)");
+        DARE_CHECK_ARG(box[d] / net->cf <= net->max_dims[d], "box larger than the network was created for");
     }
-    DARE_CHECK_ARG(static_cast<long long>(box[0]) * box[1] * box[2] <= net->max_v0, "box too large");
+    DARE_CHECK_ARG(static_cast<long long>(box[0] / net->cf) * (box[1] / net->cf) * (box[2] / net->cf) <= net->max_v0,
+                   "box too large");
+    DARE_CHECK_ARG(static_cast<long long>(box[0]) * box[1] * box[2] < (1LL << 32), "box too large");
     return DARE_OK;
 }
 
@@ -1340,6 +1407,8 @@ int dare_convnet_destroy(dare_convnet* net) {
     if (!net) return DARE_OK;
     (void)hipSetDevice(net->device);
     (void)hipDeviceSynchronize();
+    if (net->predict_exec) (void)hipGraphExecDestroy(net->predict_exec);
+    if (net->cap_stream) (void)hipStreamDestroy(net->cap_stream);
     void* ptrs[] = {net->w, net->g, net->m, net->v, net->wf, net->wb, net->ws6, net->ws8, net->x0, net->a1, net->e1, net->p1, net->a3,
                     net->e2, net->p2, net->a5, net->e3, net->u2, net->a7, net->s2, net->d2, net->u1, net->a9,
                     net->s1, net->d1, net->out, net->dout, net->t0a, net->t0b, net->t0c, net->t0w, net->h1a,
@@ -1398,11 +1467,12 @@ int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
     int rc = check_grid(net, g, crop);
     if (rc) return rc;
     DARE_CHECK_ARG(nextra >= 0 && nextra <= net->in_channels - 3 && nextra <= 8, "too many extra channels");
-    const FieldGeo F = field_geo(g, crop);
-    net->dims[0] = F.n0;
-    net->dims[1] = F.n1;
-    net->dims[2] = F.n2;
+    const FieldGeo F = field_geo(g, crop, net->cf);
+    net->dims[0] = F.c0;
+    net->dims[1] = F.c1;
+    net->dims[2] = F.c2;
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    const long long nv = static_cast<long long>(F.c0) * F.c1 * F.c2;
     Extras ex{};
     ex.n = nextra;
     for (int e = 0; e < nextra; e++) {
@@ -1415,8 +1485,12 @@ int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
     hipLaunchKernelGGL(k_scales, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid, resid_scale,
                        net->part);
     hipLaunchKernelGGL(k_scales_finish, dim3(1), dim3(192), 0, ctx->stream, net->part, static_cast<int>(gs), net->scales);
-    hipLaunchKernelGGL(k_pack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid,
-                       resid_scale, ex, net->scales, net->x0);
+    if (net->cf == 2)
+        hipLaunchKernelGGL(k_pack<2>, dim3(ew_grid(nv)), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid,
+                           resid_scale, ex, net->scales, net->x0);
+    else
+        hipLaunchKernelGGL(k_pack<1>, dim3(ew_grid(nv)), dim3(kNetThreads), 0, ctx->stream, F, phi, phi_prev, resid,
+                           resid_scale, ex, net->scales, net->x0);
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }
@@ -1458,18 +1532,29 @@ int dare_convnet_set_inference(dare_convnet* net, int32_t on) {
     return DARE_OK;
 }
 
+int dare_convnet_set_coarsening(dare_convnet* net, int32_t factor) {
+    DARE_CHECK_ARG(net, "NULL argument");
+    DARE_CHECK_ARG(factor == 1 || factor == 2, "coarsening factor must be 1 or 2");
+    net->cf = factor;
+    return DARE_OK;
+}
+
 // x_guess[row] = base[row] + S * out[row]
 int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* base, double* x_guess,
                         const int32_t* crop) {
     DARE_CHECK_ARG(ctx && net && base && x_guess, "NULL argument");
     int rc = check_grid(net, g, crop);
     if (rc) return rc;
-    const FieldGeo F = field_geo(g, crop);
-    DARE_CHECK_ARG(F.n0 == net->dims[0] && F.n1 == net->dims[1] && F.n2 == net->dims[2],
+    const FieldGeo F = field_geo(g, crop, net->cf);
+    DARE_CHECK_ARG(F.c0 == net->dims[0] && F.c1 == net->dims[1] && F.c2 == net->dims[2],
                    "box differs from the one of the last forward pass");
     const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
-    hipLaunchKernelGGL(k_unpack, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, base, net->out, net->scales,
-                       x_guess);
+    if (net->cf == 2)
+        hipLaunchKernelGGL(k_unpack<2>, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, base, net->out,
+                           net->scales, x_guess);
+    else
+        hipLaunchKernelGGL(k_unpack<1>, dim3(ew_grid(n)), dim3(kNetThreads), 0, ctx->stream, F, base, net->out,
+                           net->scales, x_guess);
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }
@@ -1484,13 +1569,17 @@ int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
         rc = dalloc(&net->dout, net->max_v0 * 16);
         if (rc) return rc;
     }
-    const FieldGeo F = field_geo(g, crop);
-    DARE_CHECK_ARG(F.n0 == net->dims[0] && F.n1 == net->dims[1] && F.n2 == net->dims[2],
+    const FieldGeo F = field_geo(g, crop, net->cf);
+    DARE_CHECK_ARG(F.c0 == net->dims[0] && F.c1 == net->dims[1] && F.c2 == net->dims[2],
                    "box differs from the one of the last forward pass");
-    const long long n = static_cast<long long>(F.n0) * F.n1 * F.n2;
+    const long long n = static_cast<long long>(F.c0) * F.c1 * F.c2;
     const unsigned gs = std::min<unsigned>(ew_grid(n), 1024);
-    hipLaunchKernelGGL(k_mse, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, base, x_true, net->out, net->scales,
-                       net->dout, net->part);
+    if (net->cf == 2)
+        hipLaunchKernelGGL(k_mse<2>, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, base, x_true, net->out,
+                           net->scales, net->dout, net->part);
+    else
+        hipLaunchKernelGGL(k_mse<1>, dim3(gs), dim3(kNetThreads), 0, ctx->stream, F, base, x_true, net->out,
+                           net->scales, net->dout, net->part);
     hipLaunchKernelGGL(k_mse_finish, dim3(1), dim3(64), 0, ctx->stream, net->part, static_cast<int>(gs), n, net->loss);
     DARE_LAUNCH_CHECK();
     if (loss_host) {
@@ -1611,6 +1700,75 @@ int dare_convnet_tensors(const dare_convnet* net, void** ptrs /* 19 */, double* 
                    net->e3, net->u2, net->a7, net->s2, net->d2, net->u1, net->a9, net->s1, net->d1};
     for (int q = 0; q < 19; q++) ptrs[q] = p[q];
     if (scales_host) DARE_HIP_TRY(hipMemcpy(scales_host, net->scales, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return DARE_OK;
+}
+
+// pack -> forward in inference mode -> unpack, replayed from a HIP graph: the kernels of the
+// three steps are captured on the network's own stream at the first call and whenever an
+// argument (a pointer, the grid, a scale) differs from the captured call's, then launched as
+// one graph on the context's stream. The graph holds kernels only.
+int dare_convnet_predict(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi,
+                         const double* phi_prev, const double* resid, double resid_scale, int32_t nextra,
+                         const double* const* extras, const double* extra_scale, const double* extra_offset,
+                         const double* x_base, double* x_guess) {
+    DARE_CHECK_ARG(ctx && net && g && phi && phi_prev && resid && x_base && x_guess, "NULL argument");
+    DARE_CHECK_ARG(nextra >= 0 && nextra <= 8 && (nextra == 0 || (extras && extra_scale && extra_offset)),
+                   "invalid extra fields");
+    std::vector<unsigned char> key;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        key.insert(key.end(), c, c + n);
+    };
+    put(g, sizeof(dare_grid));
+    const void* ptrs[5] = {phi, phi_prev, resid, x_base, x_guess};
+    put(ptrs, sizeof(ptrs));
+    put(&resid_scale, sizeof(double));
+    put(&nextra, sizeof(nextra));
+    for (int e = 0; e < nextra; e++) {
+        put(&extras[e], sizeof(void*));
+        put(&extra_scale[e], sizeof(double));
+        put(&extra_offset[e], sizeof(double));
+    }
+    put(&net->cf, sizeof(int));
+    put(&ctx->device, sizeof(int));
+    if (!net->predict_exec || key != net->predict_key) {
+        if (net->predict_exec) (void)hipGraphExecDestroy(net->predict_exec);
+        net->predict_exec = nullptr;
+        net->predict_key.clear();
+        if (!net->cap_stream) DARE_HIP_TRY(hipStreamCreateWithFlags(&net->cap_stream, hipStreamNonBlocking));
+        dare_ctx cc = *ctx;          // the same device, the network's capture stream
+        cc.stream = net->cap_stream;
+        const bool was_inference = net->inference;
+        net->inference = true;
+        DARE_HIP_TRY(hipStreamBeginCapture(net->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = dare_convnet_pack(&cc, net, g, phi, phi_prev, resid, resid_scale, nextra, extras, extra_scale,
+                                   extra_offset, nullptr);
+        if (rc == DARE_OK) rc = dare_conv_net_fwd(&cc, net);
+        if (rc == DARE_OK) rc = dare_convnet_unpack(&cc, net, g, x_base, x_guess, nullptr);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(net->cap_stream, &graph);
+        net->inference = was_inference;
+        if (rc != DARE_OK || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            if (rc == DARE_OK) {
+                set_error("graph capture of the prediction failed");
+                rc = DARE_ERR_HIP;
+            }
+            return rc;
+        }
+        const hipError_t ei = hipGraphInstantiate(&net->predict_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            (void)hipGetLastError();
+            net->predict_exec = nullptr;
+            set_error("graph instantiation of the prediction failed");
+            return DARE_ERR_HIP;
+        }
+        net->predict_key = key;
+    }
+    DARE_HIP_TRY(hipGraphLaunch(net->predict_exec, ctx->stream));
+    net->acts_for_bwd = false;
     return DARE_OK;
 }
 

@@ -532,21 +532,31 @@ public:
                      const std::shared_ptr<ParameterList>& params) {
         return Solve(pack, type, nullptr, A, x, B, params);
     }
-    // TrilinosSolver.inl:97-127 + CreatePreconditionerIfPack2 (:164-175). This build provides
-    // Ifpack2 "RELAXATION" of type "Jacobi" with one sweep and damping 1, i.e. M^-1 = diag(A)^-1,
-    // which the device solver applies on the right; None returns null as the reference does.
-    // Other types throw (the reference prints an error and terminates).
+    // TrilinosSolver.inl:97-127 + CreatePreconditionerIfPack2 (:164-175) and
+    // CreatePreconditionerMueLu (:180-246). This build provides
+    //  - Ifpack2 "RELAXATION" of type "Jacobi" with one sweep and damping 1, i.e.
+    //    M^-1 = diag(A)^-1 (any operator);
+    //  - MueLu "AMG": the aggregation multigrid V-cycle of the device solver (2x2x2 aggregates,
+    //    damped-Jacobi smoothing), for device-assembled one-component operators on a 3-D
+    //    Cartesian grid (Build(grep, field, terms)); MueLu's parameter list ("problem: type",
+    //    "verbosity") is accepted and not used, and the solve refuses operators the hierarchy
+    //    cannot coarsen (generic CSR from a host functor, two components).
+    // Both are applied on the right; None returns null as the reference does. Other types throw
+    // (the reference prints an error and terminates).
     std::shared_ptr<Preconditioner> BuildPreconditioner(PreCondPackage pack, const std::string& type,
                                                         const std::shared_ptr<ParameterList>& param,
                                                         const std::shared_ptr<DeviceCsr>&) {
         if (pack == PreCondPackage::None) return nullptr;
+        if (pack == PreCondPackage::MueLu && type == "AMG")
+            return std::make_shared<Preconditioner>(Preconditioner{DARE_PRECOND_MG});
         const std::string relax = param ? param->get<std::string>("relaxation: type", "Jacobi") : "Jacobi";
         const int sweeps = param ? param->get<int>("relaxation: sweeps", 1) : 1;
         const double omega = param ? param->get<double>("relaxation: damping factor", 1.) : 1.;
         if (pack == PreCondPackage::Ifpack2 && type == "RELAXATION" && relax == "Jacobi" && sweeps == 1 && omega == 1.)
             return std::make_shared<Preconditioner>(Preconditioner{DARE_PRECOND_JACOBI});
         throw std::runtime_error("Preconditioner of type " + type +
-                                 " is unknown to this build (Ifpack2 RELAXATION / Jacobi, one sweep, damping 1)");
+                                 " is unknown to this build (Ifpack2 RELAXATION / Jacobi, one sweep, damping 1;"
+                                 " MueLu AMG)");
     }
     int GetNumIterations() const { return num_iter_; }      // TrilinosSolver.inl:263-265
     double GetResidual() const { return residual_; }

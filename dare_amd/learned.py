@@ -37,6 +37,8 @@ def _sig():
         "dare_convnet_pack": (i32, [vp, vp, G, vp, vp, vp, dbl, i32, ctypes.POINTER(vp), vp, vp, vp]),
         "dare_conv_net_fwd": (i32, [vp, vp]),
         "dare_convnet_set_inference": (i32, [vp, i32]),
+        "dare_convnet_set_coarsening": (i32, [vp, i32]),
+        "dare_convnet_predict": (i32, [vp, vp, G, vp, vp, vp, dbl, i32, ctypes.POINTER(vp), vp, vp, vp, vp]),
         "dare_convnet_unpack": (i32, [vp, vp, G, vp, vp, vp]),
         "dare_convnet_loss": (i32, [vp, vp, G, vp, vp, ctypes.POINTER(dbl), vp]),
         "dare_conv_net_bwd": (i32, [vp, vp]),
@@ -69,14 +71,21 @@ def he_init(in_channels, seed=0, out_scale=1e-3):
 
 
 class WarmStartNet:
-    def __init__(self, ctx, max_dims, in_channels=3, seed=0, params=None):
+    """max_dims: the largest grid box the network will see. coarsen = 2 runs the network at half
+    resolution (2x2x2 means in, trilinear interpolation of the correction out): 8x fewer
+    voxels for the smooth part of the step, which is what the solver is slow to remove."""
+
+    def __init__(self, ctx, max_dims, in_channels=3, seed=0, params=None, coarsen=1):
         L = _sig()
         self.ctx = ctx
         self.in_channels = in_channels
+        self.coarsen = int(coarsen)
         self.h = ctypes.c_void_p()
         self._inference = False
-        dims = (ctypes.c_int32 * 3)(*max_dims)
+        dims = (ctypes.c_int32 * 3)(*[d // self.coarsen for d in max_dims])
         check(L.dare_convnet_create(ctx.h, in_channels, dims, ctypes.byref(self.h)), "dare_convnet_create")
+        if self.coarsen != 1:
+            check(L.dare_convnet_set_coarsening(self.h, self.coarsen), "dare_convnet_set_coarsening")
         n = ctypes.c_int64()
         offs = (ctypes.c_int64 * 12)()
         check(L.dare_convnet_num_params(self.h, ctypes.byref(n), offs), "dare_convnet_num_params")
@@ -103,13 +112,21 @@ class WarmStartNet:
         return g
 
     def save(self, path, meta=None):
-        info = {"in_channels": self.in_channels, "layers": LAYERS, **(meta or {})}
+        info = {"in_channels": self.in_channels, "layers": LAYERS, "coarsen": self.coarsen, **(meta or {})}
         np.savez(path, params=self.params(), meta=np.array(json.dumps(info)))
 
     @staticmethod
     def load_params(path):
+        """(parameters, meta); meta["coarsen"] is 1 for networks saved before coarsening existed."""
         with np.load(path, allow_pickle=False) as z:
-            return z["params"].astype(np.float32), json.loads(str(z["meta"]))
+            meta = json.loads(str(z["meta"]))
+            meta.setdefault("coarsen", 1)
+            return z["params"].astype(np.float32), meta
+
+    @classmethod
+    def load(cls, ctx, path, max_dims):
+        w, meta = cls.load_params(path)
+        return cls(ctx, max_dims, in_channels=meta["in_channels"], params=w, coarsen=meta["coarsen"])
 
     # ------------------------------------------------------------------ passes
     @staticmethod
@@ -137,7 +154,18 @@ class WarmStartNet:
                                          self._crop(crop)), "dare_convnet_unpack")
 
     def predict(self, grid, phi, phi_prev, x_base, resid, resid_scale, x_out, extras=()):
-        """x_out (solver rows) = x_base + the network's correction; resid = b - A x_base."""
+        """x_out (solver rows) = x_base + the network's correction; resid = b - A x_base. One
+        graph launch (dare_convnet_predict), the same result as pack -> forward -> unpack."""
+        ne = len(extras)
+        fptr = (ctypes.c_void_p * max(ne, 1))(*[_ptr(f) for f, _, _ in extras])
+        sc = np.array([s for _, s, _ in extras] or [0.0], np.float64)
+        of = np.array([o for _, _, o in extras] or [0.0], np.float64)
+        check(_sig().dare_convnet_predict(self.ctx.h, self.h, ctypes.byref(grid), _ptr(phi), _ptr(phi_prev),
+                                          _ptr(resid), float(resid_scale), ne, fptr, sc.ctypes.data, of.ctypes.data,
+                                          _ptr(x_base), _ptr(x_out)), "dare_convnet_predict")
+
+    def predict_eager(self, grid, phi, phi_prev, x_base, resid, resid_scale, x_out, extras=()):
+        """predict as three separate calls (no graph)."""
         self.pack(grid, phi, phi_prev, resid, resid_scale, extras)
         self.forward(inference=True)
         self.unpack(grid, x_base, x_out)

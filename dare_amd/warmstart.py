@@ -5,6 +5,7 @@
     python -m dare_amd.warmstart train --n 256 --workload vardiff --train-seeds 400 --steps 6 \
         --crop 64 --lr 1e-3 --lr-final 5e-5 --weights dare_amd/weights/ws_vardiff_128.npz \
         --out dare_amd/weights/ws_vardiff_256.npz        (fine-tuning at 256^3)
+    python -m dare_amd.warmstart train --n 128 --coarsen 2 --crop 64 ...   (half resolution)
 
 train: for every training seed, the plain FP64 BiCGStab2 advances the advection-diffusion
 trajectory (C5AdvDiff) from the reference's guess (phi^n); each step's (phi^n, phi^{n-1},
@@ -35,13 +36,16 @@ def _log(msg):
 
 
 def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_final=1e-4, net=None,
-          train_every=1, log_every=60.0, max_seconds=None, seed=0, workload="advdiff", crop=None):
+          train_every=1, log_every=60.0, max_seconds=None, seed=0, workload="advdiff", crop=None, coarsen=1):
     """Online training on plain-solver trajectories. With `crop` = c, each step trains on one
-    random c^3 sub-box of the grid (the network is fully convolutional). Returns (net, info)."""
+    random c^3 sub-box of the grid (the network is fully convolutional; with coarsening 2 the
+    crop origin is even, so the sub-box's 2x2x2 blocks are blocks of the grid). Returns
+    (net, info)."""
     import torch
     make = WORKLOADS[workload]
     crng = np.random.default_rng(seed + 17)
-    net = net or WarmStartNet(ctx, [n, n, n], in_channels=make.in_channels_for(), seed=seed)
+    net = net or WarmStartNet(ctx, [n, n, n], in_channels=make.in_channels_for(), seed=seed, coarsen=coarsen)
+    cf = net.coarsen
     seeds = list(train_seeds)
     total = len(seeds) * max(steps - 1, 1) // train_every
     done, t_start, t_log = 0, time.perf_counter(), time.perf_counter()
@@ -59,7 +63,7 @@ def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_fina
                 cur_lr = lr * (lr_final / lr) ** frac
                 box = None
                 if crop:
-                    o = [int(crng.integers(0, n - crop + 1)) for _ in range(3)]
+                    o = [cf * int(crng.integers(0, (n - crop) // cf + 1)) for _ in range(3)]
                     box = o + [crop] * 3
                 losses.append(net.train_step(W.grid, W.phi, W.phi_prev, W.xb, W.r0, W.rhs_scale, W.x, lr=cur_lr,
                                              extras=W.extras, crop=box))
@@ -76,6 +80,7 @@ def train(ctx, n=128, train_seeds=range(1000, 1200), steps=200, lr=1e-3, lr_fina
             break
     torch.cuda.synchronize()
     info = {"workload": workload, "samples": done, "seeds_used": si + 1, "steps": steps, "n": n, "crop": crop,
+            "coarsen": cf,
             "seconds": round(time.perf_counter() - t_start, 1),
             "loss_first_100": float(np.mean(losses[:100])) if losses else None,
             "loss_last_100": float(np.mean(losses[-100:])) if losses else None,
@@ -165,6 +170,7 @@ def main(argv=None):
     ap.add_argument("--lr-final", type=float, default=1e-4)
     ap.add_argument("--train-every", type=int, default=1)
     ap.add_argument("--crop", type=int, default=None)
+    ap.add_argument("--coarsen", type=int, default=None, help="1 or 2 (default: the weights' own, else 1)")
     ap.add_argument("--max-seconds", type=float, default=None)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--out", default=None)
@@ -174,10 +180,12 @@ def main(argv=None):
     report = {}
     net = None
     if a.cmd in ("train", "train-eval"):
-        init = None
+        init, cf = None, a.coarsen or 1
         if a.weights:
-            init, _ = WarmStartNet.load_params(a.weights)
-        net = WarmStartNet(ctx, [a.n] * 3, in_channels=WORKLOADS[a.workload].in_channels_for(), params=init)
+            init, meta = WarmStartNet.load_params(a.weights)
+            cf = a.coarsen or meta["coarsen"]
+        net = WarmStartNet(ctx, [a.n] * 3, in_channels=WORKLOADS[a.workload].in_channels_for(), params=init,
+                           coarsen=cf)
         net, info = train(ctx, n=a.n, train_seeds=range(1000, 1000 + a.train_seeds), steps=a.steps, lr=a.lr,
                           lr_final=a.lr_final, net=net, train_every=a.train_every, max_seconds=a.max_seconds, workload=a.workload,
                           crop=a.crop)
@@ -187,8 +195,7 @@ def main(argv=None):
             net.save(a.out, meta={"train": info, "workload": a.workload, "tol": TOL})
     if a.cmd in ("eval", "train-eval"):
         if net is None:
-            w, meta = WarmStartNet.load_params(a.weights)
-            net = WarmStartNet(ctx, [a.n] * 3, in_channels=meta["in_channels"], params=w)
+            net = WarmStartNet.load(ctx, a.weights, [a.n] * 3)
         ev = evaluate(ctx, net, n=a.n, eval_seeds=range(5000, 5000 + a.eval_seeds), steps=a.eval_steps or a.steps,
                       workload=a.workload)
         report["eval"] = ev

@@ -304,6 +304,22 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net);
  * traffic per forward pass); every other tensor and the output are unchanged. While the last
  * forward pass ran in this mode dare_conv_net_bwd refuses with DARE_ERR_INVALID. */
 int dare_convnet_set_inference(dare_convnet* net, int32_t on);
+/* Coarsening (1 after create): with factor 2 the network runs at half resolution. pack feeds
+ * each network voxel the mean of the channel values of its 2x2x2 cells; unpack adds S times
+ * the trilinear interpolation of the output at every cell centre (weights 3/4, 1/4 of the two
+ * nearest voxels per axis, the nearest repeated at the box faces); loss compares the output
+ * with the 2x2x2 mean of (x_true - x_base) / S. Boxes are then multiples of 8 whose halves fit
+ * max_dims, which counts network voxels. No reference counterpart. */
+int dare_convnet_set_coarsening(dare_convnet* net, int32_t factor);
+/* pack -> fwd (inference mode) -> unpack of the whole grid in one call: the three steps'
+ * kernels are captured as a HIP graph at the first call (and again whenever any argument
+ * differs from the captured call's) and replayed on the context's stream; the results equal
+ * those of the three separate calls. Not for use while the context's stream is itself being
+ * captured. No reference counterpart. */
+int dare_convnet_predict(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* phi,
+                         const double* phi_prev, const double* resid, double resid_scale, int32_t nextra,
+                         const double* const* extras, const double* extra_scale, const double* extra_offset,
+                         const double* x_base, double* x_guess);
 int dare_convnet_unpack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* x_base, double* x_guess,
                         const int32_t* crop);
 int dare_convnet_loss(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, const double* x_base, const double* x_true,

@@ -239,6 +239,68 @@ def loss_and_dout(out, phi_int, x_true_int, s2, exact=False):
     return float(np.mean(d * d)), (g if exact else bf16(g))
 
 
+def _block_sum(a, cf=2):
+    """Sum over cf^3 blocks of a [D, H, W] array, added in the order (a, b, c) of the block
+    offsets along (z, y, x), c fastest (the device's order, so the doubles agree bit for bit)."""
+    acc = np.zeros(tuple(m // cf for m in a.shape))
+    for i in range(cf):
+        for j in range(cf):
+            for k in range(cf):
+                acc = acc + a[i::cf, j::cf, k::cf]
+    return acc
+
+
+def pack_coarse(phi_int, phim_int, resid_int, resid_scale, extras=(), in_ch=8, cf=2):
+    """Input tensor at 1/cf resolution [D/cf, H/cf, W/cf, 8]: each voxel the mean of the
+    channel values (as in pack) of its cf^3 cells; the scales are those of the cells."""
+    s0, s1, s2 = scales(phi_int, phim_int, resid_int, resid_scale)
+    avg = 1.0 / cf ** 3
+    x = np.zeros(tuple(m // cf for m in phi_int.shape) + (in_ch,))
+    x[..., 0] = bf16(_block_sum(phi_int * (1.0 / s0), cf) * avg)
+    x[..., 1] = bf16(_block_sum((phi_int - phim_int) * (1.0 / s1), cf) * avg)
+    x[..., 2] = bf16(_block_sum(resid_int * resid_scale * (1.0 / s2), cf) * avg)
+    for e, (f, scale, offset) in enumerate(extras):
+        x[..., 3 + e] = bf16(_block_sum((f - offset) * scale, cf) * avg)
+    return x, (s0, s1, s2)
+
+
+def prolong_trilinear(out):
+    """Cell values of a half-resolution field [D, H, W] -> [2D, 2H, 2W]: per axis the cell
+    centre lies a quarter voxel from its voxel's centre, so it takes 3/4 of that voxel and 1/4
+    of the neighbour on its side (the voxel itself again at the box faces); the eight products
+    are added with the weight (wz wy) wx in the order of (z, y, x) choices, nearest first."""
+    nc = out.shape
+    idx = []
+    for d in range(3):
+        p = np.arange(2 * nc[d])
+        v0 = p >> 1
+        v1 = np.clip(v0 + np.where(p & 1, 1, -1), 0, nc[d] - 1)
+        idx.append((v0, v1))
+    res = np.zeros(tuple(2 * m for m in nc))
+    for a in range(2):
+        for b in range(2):
+            for c in range(2):
+                w = (0.25 if a else 0.75) * (0.25 if b else 0.75) * (0.25 if c else 0.75)
+                z = idx[0][a][:, None, None]
+                y = idx[1][b][None, :, None]
+                x = idx[2][c][None, None, :]
+                res = res + w * out[z, y, x]
+    return res
+
+
+def unpack_coarse(base_int, out, s2):
+    """The guess at the cells: base + s2 * (trilinear interpolation of the voxel output)."""
+    return base_int + s2 * prolong_trilinear(np.asarray(out, np.float64))
+
+
+def loss_and_dout_coarse(out, base_int, x_true_int, s2, cf=2, exact=False):
+    """MSE of the voxel output against the cf^3 mean of the step in units of s2."""
+    t = _block_sum((x_true_int - base_int) * (1.0 / s2), cf) * (1.0 / cf ** 3)
+    d = out - t
+    g = 2.0 * d / d.size
+    return float(np.mean(d * d)), (g if exact else bf16(g))
+
+
 def adam(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     """One Adam step (Kingma & Ba 2015, Algorithm 1) in float32, in the order the device
     kernel evaluates it: m = b1 m + (1 - b1) g; v = b2 v + (1 - b2) g g;

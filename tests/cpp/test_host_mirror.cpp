@@ -17,7 +17,8 @@
 //                       descriptor of a staggered representation keeps the grid's dx), and a
 //                       device-path solve with fixed boundary cells (SolveStaggeredY's profile)
 //   SolveJacobi       — SolveLaplace with Ifpack2 RELAXATION/Jacobi as M
-//                       (TrilinosSolver.inl:97-175), and the device fast path with and without it
+//                       (TrilinosSolver.inl:97-175), and the device fast path with none, Jacobi
+//                       and MueLu AMG (the aggregation multigrid, TrilinosSolver.inl:180-246)
 // Exit code 0 when every case passes.
 #include <cmath>
 #include <cstdio>
@@ -179,28 +180,38 @@ static void SolveJacobi() {
     for (int f = 0; f < 6; f++) terms.Dirichlet(f, 0.25 * f);
     auto params = std::make_shared<ParameterList>();
     params->set("Maximum Iterations", 500);
-    std::vector<double> sol[2];
-    int passes[2];
-    for (int use = 0; use < 2; use++) {
+    // none, Ifpack2 RELAXATION/Jacobi, MueLu AMG (the aggregation multigrid, with the
+    // reference test's MueLu parameters)
+    auto amg_param = std::make_shared<ParameterList>();
+    amg_param->set("problem: type", std::string("MHD"));
+    amg_param->set("verbosity", std::string("none"));
+    std::vector<double> sol[3];
+    int passes[3];
+    for (int use = 0; use < 3; use++) {
         Matrix::Trilinos<double> msystem;
         msystem.Build(grep, phi, terms, false);
         params->set("Convergence Tolerance", 1e-13 * msystem.GetB()->norm2());
-        auto M = use ? solver.BuildPreconditioner(Matrix::PreCondPackage::Ifpack2, "RELAXATION", nullptr, msystem.GetA())
-                     : nullptr;
+        std::shared_ptr<Matrix::Preconditioner> M;
+        if (use == 1) M = solver.BuildPreconditioner(Matrix::PreCondPackage::Ifpack2, "RELAXATION", nullptr, msystem.GetA());
+        if (use == 2) M = solver.BuildPreconditioner(Matrix::PreCondPackage::MueLu, "AMG", amg_param, msystem.GetA());
         auto ret = solver.Solve(Matrix::SolverPackage::BumbleBee, "BICGSTAB2", M, msystem.GetA(), msystem.GetX(),
                                 msystem.GetB(), params);
         EXPECT(ret == Matrix::ReturnType::Converged, "device solve %d did not converge", use);
         sol[use] = msystem.GetX()->getData();
         passes[use] = solver.GetNumIterations();
     }
-    double err = 0., mx = 0.;
+    double err[3] = {0., 0., 0.}, mx = 0.;
     for (std::size_t r = 0; r < sol[0].size(); r++) {
-        err = std::fmax(err, std::fabs(sol[0][r] - sol[1][r]));
+        for (int u = 1; u < 3; u++) err[u] = std::fmax(err[u], std::fabs(sol[0][r] - sol[u][r]));
         mx = std::fmax(mx, std::fabs(sol[0][r]));
     }
-    EXPECT(err <= 1e-10 * mx, "Jacobi solution differs by %g (max %g)", err, mx);
-    std::printf("  SolveJacobi device path: %d passes plain, %d with Jacobi, difference %.1e\n", passes[0], passes[1],
-                err / mx);
+    EXPECT(err[1] <= 1e-10 * mx, "Jacobi solution differs by %g (max %g)", err[1], mx);
+    EXPECT(err[2] <= 1e-10 * mx, "multigrid solution differs by %g (max %g)", err[2], mx);
+    // constant diffusivity: diag(A) is nearly a multiple of I, so Jacobi saves little here
+    EXPECT(passes[2] < passes[1] && passes[1] <= passes[0], "passes none %d, Jacobi %d, AMG %d", passes[0],
+           passes[1], passes[2]);
+    std::printf("  SolveJacobi device path: %d passes plain, %d with Jacobi, %d with AMG, difference %.1e / %.1e\n",
+                passes[0], passes[1], passes[2], err[1] / mx, err[2] / mx);
 }
 
 static void SolveScalar1D() {

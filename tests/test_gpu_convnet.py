@@ -258,3 +258,81 @@ def test_adam_step_matches_oracle(ctx, rng):
         assert np.any(w1 != w0)
         w0 = w1
     net.close()
+
+
+NC = [16, 24, 32]       # a box whose halves are multiples of 4
+
+
+def test_coarse_pack_unpack_loss_match_oracle(ctx, rng):
+    """Half-resolution mode (coarsening 2): the packed input, the interpolated guess and the
+    loss gradient are bit-exact against the oracle (oracle/convnet.py pack_coarse,
+    unpack_coarse, loss_and_dout_coarse); the loss value agrees to rounding."""
+    import torch
+    net = WarmStartNet(ctx, NC, in_channels=4, seed=3, coarsen=2)
+    net.set_params(he_init(4, seed=3, out_scale=1.0))
+    g = da.Grid.make(NC, size=[1, 1, 1], ghost=2)
+    dev, (phi, phim, rhs, ext) = fields(rng, NC)
+    extras = [(dev[3], 2.0, 0.5)]
+    net.pack(g, dev[0], dev[1], dev[2], 0.7, extras)
+    ptr, scales = net.tensors()
+    half = [m // 2 for m in NC]
+    x0 = d2h_bf16(ctx, ptr["x0"], half + [8])
+    exp, s = cn.pack_coarse(phi, phim, rhs.reshape(NC), 0.7, extras=[(ext, 2.0, 0.5)])
+    assert np.allclose(scales, s, rtol=0, atol=0)
+    assert np.array_equal(x0, exp)
+    net.forward()
+    base = dev[0].view(*[m + 4 for m in NC])[2:-2, 2:-2, 2:-2].contiguous().view(-1)
+    xg = torch.empty_like(base)
+    net.unpack(g, base, xg)
+    out = d2h_f32(ctx, ptr["out"], half)
+    assert np.array_equal(xg.cpu().numpy().reshape(NC), cn.unpack_coarse(phi, out, scales[2]))
+    x_true = torch.from_numpy(rng.normal(size=int(np.prod(NC)))).cuda()
+    loss = net.loss(g, base, x_true)
+    loss_ref, dout = cn.loss_and_dout_coarse(out, phi, x_true.cpu().numpy().reshape(NC), scales[2])
+    assert abs(loss - loss_ref) <= 1e-12 * abs(loss_ref)
+    ptr, _ = net.tensors()
+    got = d2h_bf16(ctx, ptr["dout"], half + [16])
+    assert np.array_equal(got[..., 0], dout) and not np.any(got[..., 1:])
+    net.close()
+
+
+def test_coarse_mode_refuses_boxes_that_do_not_halve(ctx):
+    net = WarmStartNet(ctx, [16, 16, 16], in_channels=3, coarsen=2)
+    g = da.Grid.make([12, 16, 16], size=[1, 1, 1], ghost=2)
+    import torch
+    z = torch.zeros(16 * 20 * 20, dtype=torch.float64, device="cuda")
+    r = torch.zeros(12 * 16 * 16, dtype=torch.float64, device="cuda")
+    with pytest.raises(da.DareError, match="multiples of 8"):
+        net.pack(g, z, z, r, 1.0)
+    net.close()
+
+
+@pytest.mark.parametrize("coarsen", [1, 2])
+def test_graph_predict_equals_eager_on_every_replay(ctx, rng, coarsen):
+    """dare_convnet_predict (one captured graph, replayed) against pack -> forward -> unpack
+    called separately: new data in the same buffers before every replay, eager work and
+    synchronisations between replays; every replay bit-identical to the eager result. A new
+    pointer re-captures."""
+    import torch
+    net = WarmStartNet(ctx, NC, in_channels=4, seed=3, coarsen=coarsen)
+    net.set_params(he_init(4, seed=3, out_scale=1.0))
+    g = da.Grid.make(NC, size=[1, 1, 1], ghost=2)
+    dev, _ = fields(rng, NC)
+    extras = [(dev[3], 2.0, 0.5)]
+    base = torch.empty(int(np.prod(NC)), dtype=torch.float64, device="cuda")
+    xg = torch.empty_like(base)
+    xe = torch.empty_like(base)
+    for rep in range(4):
+        for t in dev:
+            t.copy_(torch.from_numpy(rng.normal(size=t.numel())))
+        base.copy_(torch.from_numpy(rng.normal(size=base.numel())))
+        net.predict(g, dev[0], dev[1], base, dev[2], 0.7, xg, extras=extras)
+        ctx.synchronize()
+        net.predict_eager(g, dev[0], dev[1], base, dev[2], 0.7, xe, extras=extras)
+        ctx.synchronize()
+        assert torch.equal(xg, xe), rep
+    other = torch.empty_like(base)
+    net.predict(g, dev[0], dev[1], base, dev[2], 0.7, other, extras=extras)
+    ctx.synchronize()
+    assert torch.equal(other, xe)
+    net.close()

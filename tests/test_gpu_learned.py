@@ -24,8 +24,7 @@ def test_learned_guess_saves_passes(ctx):
     path = os.path.join(WEIGHTS, "ws_advdiff_128.npz")
     if not os.path.exists(path):
         pytest.skip("weights not trained yet")
-    w, meta = WarmStartNet.load_params(path)
-    net = WarmStartNet(ctx, [128] * 3, in_channels=meta["in_channels"], params=w)
+    net = WarmStartNet.load(ctx, path, [128] * 3)
     ev = evaluate(ctx, net, n=128, eval_seeds=[5000], steps=16, workload="advdiff")
     print({m: (round(ev[m]["passes_mean"], 2), ev[m]["initial_rel_residual_geomean"]) for m in ev})
     assert all(ev[m]["all_converged"] for m in ev)
@@ -42,8 +41,7 @@ def test_learned_guess_variable_diffusivity_reaches_tolerance(ctx):
     path = os.path.join(WEIGHTS, "ws_vardiff_128.npz")
     if not os.path.exists(path):
         pytest.skip("weights not trained yet")
-    w, meta = WarmStartNet.load_params(path)
-    net = WarmStartNet(ctx, [128] * 3, in_channels=meta["in_channels"], params=w)
+    net = WarmStartNet.load(ctx, path, [128] * 3)
     ev = evaluate(ctx, net, n=128, eval_seeds=[5001], steps=6, workload="vardiff")
     assert all(ev[m]["all_converged"] for m in ev)
     assert ev["learned"]["initial_rel_residual_geomean"] < ev["previous"]["initial_rel_residual_geomean"]
@@ -56,8 +54,7 @@ def test_learned_guess_reaches_fp64_tolerance(ctx):
     path = os.path.join(WEIGHTS, "ws_advdiff_128.npz")
     if not os.path.exists(path):
         pytest.skip("weights not trained yet")
-    w, meta = WarmStartNet.load_params(path)
-    net = WarmStartNet(ctx, [128] * 3, in_channels=meta["in_channels"], params=w)
+    net = WarmStartNet.load(ctx, path, [128] * 3)
     W = C5AdvDiff(ctx, 5001, n=128)
     xg = torch.empty_like(W.x)
     for k in range(4):

@@ -126,3 +126,67 @@ def test_adam_oracle_first_step_and_moments():
         w_new, mm, vv = cn.adam(w, g, mm, vv, t, lr)
         assert np.all(np.abs(w.astype(np.float64) - w_new.astype(np.float64) - exp) <= ulp + 1e-5 * lr)
         w = w_new
+
+
+def _expand(c):
+    """Each voxel repeated over its 2x2x2 cells."""
+    return np.repeat(np.repeat(np.repeat(c, 2, 0), 2, 1), 2, 2)
+
+
+def test_coarse_pack_of_block_constant_fields_is_the_pack_of_the_voxels(rng):
+    """Fields constant over every 2x2x2 block (dyadic values, so every sum is exact): the
+    half-resolution input is the full-resolution input of the voxel fields, bit for bit; and a
+    non-constant block gives its mean (a swapped or dropped cell changes the mean)."""
+    shape = (4, 6, 8)
+    c = [rng.integers(-64, 64, size=shape) / 64.0 for _ in range(4)]
+    f = [_expand(a) for a in c]
+    xc, sc = cn.pack_coarse(f[0], f[0] - f[1], f[2], 0.5, extras=[(f[3], 2.0, 0.25)])
+    xf, sf = cn.pack(c[0], c[0] - c[1], c[2], 0.5, extras=[(c[3], 2.0, 0.25)])
+    assert sc == sf
+    assert np.array_equal(xc, xf)
+    phi = rng.integers(-64, 64, size=(8, 8, 8)) / 64.0
+    x, (s0, _, _) = cn.pack_coarse(phi, phi, phi, 1.0)
+    mean = phi.reshape(4, 2, 4, 2, 4, 2).mean(axis=(1, 3, 5))
+    assert np.array_equal(x[..., 0], cn.bf16(mean / s0))
+
+
+def test_trilinear_prolongation_reproduces_linear_fields():
+    """Partition of unity, and exact reproduction of a linear field away from the box faces:
+    the centre of cell z lies at voxel coordinate (z - 1/2) / 2. Swapping the 3/4 and 1/4
+    weights, or taking the neighbour on the wrong side, breaks the linear case."""
+    nc = (4, 5, 6)
+    assert np.array_equal(cn.prolong_trilinear(np.full(nc, 3.5)), np.full(tuple(2 * m for m in nc), 3.5))
+    Z, Y, X = np.meshgrid(*[np.arange(m, dtype=float) for m in nc], indexing="ij")
+    out = 1.0 * Z + 2.0 * Y + 3.0 * X
+    fine = cn.prolong_trilinear(out)
+    z, y, x = np.meshgrid(*[(np.arange(2 * m) - 0.5) / 2 for m in nc], indexing="ij")
+    exp = z + 2.0 * y + 3.0 * x
+    inner = tuple(slice(1, 2 * m - 1) for m in nc)
+    assert np.allclose(fine[inner], exp[inner], rtol=0, atol=1e-12)
+    # at the faces the nearest voxel repeats: the first cell is 3/4 + 1/4 of voxel 0 along z
+    assert np.allclose(fine[0, 1:-1, 1:-1] - exp[0, 1:-1, 1:-1], 0.25, atol=1e-12)
+    base = np.ones(tuple(2 * m for m in nc))
+    assert np.allclose(cn.unpack_coarse(base, out, 2.0), base + 2.0 * fine, rtol=0, atol=0)
+
+
+def test_coarse_loss_is_the_loss_against_block_means(rng):
+    """With a block-constant step the half-resolution loss is the full-resolution loss of the
+    voxel fields; its gradient is the analytic derivative (central differences, float64)."""
+    shape = (3, 4, 5)
+    step = rng.integers(-64, 64, size=shape) / 64.0
+    base = rng.integers(-64, 64, size=shape) / 64.0
+    out = rng.normal(size=shape)
+    lc, gc = cn.loss_and_dout_coarse(out, _expand(base), _expand(base + step), 0.5, exact=True)
+    lf, gf = cn.loss_and_dout(out, base, base + step, 0.5, exact=True)
+    assert np.isclose(lc, lf, rtol=1e-14) and np.allclose(gc, gf, rtol=1e-14)
+    xb = rng.normal(size=(6, 8, 10))
+    xt = rng.normal(size=(6, 8, 10))
+    _, g = cn.loss_and_dout_coarse(out, xb, xt, 0.7, exact=True)
+    h = 1e-6
+    for idx in [(0, 0, 0), (2, 3, 4), (1, 2, 0)]:
+        op, om = out.copy(), out.copy()
+        op[idx] += h
+        om[idx] -= h
+        fd = (cn.loss_and_dout_coarse(op, xb, xt, 0.7, exact=True)[0]
+              - cn.loss_and_dout_coarse(om, xb, xt, 0.7, exact=True)[0]) / (2 * h)
+        assert np.isclose(fd, g[idx], rtol=1e-6, atol=1e-12)
