@@ -167,7 +167,7 @@ inline MgBricks mg_bricks(const DiaView& V) {
 }
 inline unsigned mg_brick_grid(const MgBricks& B) {   // four bricks (waves) per work-group, a multiple of kXcd
     long long g = (B.total + 3) / 4;
-    if (g > 4096) g = 4096;
+    if (g > 65536) g = 65536;                         // one brick per wave up to 256^3
     g = (g + kXcd - 1) / kXcd * kXcd;
     return static_cast<unsigned>(g);
 }
@@ -194,6 +194,19 @@ __device__ inline bool mg_brick_cell(const DiaView& V, const MgBricks& B, long l
     return p[0] < V.n[0] && p[1] < V.n[1] && p[2] < V.n[2];
 }
 
+// Row of the neighbour of cell p through slot s, or the cell's own row when the boundary edit
+// removed that neighbour: its slot holds 0, so the term it feeds is +0.0 and every load can
+// be issued unconditionally (all of a cell's loads in flight together).
+__device__ inline long long mg_nb_row(const DiaView& V, const int p[3], int s, long long self, int q[3]) {
+    if (!mg_nb(V, p, s, q)) {
+        q[0] = p[0];
+        q[1] = p[1];
+        q[2] = p[2];
+        return self;
+    }
+    return mg_row(V, q);
+}
+
 // rc = P^T (r - A W r) (PRE) or P^T r: one lane per fine cell, the aggregate's eight values
 // summed across lanes in a fixed order
 template <bool PRE, typename DT, typename XT, typename CT>
@@ -205,31 +218,41 @@ __global__ __launch_bounds__(kThreads) void k_mg_down(DiaView Vf, MgBricks Bf, c
     for (long long w = it.first; w < it.end; w += it.step) {
         int p[3];
         const bool in = mg_brick_cell(Vf, Bf, w, p);
-        double t = 0.;
-        if (in) {
-            const long long frow = mg_row(Vf, p);
-            t = static_cast<double>(r[frow]);
-            if (PRE) {
-                double y = static_cast<double>(Df[3 * nrf + frow]) *
-                           (static_cast<double>(Wf[frow]) * static_cast<double>(r[frow]));
-#pragma unroll
-                for (int s = 0; s < 7; s++) {
-                    int q[3];
-                    if (s == 3 || !mg_nb(Vf, p, s, q)) continue;
-                    const long long nb = mg_row(Vf, q);
-                    y += static_cast<double>(Df[s * nrf + frow]) *
-                         (static_cast<double>(Wf[nb]) * static_cast<double>(r[nb]));
-                }
-                t -= y;
-            }
+        if (!in) {                    // a valid cell for the loads; the lane's value is dropped
+            p[0] = p[0] < Vf.n[0] ? p[0] : Vf.n[0] - 1;
+            p[1] = p[1] < Vf.n[1] ? p[1] : Vf.n[1] - 1;
+            p[2] = p[2] < Vf.n[2] ? p[2] : Vf.n[2] - 1;
         }
+        const long long frow = mg_row(Vf, p);
+        double t = static_cast<double>(r[frow]);
+        if (PRE) {
+            long long nb[7];
+            double a[7], wn[7], rn[7];
+#pragma unroll
+            for (int s = 0; s < 7; s++) {
+                int q[3];
+                nb[s] = s == 3 ? frow : mg_nb_row(Vf, p, s, frow, q);
+            }
+#pragma unroll
+            for (int s = 0; s < 7; s++) {
+                a[s] = static_cast<double>(Df[s * nrf + frow]);
+                wn[s] = static_cast<double>(Wf[nb[s]]);
+                rn[s] = static_cast<double>(r[nb[s]]);
+            }
+            double y = a[3] * (wn[3] * rn[3]);
+#pragma unroll
+            for (int s = 0; s < 7; s++)
+                if (s != 3) y += a[s] * (wn[s] * rn[s]);
+            t -= y;
+        }
+        if (!in) t = 0.;
         t += __shfl_xor(t, 1);
         t += __shfl_xor(t, 16);
         t += __shfl_xor(t, 32);
         const int lane = threadIdx.x % 64;
         if (in && (lane & 0x31) == 0) {
-            const int a[3] = {p[0] / 2, p[1] / 2, p[2] / 2};
-            rc[mg_row(Vc, a)] = static_cast<CT>(t);
+            const int ag[3] = {p[0] / 2, p[1] / 2, p[2] / 2};
+            rc[mg_row(Vc, ag)] = static_cast<CT>(t);
         }
     }
 }
@@ -245,18 +268,27 @@ __global__ __launch_bounds__(kThreads) void k_mg_up(DiaView Vf, MgBricks Bf, con
         int p[3];
         if (!mg_brick_cell(Vf, Bf, w, p)) continue;
         const long long row = mg_row(Vf, p);
-        const double wr = static_cast<double>(Wf[row]), rr = static_cast<double>(r[row]);
-        const double zi = (PRE ? wr * rr : 0.) + static_cast<double>(zc[mg_agg(Vc, p)]);
-        double y = static_cast<double>(Df[3 * nrf + row]) * zi;
+        long long nb[7], ag[7];
 #pragma unroll
         for (int s = 0; s < 7; s++) {
             int q[3];
-            if (s == 3 || !mg_nb(Vf, p, s, q)) continue;
-            const long long nb = mg_row(Vf, q);
-            const double zq = (PRE ? static_cast<double>(Wf[nb]) * static_cast<double>(r[nb]) : 0.) +
-                              static_cast<double>(zc[mg_agg(Vc, q)]);
-            y += static_cast<double>(Df[s * nrf + row]) * zq;
+            nb[s] = s == 3 ? row : mg_nb_row(Vf, p, s, row, q);
+            ag[s] = s == 3 ? mg_agg(Vc, p) : mg_agg(Vc, q);
         }
+        double a[7], wn[7], rn[7], zn[7];
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+            a[s] = static_cast<double>(Df[s * nrf + row]);
+            wn[s] = PRE ? static_cast<double>(Wf[nb[s]]) : 0.;
+            rn[s] = PRE ? static_cast<double>(r[nb[s]]) : 0.;
+            zn[s] = static_cast<double>(zc[ag[s]]);
+        }
+        const double wr = static_cast<double>(Wf[row]), rr = static_cast<double>(r[row]);
+        const double zi = (PRE ? wn[3] * rn[3] : 0.) + zn[3];
+        double y = a[3] * zi;
+#pragma unroll
+        for (int s = 0; s < 7; s++)
+            if (s != 3) y += a[s] * ((PRE ? wn[s] * rn[s] : 0.) + zn[s]);
         out[row] = static_cast<XT>(zi + wr * (rr - y));
     }
 }
