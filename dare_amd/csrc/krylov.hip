@@ -762,11 +762,8 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     float* work = static_cast<float*>(W.buf);
     float* r32 = work + 7 * n;
     float* d32 = r32 + n;
-    double bnorm2 = 0.;
-    rc = dot_dev(ctx, n, b, b, &bnorm2);
-    if (rc) return rc;
-    const double bnorm = std::sqrt(bnorm2);
-    const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
+    // ||b|| comes out of the first residual sweep (EpiResidB), which reads b anyway
+    double bnorm = 0., tol_abs = 0.;
     const SolveOp<double> op = op64(A, true);
     if (!op.dia) {
         rc = csr_ensure_values(A, ctx->stream);
@@ -781,12 +778,23 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     const int inner_max = p->inner_max_it > 0 ? p->inner_max_it : p->max_it;
     const double inner_tol = p->inner_tol > 0. ? p->inner_tol : 1e-4;
     while (true) {
-        if (n >= kStreamRows) op.launch(ctx->stream, x, EpiResid<true>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);
-        else op.launch(ctx->stream, x, EpiResid<false>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);   // r = b - A x
+        const bool first = refine == 0;
+        if (first && n >= kStreamRows)
+            op.launch(ctx->stream, x, EpiResidB<true>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);
+        else if (first)
+            op.launch(ctx->stream, x, EpiResidB<false>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);
+        else if (n >= kStreamRows)
+            op.launch(ctx->stream, x, EpiResid<true>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);
+        else
+            op.launch(ctx->stream, x, EpiResid<false>{b, r32}, ctx->ws, W.st, TAIL_STORE, 0);   // r = b - A x
         DARE_LAUNCH_CHECK();
         KState h;
         rc = read_state(ctx, W.st, &h);
         if (rc) return rc;
+        if (first) {
+            bnorm = std::sqrt(h.dots[1]);
+            tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
+        }
         rn = std::sqrt(h.dots[0]);
         if (p->history && hl < p->history_cap) p->history[hl++] = rn;
         if (rn <= tol_abs) {

@@ -330,6 +330,21 @@ struct EpiResid {       // refinement: r = b - A x in FP64, r32 = float(r), <r,r
 };
 
 template <bool NT = false>
+struct EpiResidB {      // the first refinement residual: as EpiResid, and <b,b> from the same read of b
+    static constexpr int NDOT = 2;
+    static constexpr bool kNT = NT;
+    const double* b;
+    float* r32;
+    __device__ double load(long long i) const { return b[i]; }
+    __device__ void operator()(long long i, double y, double, double bi, double* acc) const {
+        const double ri = bi - y;
+        put<NT>(r32 + i, static_cast<float>(ri));
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+    }
+};
+
+template <bool NT = false>
 struct EpiResid64 {     // r = b - A x in FP64, <r,r>
     static constexpr int NDOT = 1;
     static constexpr bool kNT = NT;
