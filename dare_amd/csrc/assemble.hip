@@ -93,7 +93,10 @@ constexpr int kPosOfSlot[7] = {1, 3, 5, 0, 6, 4, 2};
 
 // DIM = 0 takes the dimension from G at run time; DIM = 3 (the diagonal-view fill) lets every
 // per-axis loop unroll, so the per-position arrays live in registers.
-template <int MODE, bool SORT, int DIM = 0>
+// FEAT: the terms the instantiation may meet (kFeat* bits); a cleared bit drops that term's
+// code (the host dispatches the diagonal-view fill of operators without them to FEAT = 0).
+enum { kFeatConv = 1, kFeatDiffCell = 2, kFeatSource = 4, kFeatCross = 8, kFeatFixed = 16, kFeatAll = 31 };
+template <int MODE, bool SORT, int DIM = 0, int FEAT = kFeatAll>
 __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, long long node,
                            int* cols, double* vals, double* rhs_out, double* x0_out,
                            double* slotv = nullptr, const double* slot_in = nullptr) {
@@ -103,7 +106,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
     double rhs = 0.;
     bool removed[7] = {false, false, false, false, false, false, false};
     double cross = 0.;
-    const bool has_cross = T.cross != 0;
+    const bool has_cross = (FEAT & kFeatCross) && T.cross != 0;
     long long lp = 0;
     for (int d = 0; d < dim; d++) lp += G.sl[d] * (q[d] + G.g);
     const int N = G.N;
@@ -111,7 +114,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
     // a cell on a DARE_BC_FIXED face (the first axis that has one, low face first) is an
     // identity row: every neighbour and the cross-component entry removed
     int fixed_face = -1;
-    for (int d = 0; d < dim && fixed_face < 0; d++) {
+    for (int d = 0; (FEAT & kFeatFixed) && d < dim && fixed_face < 0; d++) {
         if (q[d] == 0 && T.bc_type[2 * d] == DARE_BC_FIXED) fixed_face = 2 * d;
         else if (q[d] == G.n[d] - 1 && T.bc_type[2 * d + 1] == DARE_BC_FIXED) fixed_face = 2 * d + 1;
     }
@@ -140,7 +143,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
             for (int d = 0; d < dim; d++) {
                 double Df[2];
                 for (int h = 0; h < 2; h++) {
-                    if (T.diff_cell) {
+                    if ((FEAT & kFeatDiffCell) && T.diff_cell) {
                         const long long lq = lp + (h ? G.sl[d] : -G.sl[d]);
                         const double Dp = T.diff_cell[lp * N + c], Dq = T.diff_cell[lq * N + c];
                         const double Dlo = h ? Dp : Dq, Dhi = h ? Dq : Dp;
@@ -162,7 +165,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
             for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += o[pos] * -1.;
             rhs += 0. * -1.;
         }
-        if (T.conv_limiter >= 0) {                         // mb += div(tvd * phi)
+        if ((FEAT & kFeatConv) && T.conv_limiter >= 0) {   // mb += div(tvd * phi)
             double o[7] = {0., 0., 0., 0., 0., 0., 0.};
             double orhs = 0.;
             for (int d = 0; d < dim; d++) {
@@ -207,7 +210,7 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
             for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += o[pos];
             rhs += orhs;
         }
-        if (T.source) rhs += G.V * T.source[lp * N + c];
+        if ((FEAT & kFeatSource) && T.source) rhs += G.V * T.source[lp * N + c];
         if (has_cross) {
             const double* R = T.react + node * 4;
             nbv[0] -= G.V * R[c * 2 + c];
@@ -421,7 +424,7 @@ __global__ __launch_bounds__(256) void k_fill_values(Geo G, Terms T, long long n
 // Diagonal-view fill (3-D): one lane per cell, lanes along k, no column sort. Writes the slot
 // values (coalesced, slot-major), rhs and x0. The CSR values are materialized from the slots
 // only when a CSR consumer asks for them (k_materialize_csr, same sort and merge).
-template <bool NT, bool F32>
+template <bool NT, bool F32, int FEAT>
 __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __restrict__ dia, float* __restrict__ dia32,
                                                   int slots, double* __restrict__ rhs, double* __restrict__ x0) {
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -435,10 +438,18 @@ __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __rest
         const long long row = node * G.N + c;
         double slotv[kDiaSlots];
         double r = 0., x = 0.;
-        row_entries<kRowValues, false, 3>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
-        for (int sl = 0; sl < slots; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
-        if constexpr (F32)
-            for (int sl = 0; sl < slots; sl++) put<NT>(dia32 + sl * nrows + row, static_cast<float>(slotv[sl]));
+        row_entries<kRowValues, false, 3, FEAT>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
+        if constexpr ((FEAT & kFeatCross) == 0) {      // seven slots
+#pragma unroll
+            for (int sl = 0; sl < 7; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
+            if constexpr (F32)
+#pragma unroll
+                for (int sl = 0; sl < 7; sl++) put<NT>(dia32 + sl * nrows + row, static_cast<float>(slotv[sl]));
+        } else {
+            for (int sl = 0; sl < slots; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
+            if constexpr (F32)
+                for (int sl = 0; sl < slots; sl++) put<NT>(dia32 + sl * nrows + row, static_cast<float>(slotv[sl]));
+        }
         put<NT>(rhs + row, r);
         if (x0) put<NT>(x0 + row, x);
     }
@@ -715,18 +726,24 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         // the FP32 copy rides along once a mixed solve has used this matrix
         const bool f32 = A->dia32_mode && A->dia32 && A->cap_dia32 >= static_cast<int64_t>(slots) * nrows;
         const bool nt = A->nrows >= kStreamRows;
-        if (nt && f32)
-            hipLaunchKernelGGL((k_fill_dia<true, true>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->dia32, slots,
-                               rhs, x0);
-        else if (nt)
-            hipLaunchKernelGGL((k_fill_dia<true, false>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, nullptr, slots,
-                               rhs, x0);
-        else if (f32)
-            hipLaunchKernelGGL((k_fill_dia<false, true>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, A->dia32, slots,
-                               rhs, x0);
-        else
-            hipLaunchKernelGGL((k_fill_dia<false, false>), grid, dim3(256), 0, ctx->stream, G, T, A->dia, nullptr, slots,
-                               rhs, x0);
+        // operators with only DDT, constant / per-component diffusion and Dirichlet / Neumann
+        // edits take the instantiation without the other terms' code
+        bool plain = T.conv_limiter < 0 && !T.diff_cell && !T.source && !T.cross;
+        for (int f = 0; f < 6; f++) plain = plain && T.bc_type[f] != DARE_BC_FIXED;
+#define DARE_FILL(NTV, F32V)                                                                                   \
+    do {                                                                                                       \
+        if (plain)                                                                                             \
+            hipLaunchKernelGGL((k_fill_dia<NTV, F32V, 0>), grid, dim3(256), 0, ctx->stream, G, T, A->dia,      \
+                               F32V ? A->dia32 : nullptr, slots, rhs, x0);                                     \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_fill_dia<NTV, F32V, kFeatAll>), grid, dim3(256), 0, ctx->stream, G, T,       \
+                               A->dia, F32V ? A->dia32 : nullptr, slots, rhs, x0);                             \
+    } while (0)
+        if (nt && f32) DARE_FILL(true, true);
+        else if (nt) DARE_FILL(true, false);
+        else if (f32) DARE_FILL(false, true);
+        else DARE_FILL(false, false);
+#undef DARE_FILL
         DARE_LAUNCH_CHECK();
         A->dia32_fresh = f32 ? 1 : 0;
         A->vals_stale = 1;
