@@ -819,6 +819,33 @@ __device__ inline void dia_column_quad_sym(const DiaView& V, const float* __rest
     const float* DS = D + nr;
     const float* DB = D + 2 * nr;
     const float* DC = D + 3 * nr;
+    // everything a plane reads, loaded one plane ahead (the next plane's loads are in flight
+    // while this plane's sums are formed)
+    struct Plane {
+        f4 xn, xs, xq, cW, cS, cB, cC, cN, cE;
+        float xb, xt, cT3;
+        double pre[4];
+        bool hasW, hasE;
+    };
+    auto load = [&](int i, long long node, Plane& P) {
+        P.hasW = !(i == 0 && V.rm_lo[0]);
+        P.hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
+        const long long nnext = !P.hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
+        P.xn = ld4(x + nnext);
+        P.xs = ld4(x + node + dS);
+        P.xq = ld4(x + node + dN);
+        P.xb = x[node + dB];
+        P.xt = x[node + 3 + dT];
+        P.cW = ld4(DW + node);
+        P.cS = ld4(DS + node);
+        P.cB = ld4(DB + node);
+        P.cC = ld4nt(DC + node);
+        P.cN = ld4(DS + node + dN);          // S of the north neighbour (unused without it)
+        P.cE = ld4(DW + nnext);              // W of the east neighbour
+        P.cT3 = DB[node + 3 + dT];           // B of the top neighbour of cell k + 3
+#pragma unroll
+        for (int m = 0; m < 4; m++) P.pre[m] = epi.load(node + m);
+    };
     long long node = (static_cast<long long>(i0) * n1 + j) * n2 + k;
     f4 xp, xc;
     {
@@ -827,43 +854,40 @@ __device__ inline void dia_column_quad_sym(const DiaView& V, const float* __rest
         xc = ld4(x + node);
         xp = ld4(x + nprev);
     }
+    Plane cur;
+    load(i0, node, cur);
     for (int i = i0; i < i1; i++) {
-        const bool hasW = !(i == 0 && V.rm_lo[0]);
-        const bool hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
-        const long long nnext = !hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
-        const f4 xn = ld4(x + nnext), xs = ld4(x + node + dS), xq = ld4(x + node + dN);
-        const float xb = x[node + dB], xt = x[node + 3 + dT];
-        const f4 cW = ld4(DW + node), cS = ld4(DS + node), cB = ld4(DB + node), cC = ld4nt(DC + node);
-        const f4 cN = ld4(DS + node + dN);          // S of the north neighbour (unused without it)
-        const f4 cE = ld4(DW + nnext);              // W of the east neighbour
-        const float cT3 = DB[node + 3 + dT];        // B of the top neighbour of cell k + 3
-        const double p0 = epi.load(node), p1 = epi.load(node + 1), p2 = epi.load(node + 2), p3 = epi.load(node + 3);
+        Plane nxt;
+        const int inext = i + 1 < i1 ? i + 1 : i;            // the last plane reloads itself
+        load(inext, node + (i + 1 < i1 ? sI : 0), nxt);
         const float xcv[4] = {xc.x, xc.y, xc.z, xc.w}, xpv[4] = {xp.x, xp.y, xp.z, xp.w};
-        const float xsv[4] = {xs.x, xs.y, xs.z, xs.w}, xqv[4] = {xq.x, xq.y, xq.z, xq.w};
-        const float xnv[4] = {xn.x, xn.y, xn.z, xn.w};
-        const float wv[4] = {cW.x, cW.y, cW.z, cW.w}, sv[4] = {cS.x, cS.y, cS.z, cS.w};
-        const float bv[4] = {cB.x, cB.y, cB.z, cB.w}, cv[4] = {cC.x, cC.y, cC.z, cC.w};
-        const float tv[4] = {cB.y, cB.z, cB.w, cT3};
-        const float nv[4] = {cN.x, cN.y, cN.z, cN.w}, ev[4] = {cE.x, cE.y, cE.z, cE.w};
+        const float xsv[4] = {cur.xs.x, cur.xs.y, cur.xs.z, cur.xs.w};
+        const float xqv[4] = {cur.xq.x, cur.xq.y, cur.xq.z, cur.xq.w};
+        const float xnv[4] = {cur.xn.x, cur.xn.y, cur.xn.z, cur.xn.w};
+        const float wv[4] = {cur.cW.x, cur.cW.y, cur.cW.z, cur.cW.w}, sv[4] = {cur.cS.x, cur.cS.y, cur.cS.z, cur.cS.w};
+        const float bv[4] = {cur.cB.x, cur.cB.y, cur.cB.z, cur.cB.w}, cv[4] = {cur.cC.x, cur.cC.y, cur.cC.z, cur.cC.w};
+        const float tv[4] = {cur.cB.y, cur.cB.z, cur.cB.w, cur.cT3};
+        const float nv[4] = {cur.cN.x, cur.cN.y, cur.cN.z, cur.cN.w}, ev[4] = {cur.cE.x, cur.cE.y, cur.cE.z, cur.cE.w};
         double y[4];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             float s = 0.f;
-            s += hasW ? wv[m] * xpv[m] : 0.f;
+            s += cur.hasW ? wv[m] * xpv[m] : 0.f;
             s += hasS ? sv[m] * xsv[m] : 0.f;
-            if (m == 0) s += hasB ? bv[m] * xb : 0.f;
+            if (m == 0) s += hasB ? bv[m] * cur.xb : 0.f;
             else s += bv[m] * xcv[m - 1];
             s += cv[m] * xcv[m];
-            if (m == 3) s += hasT ? tv[m] * xt : 0.f;
+            if (m == 3) s += hasT ? tv[m] * cur.xt : 0.f;
             else s += tv[m] * xcv[m + 1];
             s += hasN ? nv[m] * xqv[m] : 0.f;
-            s += hasE ? ev[m] * xnv[m] : 0.f;
+            s += cur.hasE ? ev[m] * xnv[m] : 0.f;
             y[m] = static_cast<double>(s);
         }
-        const double xcd[4] = {xcv[0], xcv[1], xcv[2], xcv[3]}, pre[4] = {p0, p1, p2, p3};
-        epi_quad(epi, node, y, xcd, pre, acc);
+        const double xcd[4] = {xcv[0], xcv[1], xcv[2], xcv[3]};
+        epi_quad(epi, node, y, xcd, cur.pre, acc);
         xp = xc;
-        xc = xn;
+        xc = cur.xn;
+        cur = nxt;
         node += sI;
     }
 }
