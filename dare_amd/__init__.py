@@ -18,7 +18,7 @@ CDS, UPWIND, VANALBADA, MINMOD, VANLEER = 0, 1, 2, 3, 4
 LIMITERS = {"CDS": CDS, "UPWIND": UPWIND, "VANALBADA": VANALBADA, "MINMOD": MINMOD, "VANLEER": VANLEER}
 GHOST_LEAVE, GHOST_PERIODIC, GHOST_DIRICHLET, GHOST_NEUMANN, GHOST_CONSTANT = 0, 1, 2, 3, 4
 BC_NONE, BC_DIRICHLET, BC_NEUMANN, BC_FIXED = 0, 1, 2, 3
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG, PRECOND_JACOBI_LEFT = 0, 1, 2, 3
 
 DARE_OK = 0
 DARE_ERR_BREAKDOWN = -5

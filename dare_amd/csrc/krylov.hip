@@ -320,6 +320,23 @@ __global__ __launch_bounds__(kThreads) void k_csr_colscale(long long nnz, const 
         out[j] = static_cast<OT>(val[j] * dinv[col[j]]);
 }
 
+// left Jacobi: diag(A)^-1 A, each row scaled by its own 1 / diag (slot-major view, slots slots)
+__global__ __launch_bounds__(kThreads) void k_dia_rowscale(long long nr, int slots, const double* __restrict__ D,
+                                                           const double* __restrict__ dinv, double* __restrict__ out) {
+    for (long long r = blockIdx.x * (long long)kThreads + threadIdx.x; r < nr; r += (long long)gridDim.x * kThreads) {
+        const double s = dinv[r];
+        for (int sl = 0; sl < slots; sl++) out[sl * nr + r] = D[sl * nr + r] * s;
+    }
+}
+__global__ __launch_bounds__(kThreads) void k_csr_rowscale(long long n, const int32_t* __restrict__ rp,
+                                                           const double* __restrict__ val,
+                                                           const double* __restrict__ dinv, double* __restrict__ out) {
+    for (long long r = blockIdx.x * (long long)kThreads + threadIdx.x; r < n; r += (long long)gridDim.x * kThreads) {
+        const double s = dinv[r];
+        for (int j = rp[r]; j < rp[r + 1]; j++) out[j] = val[j] * s;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_scale_by(long long n, double* __restrict__ x, const double* __restrict__ s,
                                                        int invert) {
     for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
@@ -553,10 +570,13 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     DARE_CHECK_ARG(ctx && A && b && x && p && res, "NULL argument");
     DARE_CHECK_ARG(A->nrows > 0 && A->row_ptr && A->col && A->val, "empty matrix");
     const long long n = A->nrows;
-    DARE_CHECK_ARG(p->precond >= 0 && p->precond <= 2, "precond must be 0 (none), 1 (Jacobi) or 2 (multigrid)");
+    DARE_CHECK_ARG(p->precond >= 0 && p->precond <= 3,
+                   "precond must be 0 (none), 1 (Jacobi), 2 (multigrid) or 3 (Jacobi on the left)");
     const bool use_mg = p->precond == DARE_PRECOND_MG;
+    const bool left = p->precond == DARE_PRECOND_JACOBI_LEFT;
     Workspace W;
-    int rc = ws_reserve(ctx, sizeof(double) * (use_mg ? 8 : 7) * n, &W);   // + the preconditioned iterate
+    // + the preconditioned iterate (multigrid) or the scaled right-hand side (left Jacobi)
+    int rc = ws_reserve(ctx, sizeof(double) * (use_mg || left ? 8 : 7) * n, &W);
     if (rc) return rc;
     double bnorm2 = 0.;
     rc = dot_dev(ctx, n, b, b, &bnorm2);
@@ -595,10 +615,38 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
         hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, x, A->dinv, 0);  // y0 = M x0
         DARE_LAUNCH_CHECK();
     }
+    // left Jacobi (BiCGStab2.inl:276-280 and the M^-1 A applications of steps 6, 9, 13): the
+    // method runs on diag(A)^-1 A x = diag(A)^-1 b, so its residual, and the stopping test on
+    // it, are the preconditioned ones, as in the reference; the RBNORM normaliser stays ||b||
+    const double* bs = b;
+    if (left) {
+        dare_csr* Am = const_cast<dare_csr*>(A);
+        rc = jacobi_setup(ctx, Am);
+        if (rc) return rc;
+        if (op.dia) {
+            const int slots = A->dia_view.cross ? 8 : 7;
+            rc = grow_buffer(&Am->dia_pre, &Am->cap_dia_pre, static_cast<int64_t>(slots) * n, "preconditioned view");
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_dia_rowscale, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, slots, A->dia,
+                               A->dinv, Am->dia_pre);
+            op.d = make_dia<double>(A->dia_view, Am->dia_pre, true);
+        } else {
+            rc = grow_buffer(&Am->dia_pre, &Am->cap_dia_pre, A->nnz, "preconditioned values");
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_csr_rowscale, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, A->row_ptr,
+                               A->val, A->dinv, Am->dia_pre);
+            op.csr.val = Am->dia_pre;
+        }
+        double* bl = static_cast<double*>(W.buf) + 7 * n;
+        DARE_HIP_TRY(hipMemcpyAsync(bl, b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, n, bl, A->dinv, 1);
+        DARE_LAUNCH_CHECK();
+        bs = bl;
+    }
     int iters = 0, status = 0, hl = 0;
     double rn = 0.;
     bool conv = false;
-    rc = bicgstab2_run<double, double>(ctx, op, b, x, tol_abs, p->max_it, p->history, p->history_cap, &hl, &iters,
+    rc = bicgstab2_run<double, double>(ctx, op, bs, x, tol_abs, p->max_it, p->history, p->history_cap, &hl, &iters,
                                        &rn, &status, &conv, static_cast<double*>(W.buf), 2, false, W.st, ctx->ws, mg,
                                        static_cast<double*>(W.buf) + 7 * n);
     if (rc) return rc;

@@ -58,6 +58,10 @@ extern "C" {
 /* aggregation multigrid V-cycle on the diagonal view (one-component 3-D operators): the
  * stand-in for MueLu "AMG" (MatrixSystem/TrilinosSolver.inl:180-246) */
 #define DARE_PRECOND_MG 2
+/* Jacobi in BiCGStab2's left slot (BiCGStab2.inl:276-280 and the M^-1 A applications of
+ * steps 6, 9, 13): the method runs on diag(A)^-1 A x = diag(A)^-1 b, its residual and stopping
+ * test are the preconditioned ones; FP64 solve only */
+#define DARE_PRECOND_JACOBI_LEFT 3
 
 typedef struct dare_ctx dare_ctx;
 typedef struct dare_csr dare_csr;
@@ -111,7 +115,8 @@ typedef struct {
                               * the right); the diagonal-view operator is column-scaled, a
                               * generic CSR matrix gets a scaled copy of its values; 2:
                               * multigrid V-cycle M^-1 applied before every operator
-                              * application (DARE_PRECOND_MG) */
+                              * application (DARE_PRECOND_MG); 3: Jacobi on the left
+                              * (DARE_PRECOND_JACOBI_LEFT, dare_bicgstab2_solve only) */
 } dare_solver_params;
 
 typedef struct {

@@ -77,7 +77,7 @@ def test_invalid_arguments_are_refused_and_the_context_survives(ctx):
 
     # solver parameters
     da.fill_values(ctx, A, g, t, phi, rhs, x)
-    _raises(lambda: da.bicgstab2(ctx, A, rhs, x, precond=3), "precond must be 0 (none), 1 (Jacobi) or 2 (multigrid)")
+    _raises(lambda: da.bicgstab2(ctx, A, rhs, x, precond=4), "precond must be 0 (none), 1 (Jacobi), 2 (multigrid) or 3")
     _raises(lambda: da.refine_mixed(ctx, A, rhs, x, precond=-1), "precond must be 0 (none), 1 (Jacobi) or 2")
 
     # batch configuration and systems

@@ -1,7 +1,9 @@
-"""Jacobi right preconditioning (precond = 1): the solver runs on A M^-1, M = diag(A), with the
+"""Preconditioners. Jacobi right preconditioning (precond = 1): the solver runs on A M^-1, M = diag(A), with the
 column-scaled diagonal view (or scaled CSR values for a generic matrix), and returns x = M^-1 y. The solution is the same x* (checked
 against the oracle's unpreconditioned solve at a tight tolerance), the stopping test is on the
-true residual, and on a variable-diffusivity operator it needs fewer passes."""
+true residual, and on a variable-diffusivity operator it needs fewer passes. Left Jacobi
+(precond = 3) runs on diag(A)^-1 A x = diag(A)^-1 b; the multigrid V-cycle (precond = 2) in the
+right slot."""
 import numpy as np
 import pytest
 
@@ -82,6 +84,55 @@ def test_jacobi_on_generic_csr_matches_oracle(ctx, rng, name):
     r = da.refine_mixed(ctx, A, dev(orhs), xm, tol=1e-10, tol_kind=1, inner_tol=1e-4, precond=1)
     assert r["converged"] == 1
     assert np.linalg.norm(orhs - orc.spmv(rp, col, val, host(xm))) <= 1e-10 * np.linalg.norm(orhs)
+    A.close()
+
+
+def _left_scaled(rp, col, val, b):
+    """diag(A)^-1 A and diag(A)^-1 b, row by row (the system left Jacobi iterates on)."""
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    diag = np.zeros(n)
+    on = col == rows
+    diag[rows[on]] = val[on]
+    dinv = 1.0 / diag
+    return val * dinv[rows], b * dinv
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+def test_left_jacobi_is_bicgstab2_on_the_row_scaled_system(ctx, rng, name):
+    """precond = 3 (Jacobi in the left slot, BiCGStab2.inl:276-280): the oracle's BiCGStab2 on
+    (diag(A)^-1 A, diag(A)^-1 b) takes the same passes (+-1) to the same solution, and that
+    solution solves A x = b."""
+    go, A, rhs, x0, (rp, col, val, orhs, ox0), keep = device_assembly(ctx, rng, name)
+    vs, bs = _left_scaled(rp, col, val, orhs)
+    tol = 1e-12 * np.linalg.norm(bs)
+    xo, ok, it, res, _ = orc.bicgstab2(rp, col, vs, bs, ox0, tol, 300)
+    assert ok
+    r = da.bicgstab2(ctx, A, rhs, x0, tol=tol, tol_kind=0, max_it=300, precond=da.PRECOND_JACOBI_LEFT)
+    assert r["converged"] == 1 and abs(r["iterations"] - it) <= 1, (r["iterations"], it)
+    assert r["residual"] <= tol
+    x = host(x0)
+    assert np.max(np.abs(x - xo)) <= 1e-10 * np.max(np.abs(xo))
+    assert np.linalg.norm(orhs - orc.spmv(rp, col, val, x)) <= 1e-9 * np.linalg.norm(orhs)
+    A.close()
+
+
+def test_left_jacobi_on_generic_csr_and_refusals(ctx, rng):
+    """Left Jacobi on a plain CSR upload (rows scaled in a copy of the values); the mixed
+    solve and the allocation-free entry point refuse it."""
+    go, Ad, rhs, x0, (rp, col, val, orhs, ox0), keep = device_assembly(ctx, rng, "wide_dirichlet")
+    Ad.close()
+    A = da.Csr(ctx)
+    da.upload_host_csr(ctx, A, rp, col, val)
+    vs, bs = _left_scaled(rp, col, val, orhs)
+    tol = 1e-12 * np.linalg.norm(bs)
+    xo, ok, it, res, _ = orc.bicgstab2(rp, col, vs, bs, ox0, tol, 300)
+    x = dev(ox0)
+    r = da.bicgstab2(ctx, A, dev(orhs), x, tol=tol, tol_kind=0, max_it=300, precond=da.PRECOND_JACOBI_LEFT)
+    assert r["converged"] == 1 and abs(r["iterations"] - it) <= 1
+    assert np.max(np.abs(host(x) - xo)) <= 1e-10 * np.max(np.abs(xo))
+    with pytest.raises(da.DareError, match="precond must be"):
+        da.refine_mixed(ctx, A, dev(orhs), dev(ox0), tol=1e-10, tol_kind=1, precond=da.PRECOND_JACOBI_LEFT)
     A.close()
 
 
