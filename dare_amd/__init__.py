@@ -204,6 +204,8 @@ def lib():
         "dare_spmv_dot": (i32, [vp, vp, vp, vp, vp, ctypes.POINTER(dbl)]),
         "dare_axpby": (i32, [vp, i64, dbl, vp, dbl, vp]),
         "dare_residual": (i32, [vp, vp, vp, vp, vp, ctypes.POINTER(dbl)]),
+        "dare_csr_symmetric": (i32, [vp, vp, ctypes.POINTER(i32)]),
+        "dare_csr_set_symmetric_traversal": (i32, [vp, i32]),
         "dare_bicgstab2_solve": (i32, [vp, vp, vp, vp, P, R]),
         "dare_bicgstab2_solve_noalloc": (i32, [vp, vp, vp, vp, P, R]),
         "dare_refine_solve_mixed": (i32, [vp, vp, vp, vp, P, R]),
@@ -397,6 +399,19 @@ def residual(ctx, csr, b, x, r, norm=False):
     check(lib().dare_residual(ctx.h, csr.h, _ptr(b), _ptr(x), _ptr(r), ctypes.byref(v) if norm else None),
           "dare_residual")
     return v.value if norm else None
+
+
+def csr_symmetric(ctx, csr):
+    """True when the FP64 operator applications and solves use the symmetric (four-slot)
+    traversal of the matrix's diagonal view: the view is exactly symmetric."""
+    v = ctypes.c_int32()
+    check(lib().dare_csr_symmetric(ctx.h, csr.h, ctypes.byref(v)), "dare_csr_symmetric")
+    return bool(v.value)
+
+
+def set_symmetric_traversal(csr, enable):
+    """enable=False: every traversal of the matrix reads its seven slots (same results)."""
+    check(lib().dare_csr_set_symmetric_traversal(csr.h, 1 if enable else 0), "dare_csr_set_symmetric_traversal")
 
 
 def copy_to_field(ctx, grid, x, phi):

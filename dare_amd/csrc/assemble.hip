@@ -606,6 +606,7 @@ int dare_csr_destroy(dare_csr* A) {
     if (A->dia) hipFree(A->dia);
     if (A->dia32) hipFree(A->dia32);
     if (A->asym32) hipFree(A->asym32);
+    if (A->asym64) hipFree(A->asym64);
     if (A->dinv) hipFree(A->dinv);
     if (A->dia_pre) hipFree(A->dia_pre);
     mg_free(A->mg);
@@ -693,6 +694,12 @@ int dare_csr_build_structure(dare_ctx* ctx, dare_csr* A, const dare_grid* g, con
             }
             A->cap_dia = need;
         }
+        if (!A->asym64 && hipMalloc(&A->asym64, sizeof(int)) != hipSuccess) {
+            hipGetLastError();
+            set_error("symmetry flag allocation failed");
+            return DARE_ERR_OOM;
+        }
+        A->sym64 = -1;
         A->has_dia = 1;
     }
     return DARE_OK;
@@ -747,12 +754,21 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         DARE_LAUNCH_CHECK();
         A->dia32_fresh = f32 ? 1 : 0;
         A->vals_stale = 1;
-        bool sym = G.N == 1 && t->conv_limiter < 0 && t->diff_cell == nullptr && t->react == nullptr;
+        // Symmetric by construction: a face coefficient is A_d * (dnr_d * D_f) in both rows it
+        // couples (with the signs exchanged twice), and the per-cell harmonic mean takes its two
+        // diffusivities in face order (low cell first) from either side. Not claimed for fixed
+        // faces (identity rows), for per-cell diffusivity across a periodic wrap (the wrapped
+        // face reads the caller's ghost values, which need not be the periodic images) or for a
+        // single-cell axis (a cell coupled to itself through both faces; the structure refuses
+        // it unless periodic).
+        bool sym = G.N == 1 && t->conv_limiter < 0 && t->react == nullptr;
         for (int f = 0; f < 6; f++) sym = sym && t->bc_type[f] != DARE_BC_FIXED;
-        // a single-cell axis couples a cell to itself through both faces; the structure refuses
-        // it unless periodic, and the symmetric traversal is not claimed for it either way
-        for (int d = 0; d < G.dim; d++) sym = sym && G.n[d] > 1;
+        for (int d = 0; d < G.dim; d++) sym = sym && G.n[d] > 1 && !(t->diff_cell && G.per[d]);
         A->sym_fill = sym ? 1 : 0;
+        // the cases just excluded may still give an exactly symmetric view: the solvers check
+        // those on the device
+        A->sym_cand = (G.N == 1 && t->conv_limiter < 0 && t->react == nullptr) ? 1 : 0;
+        A->sym64 = -1;
         return DARE_OK;
     }
     hipLaunchKernelGGL(k_fill_values, dim3(ceil_div(nrows, 256)), dim3(256), 0, ctx->stream, G, T, nrows,
@@ -786,6 +802,8 @@ int dare_csr_upload_host(dare_ctx* ctx, dare_csr* A, int64_t nrows, int64_t nnz,
     A->vals_stale = 0;
     A->dia32_fresh = 0;
     A->sym_fill = 0;
+    A->sym_cand = 0;
+    A->sym64 = -1;
     return DARE_OK;
 }
 

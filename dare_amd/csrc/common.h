@@ -114,9 +114,19 @@ struct dare_csr {
     // W, S, B of its neighbour), set by the check each mixed solve runs on a fresh copy
     int* asym32 = nullptr;
     // 1: the last device fill's operator is exactly symmetric by construction (one component,
-    // constant diffusivity, no convection, reaction or fixed faces: every face coefficient is
-    // the same expression for both rows it couples)
+    // no convection, reaction or fixed faces, per-cell diffusivity only without a periodic axis:
+    // every face coefficient is the same expression for both rows it couples)
     int sym_fill = 0;
+    // 1: the last device fill may be exactly symmetric (one component, no convection or
+    // reaction): the FP64 and FP32 views are then checked on the device unless sym_fill holds
+    int sym_cand = 0;
+    // the FP64 view's symmetry for the solvers: -1 not yet known since the last fill, 0 not
+    // exactly symmetric, 1 symmetric (by construction, or by the device check whose result
+    // lands in the device int asym64, allocated with the view)
+    int sym64 = -1;
+    int* asym64 = nullptr;
+    // 0: every traversal reads the seven slots (dare_csr_set_symmetric_traversal)
+    int sym_traversal = 1;
     // Jacobi right preconditioning: 1 / diag(A) and the column-scaled FP64 operator A diag(A)^-1
     // (diagonal-view slots, or CSR values for a matrix without a diagonal view)
     double* dinv = nullptr;

@@ -42,6 +42,10 @@ struct SolveOp {
                 if constexpr (std::is_same_v<VT, float> && std::is_same_v<XT, float>)
                     hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 4>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi,
                                        ws, st, tail, check);
+            } else if (d.pair == 2 && d.sym) {   // FP64 operators only (op64)
+                if constexpr (std::is_same_v<VT, double>)
+                    hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 2, true>), dim3(grid), dim3(kThreads), 0, sm, d, x,
+                                       epi, ws, st, tail, check);
             } else if (d.pair == 2)
                 hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 2>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws,
                                    st, tail, check);
@@ -190,7 +194,7 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
                     done_ok = true;
                     *status = 0;
                 }
-                if (mg) hipMemcpyAsync(x, t, sizeof(XT) * n, hipMemcpyDeviceToDevice, sm);
+                if (mg) DARE_HIP_TRY(hipMemcpyAsync(x, t, sizeof(XT) * n, hipMemcpyDeviceToDevice, sm));
                 *iters = k;
                 *rnorm = conv;
                 *converged = done_ok;
@@ -403,8 +407,34 @@ int mg_prepare(dare_ctx* ctx, dare_csr* A, const SolveOp<VT>& op, const VT* D0, 
     return rc;
 }
 
-static SolveOp<double> op64(const dare_csr* A, bool allow_dia) {
-    SolveOp<double> op;
+// A->sym64 for the last fill: 1 by construction (sym_fill), from the device's slot-by-slot
+// check for an operator that may be symmetric (sym_cand; one readback per fill), else 0
+static int sym64_prepare(dare_ctx* ctx, dare_csr* A) {
+    if (!A->has_dia || A->sym64 >= 0) return DARE_OK;
+    const bool one = A->dia_view.N == 1 && !A->dia_view.cross;
+    if (one && A->sym_fill) {
+        A->sym64 = 1;
+    } else if (one && A->sym_cand && A->asym64) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, ctx->stream, A->asym64, 0);
+        hipLaunchKernelGGL((k_dia_asym<double>), dim3(grid_for(A->nrows)), dim3(kThreads), 0, ctx->stream,
+                           A->dia_view, static_cast<const double*>(A->dia), A->asym64);
+        DARE_LAUNCH_CHECK();
+        DARE_HIP_TRY(hipMemcpyAsync(ctx->host_pinned, A->asym64, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        DARE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        int flag = 1;
+        std::memcpy(&flag, ctx->host_pinned, sizeof(int));
+        A->sym64 = flag == 0 ? 1 : 0;
+    } else {
+        A->sym64 = 0;
+    }
+    return DARE_OK;
+}
+
+// The FP64 operator of A: the diagonal view (its symmetric pair traversal when the view is
+// exactly symmetric) or the CSR values.
+static int op64(dare_ctx* ctx, const dare_csr* A, bool allow_dia, SolveOp<double>* out) {
+    SolveOp<double>& op = *out;
+    op = SolveOp<double>{};
     op.n = A->nrows;
     op.csr = CsrOp<double>{A->row_ptr, A->col, A->val, A->nrows};
     op.dia = allow_dia && A->has_dia;
@@ -412,7 +442,12 @@ static SolveOp<double> op64(const dare_csr* A, bool allow_dia) {
         op.d = make_dia<double>(A->dia_view, A->dia, true);
         op.dia = kXcd * op.d.per_xcd <= kReduceBlocks;   // else the CSR traversal
     }
-    return op;
+    if (op.dia && op.d.pair == 2 && A->sym_traversal) {
+        const int rc = sym64_prepare(ctx, const_cast<dare_csr*>(A));
+        if (rc) return rc;
+        op.d.sym = A->sym64 == 1;
+    }
+    return DARE_OK;
 }
 
 }  // namespace
@@ -490,9 +525,11 @@ int dare_spmv(dare_ctx* ctx, const dare_csr* A, const double* x, double* y) {
     // sorted column order, so a device-assembled matrix is applied from its diagonal view
     // (80 B/row, no CSR values to materialize) with bit-identical results; otherwise the CSR.
     const bool wrap = A->has_dia && (A->dia_view.per[0] || A->dia_view.per[1] || A->dia_view.per[2]);
-    SolveOp<double> op = op64(A, !wrap);
+    SolveOp<double> op;
+    int rc = op64(ctx, A, !wrap, &op);
+    if (rc) return rc;
     if (!op.dia) {
-        int rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
+        rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
         if (rc) return rc;
     }
     if (A->nrows >= kStreamRows) op.launch(ctx->stream, x, EpiStore<double, true>{y}, ctx->ws, nullptr, -1, 0);
@@ -510,7 +547,9 @@ int dare_residual(dare_ctx* ctx, const dare_csr* A, const double* b, const doubl
     Workspace W;
     int rc = ws_reserve(ctx, 0, &W);
     if (rc) return rc;
-    SolveOp<double> op = op64(A, true);
+    SolveOp<double> op;
+    rc = op64(ctx, A, true, &op);
+    if (rc) return rc;
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
         if (rc) return rc;
@@ -540,7 +579,9 @@ int dare_spmv_dot(dare_ctx* ctx, const dare_csr* A, const double* x, double* y, 
     if (rc) return rc;
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(64), 0, ctx->stream, W.st);
     // the operator the solver uses (diagonal view when the matrix has one)
-    SolveOp<double> op = op64(A, true);
+    SolveOp<double> op;
+    rc = op64(ctx, A, true, &op);
+    if (rc) return rc;
     if (A->nrows >= kStreamRows) op.launch(ctx->stream, x, EpiDot<double, true>{y, w}, ctx->ws, W.st, TAIL_STORE, 1);
     else op.launch(ctx->stream, x, EpiDot<double, false>{y, w}, ctx->ws, W.st, TAIL_STORE, 1);
     DARE_LAUNCH_CHECK();
@@ -550,6 +591,23 @@ int dare_spmv_dot(dare_ctx* ctx, const dare_csr* A, const double* x, double* y, 
         if (rc) return rc;
         *out_host = h.dots[0];
     }
+    return DARE_OK;
+}
+
+int dare_csr_symmetric(dare_ctx* ctx, const dare_csr* A, int32_t* out) {
+    DARE_CHECK_ARG(ctx && A && out, "NULL argument");
+    *out = 0;
+    if (A->nrows == 0) return DARE_OK;
+    SolveOp<double> op;
+    const int rc = op64(ctx, A, true, &op);
+    if (rc) return rc;
+    *out = op.dia && op.d.pair == 2 && op.d.sym ? 1 : 0;
+    return DARE_OK;
+}
+
+int dare_csr_set_symmetric_traversal(dare_csr* A, int32_t enable) {
+    DARE_CHECK_ARG(A, "NULL argument");
+    A->sym_traversal = enable != 0;
     return DARE_OK;
 }
 
@@ -583,7 +641,9 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     if (rc) return rc;
     const double bnorm = std::sqrt(bnorm2);
     const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
-    SolveOp<double> op = op64(A, true);
+    SolveOp<double> op;
+    rc = op64(ctx, A, true, &op);
+    if (rc) return rc;
     const bool pre = p->precond == 1;
     MgHierarchy* mg = nullptr;
     if (use_mg) {
@@ -693,7 +753,9 @@ int dare_bicgstab2_solve_noalloc(dare_ctx* ctx, const dare_csr* A, const double*
     if (rc) return rc;
     const double bnorm = std::sqrt(bnorm2);
     const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
-    SolveOp<double> op = op64(A, true);
+    SolveOp<double> op;
+    rc = op64(ctx, A, true, &op);
+    if (rc) return rc;
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
         if (rc) return rc;
@@ -758,7 +820,7 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         }
         op32.d = make_dia<float>(A->dia_view, A->dia32, true);
         op32.dia = kXcd * op32.d.per_xcd <= kReduceBlocks;
-        if (op32.dia && op32.d.pair == 4 && !pre) {
+        if (op32.dia && op32.d.pair == 4 && !pre && A->sym_traversal) {
             // exactly symmetric FP32 view (a symmetric operator, Jacobi column scaling off):
             // the quad traversal may then read E, N, T from the neighbours' W, S, B slots
             if (!A->asym32 && hipMalloc(&A->asym32, sizeof(int)) != hipSuccess) {
@@ -766,10 +828,13 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
                 set_error("symmetry flag allocation failed");
                 return DARE_ERR_OOM;
             }
-            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, ctx->stream, A->asym32, 0);
-            if (!A->sym_fill)   // not symmetric by construction: compare the slots
-                hipLaunchKernelGGL(k_dia32_asym, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, A->dia_view,
-                                   static_cast<const float*>(A->dia32), A->asym32);
+            // convection or reaction: never symmetric; otherwise symmetric by construction or
+            // compared slot by slot
+            const bool cand = A->sym_fill || A->sym_cand;
+            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, ctx->stream, A->asym32, cand ? 0 : 1);
+            if (cand && !A->sym_fill)
+                hipLaunchKernelGGL((k_dia_asym<float>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream,
+                                   A->dia_view, static_cast<const float*>(A->dia32), A->asym32);
             op32.d.asym = A->asym32;
         }
     } else {
@@ -812,7 +877,9 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     float* d32 = r32 + n;
     // ||b|| comes out of the first residual sweep (EpiResidB), which reads b anyway
     double bnorm = 0., tol_abs = 0.;
-    const SolveOp<double> op = op64(A, true);
+    SolveOp<double> op;
+    rc = op64(ctx, A, true, &op);
+    if (rc) return rc;
     if (!op.dia) {
         rc = csr_ensure_values(A, ctx->stream);
         if (rc) return rc;

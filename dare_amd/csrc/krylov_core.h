@@ -212,6 +212,7 @@ struct DiaOp {
     int per_xcd;        // tiles owned by each XCD (grid = kXcd * per_xcd)
     int pair;           // cells per lane along k (2: tiles of 128 (k) x 4 (j), one component, n2 even)
     const int* asym = nullptr;   // FP32 quad form: *asym == 0 selects the symmetric traversal
+    bool sym = false;            // pair form: the view is exactly symmetric (dia_column_pair_sym)
 };
 
 template <typename VT>
@@ -719,6 +720,91 @@ __device__ inline void dia_column_pair(const DiaView& V, const VT* __restrict__ 
     }
 }
 
+// dia_column_pair for an exactly symmetric view (symmetric by construction, or found so by
+// k_dia_asym): the E, N and T coefficients of a row are read as the W, S and B coefficients
+// of its east, north and top neighbours, so four slots stream from HBM instead of seven. The
+// partner values are the same numbers and each sum keeps dia_column_pair's order, so y is
+// bit-identical. The east neighbour's W is the next plane's own W: it is loaded once and kept
+// in registers. S and B are read again as partners of the rows south and below (from the
+// caches), so they carry no non-temporal hint; C and the east W do.
+template <typename VT, typename XT, class Epi>
+__device__ inline void dia_column_pair_sym(const DiaView& V, const VT* __restrict__ D, const XT* __restrict__ x,
+                                           int k, int j, int i0, int i1, const Epi& epi, double* acc) {
+    const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
+    const long long sI = static_cast<long long>(n1) * n2, sJ = n2;
+    const long long nr = V.nrows;
+    const bool hasS = !(j == 0 && V.rm_lo[1]);
+    const bool hasN = !(j == n1 - 1 && V.rm_hi[1] && !(j == 0 && V.rm_lo[1]));
+    const bool hasB = !(k == 0 && V.rm_lo[2]);
+    const bool hasT = !(k + 1 == n2 - 1 && V.rm_hi[2]);
+    const long long dS = !hasS ? 0 : (j == 0) ? (long long)(n1 - 1) * sJ : -sJ;
+    const long long dN = !hasN ? 0 : (j == n1 - 1) ? -(long long)(n1 - 1) * sJ : sJ;
+    const long long dB = !hasB ? 0 : (k == 0) ? (long long)(n2 - 1) : -1;
+    const long long dT = !hasT ? 0 : (k + 1 == n2 - 1) ? -(long long)(n2 - 1) : 1;
+    const VT* DW = D;
+    const VT* DS = D + nr;
+    const VT* DB = D + 2 * nr;
+    const VT* DC = D + 3 * nr;
+    auto ldnt = [](const VT* p) {
+        if constexpr (Epi::kNT) return static_cast<double>(ld_nt(p));
+        else return static_cast<double>(*p);
+    };
+    long long node = (static_cast<long long>(i0) * n1 + j) * n2 + k;
+    double xp0, xp1, xc0, xc1;
+    {
+        const bool need = !(i0 == 0 && V.rm_lo[0]);
+        const long long nprev = !need ? node : (i0 == 0) ? node + (long long)(n0 - 1) * sI : node - sI;
+        xc0 = static_cast<double>(x[node]);
+        xc1 = static_cast<double>(x[node + 1]);
+        xp0 = static_cast<double>(x[nprev]);
+        xp1 = static_cast<double>(x[nprev + 1]);
+    }
+    double w0 = ldnt(DW + node), w1 = ldnt(DW + node + 1);
+    for (int i = i0; i < i1; i++) {
+        const bool hasW = !(i == 0 && V.rm_lo[0]);
+        const bool hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
+        // a following plane of this tile is always node + sI (i < n0 - 1 there), so the east
+        // W loaded here is that plane's own W
+        const long long nnext = !hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
+        const double xn0 = static_cast<double>(x[nnext]), xn1 = static_cast<double>(x[nnext + 1]);
+        const double xs0 = static_cast<double>(x[node + dS]), xs1 = static_cast<double>(x[node + dS + 1]);
+        const double xq0 = static_cast<double>(x[node + dN]), xq1 = static_cast<double>(x[node + dN + 1]);
+        const double xb = static_cast<double>(x[node + dB]);
+        const double xt = static_cast<double>(x[node + 1 + dT]);
+        const double sv0 = static_cast<double>(DS[node]), sv1 = static_cast<double>(DS[node + 1]);
+        const double bv0 = static_cast<double>(DB[node]), bv1 = static_cast<double>(DB[node + 1]);
+        const double cv0 = ldnt(DC + node), cv1 = ldnt(DC + node + 1);
+        const double ev0 = ldnt(DW + nnext), ev1 = ldnt(DW + nnext + 1);        // W of the east neighbour
+        const double nv0 = static_cast<double>(DS[node + dN]), nv1 = static_cast<double>(DS[node + dN + 1]);
+        const double tv1 = static_cast<double>(DB[node + 1 + dT]);              // B of cell k+1's top neighbour
+        const double pre0 = epi.load(node), pre1 = epi.load(node + 1);
+        double s0 = 0.;
+        s0 += hasW ? w0 * xp0 : 0.;
+        s0 += hasS ? sv0 * xs0 : 0.;
+        s0 += hasB ? bv0 * xb : 0.;
+        s0 += cv0 * xc0;
+        s0 += bv1 * xc1;              // T of cell k = B of cell k+1
+        s0 += hasN ? nv0 * xq0 : 0.;
+        s0 += hasE ? ev0 * xn0 : 0.;
+        double s1 = 0.;
+        s1 += hasW ? w1 * xp1 : 0.;
+        s1 += hasS ? sv1 * xs1 : 0.;
+        s1 += bv1 * xc0;
+        s1 += cv1 * xc1;
+        s1 += hasT ? tv1 * xt : 0.;
+        s1 += hasN ? nv1 * xq1 : 0.;
+        s1 += hasE ? ev1 * xn1 : 0.;
+        epi_pair(epi, node, s0, s1, xc0, xc1, pre0, pre1, acc);
+        xp0 = xc0;
+        xp1 = xc1;
+        xc0 = xn0;
+        xc1 = xn1;
+        w0 = ev0;
+        w1 = ev1;
+        node += sI;
+    }
+}
+
 // dia_column_quad: the FP32 operator of the inner solves (one component, FP32 slots and
 // vectors), four k-adjacent rows per lane, so every slot, x and vector access is one 16-byte
 // float4 and a wave moves 1 KB per memory instruction. The products and sums are FP32 (the
@@ -789,7 +875,7 @@ __device__ inline void dia_column_quad(const DiaView& V, const float* __restrict
     }
 }
 
-// dia_column_quad for an exactly symmetric FP32 view (checked by k_dia32_asym): the E, N and T
+// dia_column_quad for an exactly symmetric FP32 view (checked by k_dia_asym): the E, N and T
 // coefficients of a row are read as the W, S and B coefficients of its east, north and top
 // neighbours, so four slots stream from HBM instead of seven. The partner values are the same
 // floats and the arithmetic is that of dia_column_quad, so the results are bit-identical. The
@@ -892,14 +978,19 @@ __device__ inline void dia_column_quad_sym(const DiaView& V, const float* __rest
     }
 }
 
-// Symmetry check of a one-component FP32 view: flag = 1 if the E, N or T coefficient of any
-// row differs (bit for bit) from the W, S or B coefficient of the neighbour it couples to, with
-// the neighbour and presence rules of dia_column_quad. flag must be 0 before the launch.
-__global__ __launch_bounds__(kThreads) void k_dia32_asym(DiaView V, const float* __restrict__ D, int* __restrict__ flag) {
+// Symmetry check of a one-component view (FP32 or FP64 slots): flag = 1 if the E, N or T
+// coefficient of any row differs (bit for bit) from the W, S or B coefficient of the neighbour
+// it couples to, with the neighbour and presence rules of dia_column. flag must be 0 before the
+// launch.
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void k_dia_asym(DiaView V, const VT* __restrict__ D, int* __restrict__ flag) {
     const long long nr = V.nrows;
     const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
     const long long sI = static_cast<long long>(n1) * n2, sJ = n2;
-    auto bits = [](float v) { return __float_as_uint(v); };
+    auto bits = [](VT v) {
+        if constexpr (std::is_same_v<VT, float>) return __float_as_uint(v);
+        else return static_cast<unsigned long long>(__double_as_longlong(v));
+    };
     bool bad = false;
     for (long long row = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x; row < nr;
          row += static_cast<long long>(gridDim.x) * kThreads) {
@@ -922,7 +1013,8 @@ __global__ __launch_bounds__(kThreads) void k_dia32_asym(DiaView V, const float*
 
 __global__ void k_set_int(int* __restrict__ p, int v) { p[0] = v; }
 
-template <int NC, typename VT, typename XT, class Epi, int PAIR = 1>
+// PAIR: cells per lane along k (1, 2, or 4 for FP32); SYM (pairs): the symmetric traversal
+template <int NC, typename VT, typename XT, class Epi, int PAIR = 1, bool SYM = false>
 __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
                                                        KState* st, int tail, int check_status) {
     double acc[Epi::NDOT];
@@ -945,6 +1037,8 @@ __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* _
                 static_assert(std::is_same_v<VT, float> && std::is_same_v<XT, float>, "quad traversal is FP32");
                 if (A.asym != nullptr && *A.asym == 0) dia_column_quad_sym<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
                 else dia_column_quad<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
+            } else if constexpr (PAIR == 2 && SYM) {
+                dia_column_pair_sym<VT, XT, Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
             } else if constexpr (PAIR == 2) {
                 dia_column_pair<VT, XT, Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
             } else {

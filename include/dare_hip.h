@@ -221,6 +221,17 @@ int dare_spmv_dot(dare_ctx* ctx, const dare_csr* csr, const double* x, double* y
 int dare_axpby(dare_ctx* ctx, int64_t n, double a, const double* x, double b, double* y);
 /* r = b - A x (the operator the solver uses) and, if norm_host != NULL, ||r|| (BiCGStab2.inl:265-266). */
 int dare_residual(dare_ctx* ctx, const dare_csr* csr, const double* b, const double* x, double* r, double* norm_host);
+/* *out = 1 when the FP64 solves, dare_residual, dare_spmv_dot (and dare_spmv without a
+ * periodic axis) apply A through the symmetric traversal of its diagonal view (E, N, T of each
+ * row read as W, S, B of its neighbours, with results bit-identical to the seven-slot one: the
+ * view is exactly symmetric, by construction of the last fill or by the device's slot-by-slot
+ * check), 0 otherwise (no diagonal view, two components, an odd k extent, convection, a failed
+ * check). No counterpart in the reference; waits for the context stream. */
+int dare_csr_symmetric(dare_ctx* ctx, const dare_csr* csr, int32_t* out);
+/* enable = 0: every FP64 and FP32 traversal of this matrix reads its seven slots (the results
+ * are the same bits either way); 1 (the default): symmetric views take the four-slot
+ * traversal. No counterpart in the reference. */
+int dare_csr_set_symmetric_traversal(dare_csr* csr, int32_t enable);
 
 /* ---- solvers ------------------------------------------------------------------------- */
 /* FP64 BiCGStab(2). Replaces TrilinosSolver::Solve(BumbleBee, "BICGSTAB2", ...) ->

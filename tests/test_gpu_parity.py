@@ -188,6 +188,21 @@ def assembly_case(rng, name):
     elif name == "wide_dirichlet":    # two 128-cell k tiles, the second partial
         go = orc.grid([6, 5, 140], size=[0.5, 0.4, 1.0], ghost=2)
         kw = dict(dt=1e-2, diff=1e-3, bc_type=[1, 2, 2, 1, 1, 1], bc_value=[0.5, 0, 0, 1.0, 0, -0.5])
+    elif name == "vardiff_sym":       # per-cell diffusivity (harmonic face means, taken in face
+        # order) + source, no convection, no periodic axis: exactly symmetric by construction,
+        # applied with the four-slot pair traversal
+        go = orc.grid([9, 10, 12], size=[0.9, 1, 1.2], ghost=2)
+        kw = dict(dt=5e-3, diff_cell=True, source=True, bc_type=[1, 2, 1, 1, 2, 1],
+                  bc_value=[0.25, 0, -0.5, 0.75, 0, 1.0])
+    elif name == "periodic_vardiff":  # per-cell diffusivity across periodic wraps: the wrapped
+        # faces read the (random) ghost diffusivities, so the device check finds the view
+        # asymmetric and the seven-slot traversal runs
+        go = orc.grid([6, 7, 10], ghost=2, periodic=[1, 1, 1])
+        kw = dict(dt=5e-3, diff_cell=True)
+    elif name == "fixed_diff":        # fixed x faces (identity rows) without convection: a
+        # candidate for the symmetric traversal that the device check must reject
+        go = orc.grid([10, 9, 8], size=[10 / 9, 1, 1], ghost=2)
+        kw = dict(dt=5e-3, diff=2e-3, bc_type=[3, 3, 1, 2, 1, 1], bc_value=[0.5, -0.25, 0.1, 0, 0.2, 0.3])
     elif name == "react2":
         go = orc.grid([7, 8, 9], ghost=2, ncomp=2)
         kw = dict(dt=0.1, diff_comp=[1e-3, 5e-3], react=True, bc_type=[1] * 6,
@@ -333,7 +348,7 @@ def test_mixed_refinement_reaches_fp64_residual(ctx, rng, name):
 
 
 DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "periodic_even", "periodic_sym", "periodic_single",
-             "wide_dirichlet", "react2", "fixed_x", "fixed_react2"]
+             "wide_dirichlet", "react2", "fixed_x", "fixed_react2", "vardiff_sym", "periodic_vardiff", "fixed_diff"]
 
 
 def device_assembly(ctx, rng, name):
@@ -372,6 +387,67 @@ def test_diagonal_view_spmv_matches_oracle(ctx, rng, name):
     else:
         assert bits_equal(got, exp)
     assert abs(d - float(exp @ w)) <= 1e-12 * float(np.abs(exp) @ np.abs(w))
+    A.close()
+
+
+CONVECTION_CASES = {"var_conv_src", "periodic_conv", "periodic_even", "fixed_x"}
+
+
+def csr_bitwise_symmetric(rp, col, val):
+    """A == A^T entry for entry and bit for bit (structure included)."""
+    import scipy.sparse as sp
+    n = len(rp) - 1
+    M = sp.csr_matrix((val, col, rp), shape=(n, n))
+    T = M.T.tocsr()
+    M.sort_indices()
+    T.sort_indices()
+    return (np.array_equal(M.indptr, T.indptr) and np.array_equal(M.indices, T.indices)
+            and np.array_equal(M.data.view(np.uint64), T.data.view(np.uint64)))
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+def test_symmetric_traversal_only_on_symmetric_operators(ctx, rng, name):
+    """The four-slot symmetric traversal runs exactly when the operator is symmetric: the
+    solver's flag (by construction of the fill, or the device's slot-by-slot check) against
+    the oracle's CSR, A == A^T bit for bit, for the one-component operators without convection
+    whose k extent is even (the paired traversal); never for the others."""
+    go, A, rhs, x0, (rp, col, val, orhs, ox0), keep = device_assembly(ctx, rng, name)
+    eligible = go.ncomp == 1 and go.n[2] % 2 == 0 and name not in CONVECTION_CASES
+    expected = eligible and csr_bitwise_symmetric(rp, col, val)
+    assert da.csr_symmetric(ctx, A) == expected
+    if name in ("diff_dirichlet", "vardiff_sym", "periodic_sym", "wide_dirichlet"):
+        assert expected       # the cases that exercise the symmetric traversal
+    if name in ("periodic_vardiff", "fixed_diff"):
+        assert eligible and not expected
+    A.close()
+
+
+@pytest.mark.parametrize("name", ["diff_dirichlet", "vardiff_sym", "periodic_sym", "wide_dirichlet"])
+def test_symmetric_traversal_is_bit_identical(ctx, rng, name):
+    """Four-slot (symmetric) against seven-slot traversal on the same matrix: SpMV + dot, the
+    FP64 solve (FP64 pair traversal) and the mixed solve (FP32 quad traversal, FP64
+    residuals) give the same bits and the same pass counts."""
+    import torch
+    go, A, rhs, x0, ref, keep = device_assembly(ctx, rng, name)
+    n = len(ref[3])
+    x = dev(rng.normal(size=n))
+    w = dev(rng.normal(size=n))
+    out = {}
+    for on in (True, False):
+        da.set_symmetric_traversal(A, on)
+        assert da.csr_symmetric(ctx, A) == on
+        y = torch.zeros_like(x)
+        d = da.spmv_dot(ctx, A, x, y, w)
+        xs = x0.clone()
+        r = da.bicgstab2(ctx, A, rhs, xs, tol=1e-10, tol_kind=1, max_it=300)
+        xm = x0.clone()
+        rm = da.refine_mixed(ctx, A, rhs, xm, tol=1e-10, tol_kind=1, inner_tol=1e-4)
+        assert r["converged"] == 1 and rm["converged"] == 1
+        out[on] = (host(y), d, host(xs), r["iterations"], host(xm), rm["iterations"], rm["refinements"])
+    a, b = out[True], out[False]
+    assert bits_equal(a[0], b[0]) and a[1] == b[1]
+    assert bits_equal(a[2], b[2]) and a[3] == b[3]
+    assert bits_equal(a[4], b[4]) and a[5:] == b[5:]
     A.close()
 
 

@@ -396,6 +396,39 @@ def test_constant_diff_cell_equals_constant_diffusivity():
         assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
 
 
+def test_per_cell_diffusivity_operator_is_exactly_symmetric():
+    """DDT + per-cell diffusivity (harmonic face means) + source with Dirichlet / Neumann
+    faces and no periodic axis: A == A^T bit for bit (the harmonic mean takes its two cell
+    values in face order from either side), the property the device solver's four-slot
+    traversal relies on. Across a periodic wrap the face reads ghost diffusivities, and with
+    ghosts that are not the periodic images the operator is not symmetric."""
+    import scipy.sparse as sp
+
+    def bitwise_symmetric(rp, col, val):
+        n = len(rp) - 1
+        M = sp.csr_matrix((val, col, rp), shape=(n, n))
+        T = M.T.tocsr()
+        M.sort_indices()
+        T.sort_indices()
+        return (np.array_equal(M.indices, T.indices) and np.array_equal(M.indptr, T.indptr)
+                and np.array_equal(M.data.view(np.uint64), T.data.view(np.uint64)))
+
+    rng = np.random.default_rng(29)
+    for n, size, bc in [([6, 7, 8], [1.0, 0.8, 1.1], [1, 2, 1, 1, 2, 1]), ([5, 9, 4], [0.5, 1.0, 0.7], [1] * 6)]:
+        g = orc.grid(n, size=size, ghost=2)
+        npad = int(np.prod(padded_shape(g)))
+        D = np.exp(rng.normal(np.log(1e-3), 1.0, size=npad))
+        t = orc.terms(dt=5e-3, diff_cell=D, source=rng.uniform(size=npad), bc_type=bc,
+                      bc_value=list(rng.normal(size=6)))
+        rp, col, val, _, _ = orc.assemble(g, t, rng.normal(size=npad))
+        assert bitwise_symmetric(rp, col, val)
+    g = orc.grid([6, 7, 8], ghost=2, periodic=[1, 1, 1])
+    npad = int(np.prod(padded_shape(g)))
+    t = orc.terms(dt=5e-3, diff_cell=np.exp(rng.normal(np.log(1e-3), 1.0, size=npad)))
+    rp, col, val, _, _ = orc.assemble(g, t, rng.normal(size=npad))
+    assert not bitwise_symmetric(rp, col, val)
+
+
 # ----------------------------------------------------------------------------- planted mistakes
 # (mutant name, text in oracle.cpp, replacement, pins that must fail)
 MUTANTS = [
