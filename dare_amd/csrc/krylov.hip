@@ -133,9 +133,10 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
     *hist_len = 0;
     *status = 0;
     XT* y = (mg && !x_zero) ? y_buf : x;       // the iterate the passes update
+    const XT* rhv = x_zero ? b : rh;           // the shadow residual r0: b itself when x0 = 0
     auto pre = [&](const XT* in) -> const XT* { return mg ? mg_apply<VT, XT>(sm, *mg, in) : in; };
     if (x_zero) {   // x = 0 written here: no operator sweep needed for r = b - A x
-        hipLaunchKernelGGL((k_init_zero<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, b, x, r, rh, u, v, w, ws, st);
+        hipLaunchKernelGGL((k_init_zero<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, b, x, r, u, ws, st);
         launch_finish<2>(sm, ws, gr, st, TAIL_INIT, 0);
     } else {
         A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);     // (0)-(2)
@@ -161,15 +162,15 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
             break;
         }
         hipLaunchKernelGGL((k_upd_u<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, u, st);
-        A.launch(sm, pre(u), EpiDot<XT, NT>{v, rh}, ws, st, TAIL_GAMMA, 1);
+        A.launch(sm, pre(u), EpiDot<XT, NT>{v, rhv}, ws, st, TAIL_GAMMA, 1);
         hipLaunchKernelGGL((k_upd_rx<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, y, u, st);
-        A.launch(sm, pre(r), EpiDot<XT, NT>{s, rh}, ws, st, TAIL_RHO, 1);
+        A.launch(sm, pre(r), EpiDot<XT, NT>{s, rhv}, ws, st, TAIL_RHO, 1);
         hipLaunchKernelGGL((k_upd_v<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, s, v, st);
-        A.launch(sm, pre(v), EpiDot<XT, NT>{w, rh}, ws, st, TAIL_GAMMA, 1);
+        A.launch(sm, pre(v), EpiDot<XT, NT>{w, rhv}, ws, st, TAIL_GAMMA, 1);
         hipLaunchKernelGGL((k_upd_urs<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, u, r, v, s, w, st);
         if (mg) A.launch(sm, pre(s), EpiGcrP<XT, NT>{t, r, s}, ws, st, TAIL_GCR, 1);
         else A.launch(sm, s, EpiGcr<XT, NT>{t, r}, ws, st, TAIL_GCR, 1);
-        hipLaunchKernelGGL((k_end<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, y, r, s, t, u, v, w, rh, ws, st);
+        hipLaunchKernelGGL((k_end<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, y, r, s, t, u, v, w, rhv, ws, st);
         launch_finish<2>(sm, ws, gr, st, TAIL_END, 1);
         DARE_LAUNCH_CHECK();
         rc = read_state(ctx, st, &h);

@@ -241,7 +241,11 @@ DiaOp<VT> make_dia(const DiaView& V, const VT* D, bool allow_pair = false) {
 // operator() gets y = (A x)_i, xc = x_i and the loaded operands, stores results and adds to
 // the per-thread dot accumulators.
 template <typename XT, bool NT = false>
-struct EpiInit {        // r = b - A x; rh = r; u = v = w = 0; <r,r>, <rh,r>   (BiCGStab2.inl:265-273)
+// r = b - A x; rh = r; u = 0; <r,r>, <rh,r>   (BiCGStab2.inl:265-273). The reference also sets
+// v = w = 0; here both are written (by the SpMVs of steps 6 and 13) before the first pass reads
+// them, so those two zero sweeps are left out. v and w are kept in the member list for the
+// batched solver's call sites.
+struct EpiInit {
     static constexpr int NDOT = 2;
     static constexpr bool kNT = NT;
     const XT* b;
@@ -252,8 +256,6 @@ struct EpiInit {        // r = b - A x; rh = r; u = v = w = 0; <r,r>, <rh,r>   (
         put<NT>(r + i, ri);
         put<NT>(rh + i, ri);
         put<NT>(u + i, XT(0));
-        put<NT>(v + i, XT(0));
-        put<NT>(w + i, XT(0));
         const double rd = static_cast<double>(ri);
         acc[0] += rd * rd;
         acc[1] += rd * rd;
@@ -372,7 +374,7 @@ struct EpiStore {
 
 // The epilogue of a lane's two adjacent rows i, i+1 (paired traversal): by default two calls,
 // whose single stores land next to each other and merge into one wide store; EpiInit writes
-// five vectors per row, so its pair form orders the stores vector by vector for the same
+// three vectors per row, so its pair form orders the stores vector by vector for the same
 // effect. The dot accumulators see row i, then row i+1, as with two calls.
 template <class Epi>
 __device__ __forceinline__ void epi_pair(const Epi& e, long long i, double y0, double y1, double x0, double x1,
@@ -390,10 +392,6 @@ __device__ __forceinline__ void epi_pair(const EpiInit<XT, NT>& e, long long i, 
     put<NT>(e.rh + i + 1, r1);
     put<NT>(e.u + i, XT(0));
     put<NT>(e.u + i + 1, XT(0));
-    put<NT>(e.v + i, XT(0));
-    put<NT>(e.v + i + 1, XT(0));
-    put<NT>(e.w + i, XT(0));
-    put<NT>(e.w + i + 1, XT(0));
     const double d0 = static_cast<double>(r0), d1 = static_cast<double>(r1);
     acc[0] += d0 * d0;
     acc[1] += d0 * d0;
@@ -421,10 +419,6 @@ __device__ __forceinline__ void epi_quad(const EpiInit<XT, NT>& e, long long i, 
     for (int m = 0; m < 4; m++) put<NT>(e.rh + i + m, r[m]);
 #pragma unroll
     for (int m = 0; m < 4; m++) put<NT>(e.u + i + m, XT(0));
-#pragma unroll
-    for (int m = 0; m < 4; m++) put<NT>(e.v + i + m, XT(0));
-#pragma unroll
-    for (int m = 0; m < 4; m++) put<NT>(e.w + i + m, XT(0));
 #pragma unroll
     for (int m = 0; m < 4; m++) {
         const double dd = static_cast<double>(r[m]);
@@ -1219,11 +1213,12 @@ __global__ __launch_bounds__(kThreads) void k_dot(long long n, const XT* __restr
 }
 
 // Initialisation (0)-(2) for a zero initial guess: A x0 is +0 in every row, so r = b - A x0 = b
-// bit for bit; x = u = v = w = 0, rh = r, and <r,r>, <rh,r> go to the reduction as EpiInit's do.
+// bit for bit; x = u = 0, r = b, and <r,r>, <rh,r> go to the reduction as EpiInit's do. rh = r0
+// is never written again, so the caller reads b itself as rh (b stays untouched during the
+// solve); v and w are written before they are read (EpiInit).
 template <typename XT, bool NT>
 __global__ __launch_bounds__(kThreads) void k_init_zero(long long n, const XT* __restrict__ b, XT* __restrict__ x,
-                                                        XT* __restrict__ r, XT* __restrict__ rh, XT* __restrict__ u,
-                                                        XT* __restrict__ v, XT* __restrict__ w, ReduceWs ws,
+                                                        XT* __restrict__ r, XT* __restrict__ u, ReduceWs ws,
                                                         KState* st) {
     double acc[2] = {0., 0.};
     DARE_CHUNK_LOOP(n) {
@@ -1236,10 +1231,7 @@ __global__ __launch_bounds__(kThreads) void k_init_zero(long long n, const XT* _
             if (i < n) {
                 put<NT>(x + i, XT(0));
                 put<NT>(r + i, bv[q]);
-                put<NT>(rh + i, bv[q]);
                 put<NT>(u + i, XT(0));
-                put<NT>(v + i, XT(0));
-                put<NT>(w + i, XT(0));
                 const double rd = static_cast<double>(bv[q]);
                 acc[0] += rd * rd;
                 acc[1] += rd * rd;
