@@ -605,7 +605,6 @@ int dare_csr_destroy(dare_csr* A) {
     if (A->val32) hipFree(A->val32);
     if (A->dia) hipFree(A->dia);
     if (A->dia32) hipFree(A->dia32);
-    if (A->asym32) hipFree(A->asym32);
     if (A->asym64) hipFree(A->asym64);
     if (A->dinv) hipFree(A->dinv);
     if (A->dia_pre) hipFree(A->dia_pre);

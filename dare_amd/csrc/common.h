@@ -110,15 +110,13 @@ struct dare_csr {
     // the FP32 copy for mixed solves: once a mixed solve has asked for it (dia32_mode), every
     // fill writes it next to the FP64 slots; dia32_fresh says it holds float(dia) of the last fill
     int dia32_mode = 0, dia32_fresh = 0;
-    // device int: 0 when the FP32 diagonal view is exactly symmetric (E, N, T of each row equal
-    // W, S, B of its neighbour), set by the check each mixed solve runs on a fresh copy
-    int* asym32 = nullptr;
     // 1: the last device fill's operator is exactly symmetric by construction (one component,
     // no convection, reaction or fixed faces, per-cell diffusivity only without a periodic axis:
     // every face coefficient is the same expression for both rows it couples)
     int sym_fill = 0;
     // 1: the last device fill may be exactly symmetric (one component, no convection or
-    // reaction): the FP64 and FP32 views are then checked on the device unless sym_fill holds
+    // reaction): the FP64 view is then checked on the device unless sym_fill holds (the FP32
+    // copy of a symmetric FP64 view is symmetric too)
     int sym_cand = 0;
     // the FP64 view's symmetry for the solvers: -1 not yet known since the last fill, 0 not
     // exactly symmetric, 1 symmetric (by construction, or by the device check whose result

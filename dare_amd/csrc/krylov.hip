@@ -39,9 +39,14 @@ struct SolveOp {
                 hipLaunchKernelGGL((k_apply_dia<2, VT, XT, Epi>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi, ws, st,
                                    tail, check);
             else if (d.pair == 4) {   // FP32 operators only (make_dia)
-                if constexpr (std::is_same_v<VT, float> && std::is_same_v<XT, float>)
-                    hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 4>), dim3(grid), dim3(kThreads), 0, sm, d, x, epi,
-                                       ws, st, tail, check);
+                if constexpr (std::is_same_v<VT, float> && std::is_same_v<XT, float>) {
+                    if (d.sym)
+                        hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 4, true>), dim3(grid), dim3(kThreads), 0, sm, d,
+                                           x, epi, ws, st, tail, check);
+                    else
+                        hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 4>), dim3(grid), dim3(kThreads), 0, sm, d, x,
+                                           epi, ws, st, tail, check);
+                }
             } else if (d.pair == 2 && d.sym) {   // FP64 operators only (op64)
                 if constexpr (std::is_same_v<VT, double>)
                     hipLaunchKernelGGL((k_apply_dia<1, VT, XT, Epi, 2, true>), dim3(grid), dim3(kThreads), 0, sm, d, x,
@@ -822,21 +827,12 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         op32.d = make_dia<float>(A->dia_view, A->dia32, true);
         op32.dia = kXcd * op32.d.per_xcd <= kReduceBlocks;
         if (op32.dia && op32.d.pair == 4 && !pre && A->sym_traversal) {
-            // exactly symmetric FP32 view (a symmetric operator, Jacobi column scaling off):
-            // the quad traversal may then read E, N, T from the neighbours' W, S, B slots
-            if (!A->asym32 && hipMalloc(&A->asym32, sizeof(int)) != hipSuccess) {
-                hipGetLastError();
-                set_error("symmetry flag allocation failed");
-                return DARE_ERR_OOM;
-            }
-            // convection or reaction: never symmetric; otherwise symmetric by construction or
-            // compared slot by slot
-            const bool cand = A->sym_fill || A->sym_cand;
-            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, ctx->stream, A->asym32, cand ? 0 : 1);
-            if (cand && !A->sym_fill)
-                hipLaunchKernelGGL((k_dia_asym<float>), dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream,
-                                   A->dia_view, static_cast<const float*>(A->dia32), A->asym32);
-            op32.d.asym = A->asym32;
+            // the FP32 copy of an exactly symmetric FP64 view is exactly symmetric (Jacobi
+            // column scaling off): the quad traversal then reads E, N, T from the neighbours'
+            // W, S, B slots. The FP64 view's symmetry is known on the host (sym64_prepare).
+            const int src = sym64_prepare(ctx, A);
+            if (src) return src;
+            op32.d.sym = A->sym64 == 1;
         }
     } else {
         int erc = csr_ensure_values(A, ctx->stream);

@@ -211,8 +211,7 @@ struct DiaOp {
     int tile_i;         // planes per tile (grown until the grid fits the reduction slots)
     int per_xcd;        // tiles owned by each XCD (grid = kXcd * per_xcd)
     int pair;           // cells per lane along k (2: tiles of 128 (k) x 4 (j), one component, n2 even)
-    const int* asym = nullptr;   // FP32 quad form: *asym == 0 selects the symmetric traversal
-    bool sym = false;            // pair form: the view is exactly symmetric (dia_column_pair_sym)
+    bool sym = false;            // the view is exactly symmetric: the four-slot pair / quad traversals
 };
 
 template <typename VT>
@@ -869,7 +868,7 @@ __device__ inline void dia_column_quad(const DiaView& V, const float* __restrict
     }
 }
 
-// dia_column_quad for an exactly symmetric FP32 view (checked by k_dia_asym): the E, N and T
+// dia_column_quad for an exactly symmetric FP32 view (the FP32 copy of a symmetric FP64 view): the E, N and T
 // coefficients of a row are read as the W, S and B coefficients of its east, north and top
 // neighbours, so four slots stream from HBM instead of seven. The partner values are the same
 // floats and the arithmetic is that of dia_column_quad, so the results are bit-identical. The
@@ -1007,7 +1006,8 @@ __global__ __launch_bounds__(kThreads) void k_dia_asym(DiaView V, const VT* __re
 
 __global__ void k_set_int(int* __restrict__ p, int v) { p[0] = v; }
 
-// PAIR: cells per lane along k (1, 2, or 4 for FP32); SYM (pairs): the symmetric traversal
+// PAIR: cells per lane along k (1, 2, or 4 for FP32); SYM (pairs and quads): the symmetric
+// traversal, a separate instantiation so that the seven-slot one keeps its own register count
 template <int NC, typename VT, typename XT, class Epi, int PAIR = 1, bool SYM = false>
 __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* __restrict__ x, Epi epi, ReduceWs ws,
                                                        KState* st, int tail, int check_status) {
@@ -1029,7 +1029,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_dia(DiaOp<VT> A, const XT* _
             const int i1 = (i0 + A.tile_i < A.V.n[0]) ? i0 + A.tile_i : A.V.n[0];
             if constexpr (PAIR == 4) {
                 static_assert(std::is_same_v<VT, float> && std::is_same_v<XT, float>, "quad traversal is FP32");
-                if (A.asym != nullptr && *A.asym == 0) dia_column_quad_sym<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
+                if constexpr (SYM) dia_column_quad_sym<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
                 else dia_column_quad<Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
             } else if constexpr (PAIR == 2 && SYM) {
                 dia_column_pair_sym<VT, XT, Epi>(A.V, A.D, x, k, j, i0, i1, epi, acc);
