@@ -1096,6 +1096,7 @@ struct dare_convnet {
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     uint16_t *wf = nullptr, *wb = nullptr;
     uint16_t *ws6 = nullptr, *ws8 = nullptr;   // folded sub-pixel weights of layers 6 and 8
+    bool folded_stale = true;                   // ws6 / ws8 not yet folded from the current wf
     int adam_t = 0;
     bool inference = false;    // forward skips a7, a9 (read only by the backward pass)
     bool acts_for_bwd = false; // the last forward pass wrote every activation bwd reads
@@ -1210,13 +1211,11 @@ void conv(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Y, fl
 // 27 taps folded into 8 per output parity, Ws refreshed from the current BF16 weights first).
 // Y may be NULL (inference: only the sum is written).
 template <int CIN, int COUT>
-void conv_up(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Ws, uint16_t* Y, const int* d, int level,
+void conv_up(hipStream_t sm, const uint16_t* X, const uint16_t* Ws, uint16_t* Y, const int* d, int level,
              const uint16_t* E, uint16_t* Y2) {
     static_assert(CIN % 32 == 0 && COUT % 16 == 0, "sub-pixel conv: 32-channel K slices, 16-channel groups");
     const int D = d[0] >> level, H = d[1] >> level, W = d[2] >> level;
     const dim3 blk(kNetThreads);
-    hipLaunchKernelGGL((k_sub_weights<CIN, COUT>), dim3((64 * CIN * COUT + kNetThreads - 1) / kNetThreads), blk, 0, sm,
-                       Wt, Ws);
     const unsigned gy = static_cast<unsigned>((H + 3) / 4) * (COUT / 16);
     // tile width by registers (2 x 8 x CIN/32 weight fragments + the operands of two K steps):
     // two waves per SIMD at 32 (CIN 32) or 16 (CIN 64) coarse columns
@@ -1236,6 +1235,19 @@ void conv_up(hipStream_t sm, const uint16_t* X, const uint16_t* Wt, uint16_t* Ws
         hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 2, true>), grid, blk, 0, sm, X, Ws, Y, D, H, W, zb, E, Y2);
     else
         hipLaunchKernelGGL((k_conv_sub<CIN, COUT, true, 1, true>), grid, blk, 0, sm, X, Ws, Y, D, H, W, zb, E, Y2);
+}
+
+// the sub-pixel weights of layers 6 and 8 folded from the current BF16 weights, once after
+// every change of the weights (set_params, Adam step), outside any captured prediction graph
+void fold_sub_weights(hipStream_t sm, dare_convnet* net) {
+    if (!net->folded_stale) return;
+    const dim3 blk(kNetThreads);
+    constexpr int n6 = 64 * kDefs[6].cin * kDefs[6].cout, n8 = 64 * kDefs[8].cin * kDefs[8].cout;
+    hipLaunchKernelGGL((k_sub_weights<kDefs[6].cin, kDefs[6].cout>), dim3((n6 + kNetThreads - 1) / kNetThreads), blk,
+                       0, sm, net->wf + net->off[6], net->ws6);
+    hipLaunchKernelGGL((k_sub_weights<kDefs[8].cin, kDefs[8].cout>), dim3((n8 + kNetThreads - 1) / kNetThreads), blk,
+                       0, sm, net->wf + net->off[8], net->ws8);
+    net->folded_stale = false;
 }
 
 // dispatch on (cin, cout) of the network's layers
@@ -1444,6 +1456,7 @@ int dare_convnet_set_params(dare_ctx* ctx, dare_convnet* net, const float* host)
                            net->wf + net->off[l], net->wb + net->off[l], kDefs[l].cin, kDefs[l].cout, 0, 0.f, 0.f, 0.f,
                            0.f, 1.f, 1.f);
     }
+    net->folded_stale = true;
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }
@@ -1498,6 +1511,7 @@ int dare_convnet_pack(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, cons
 int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
     DARE_CHECK_ARG(ctx && net, "NULL argument");
     hipStream_t sm = ctx->stream;
+    fold_sub_weights(sm, net);
     const int* d = net->dims;
     const long long V1 = level_voxels(d, 1);
     const long long V0 = level_voxels(d, 0);
@@ -1516,9 +1530,9 @@ int dare_conv_net_fwd(dare_ctx* ctx, dare_convnet* net) {
     // decoder: the upsampling is read inside the conv (u2, u1 are written by the backward pass,
     // which needs them) and the skip sums s2 = a7 + e2, s1 = a9 + e1 come out of its epilogue
     // in inference mode the ReLU outputs a7, a9 (the backward pass's masks) are not stored
-    conv_up<64, 32>(sm, net->e3, W + o[6], net->ws6, net->inference ? nullptr : net->a7, d, 1, net->e2, net->s2);
+    conv_up<64, 32>(sm, net->e3, net->ws6, net->inference ? nullptr : net->a7, d, 1, net->e2, net->s2);
     conv<32, 32, true, 0>(sm, net->s2, W + o[7], net->d2, nullptr, d, 1);
-    conv_up<32, 16>(sm, net->d2, W + o[8], net->ws8, net->inference ? nullptr : net->a9, d, 0, net->e1, net->s1);
+    conv_up<32, 16>(sm, net->d2, net->ws8, net->inference ? nullptr : net->a9, d, 0, net->e1, net->s1);
     conv<16, 16, true, 0>(sm, net->s1, W + o[9], net->d1, nullptr, d, 0);
     conv<16, 16, false, 1>(sm, net->d1, W + o[10], nullptr, net->out, d, 0);
     DARE_LAUNCH_CHECK();
@@ -1688,6 +1702,7 @@ int dare_conv_net_step(dare_ctx* ctx, dare_convnet* net, float lr, float beta1, 
                            net->wf + net->off[l], net->wb + net->off[l], kDefs[l].cin, kDefs[l].cout, 1, lr, beta1,
                            beta2, eps, c1, c2);
     }
+    net->folded_stale = true;
     DARE_LAUNCH_CHECK();
     return DARE_OK;
 }
@@ -1731,6 +1746,9 @@ int dare_convnet_predict(dare_ctx* ctx, dare_convnet* net, const dare_grid* g, c
     }
     put(&net->cf, sizeof(int));
     put(&ctx->device, sizeof(int));
+    // the folded sub-pixel weights are refreshed on the context stream, never inside the graph,
+    // so a graph replays correctly across weight updates
+    fold_sub_weights(ctx->stream, net);
     if (!net->predict_exec || key != net->predict_key) {
         if (net->predict_exec) (void)hipGraphExecDestroy(net->predict_exec);
         net->predict_exec = nullptr;

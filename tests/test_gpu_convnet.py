@@ -335,4 +335,18 @@ def test_graph_predict_equals_eager_on_every_replay(ctx, rng, coarsen):
     net.predict(g, dev[0], dev[1], base, dev[2], 0.7, other, extras=extras)
     ctx.synchronize()
     assert torch.equal(other, xe)
+    # new weights after the capture (set_params, then an Adam step): the replayed graph uses
+    # them, the sub-pixel weights of the decoder included (folded outside the graph)
+    for update in ("set_params", "step"):
+        if update == "set_params":
+            net.set_params(he_init(4, seed=11, out_scale=1.0))
+        else:
+            net.train_step(g, dev[0], dev[1], base, dev[2], 0.7, base, lr=1e-2, extras=extras)
+        net.predict(g, dev[0], dev[1], base, dev[2], 0.7, xg, extras=extras)
+        ctx.synchronize()
+        net.predict_eager(g, dev[0], dev[1], base, dev[2], 0.7, xe, extras=extras)
+        ctx.synchronize()
+        assert torch.equal(xg, xe), update
+        assert not torch.equal(xe, other), update
+        other.copy_(xe)
     net.close()
