@@ -87,12 +87,20 @@ class Terms(ctypes.Structure):
                 ("use_diff_comp", ctypes.c_int32), ("conv_limiter", ctypes.c_int32),
                 ("vel", ctypes.c_void_p * 3), ("source", ctypes.c_void_p),
                 ("react", ctypes.c_void_p), ("bc_type", ctypes.c_int32 * 6),
-                ("bc_value", ctypes.c_double * 6)]
+                ("bc_value", ctypes.c_double * 6), ("div_weights", ctypes.c_double * 3),
+                ("div_levels", ctypes.c_int32)]
 
     @classmethod
     def make(cls, dt=0.0, diff=0.0, diff_cell=None, diff_comp=None, limiter=-1, vel=None,
-             source=None, react=None, bc_type=None, bc_value=None):
+             source=None, react=None, bc_type=None, bc_value=None, div_scheme=None):
+        """div_scheme: the time-scheme weights of the divergence terms (EULER_BACKWARD,
+        EULER_FORWARD, CRANK_NICHOLSON, ADAMS_BASHFORT, ADAMS_MOULTON below); None is
+        EULER_BACKWARD."""
         t = cls()
+        if div_scheme is not None:
+            t.div_levels = len(div_scheme)
+            for i, w in enumerate(div_scheme):
+                t.div_weights[i] = w
         t.dt = dt
         t.diff_const = diff
         t.diff_cell = _ptr(diff_cell)
@@ -113,6 +121,14 @@ class Terms(ctypes.Structure):
             t.bc_value[f] = bv[f]
         t._keep = (diff_cell, vel, source, react)
         return t
+
+
+# Divergence<Grid, TimeScheme> weights (Equations/TimeDiscretizationSchemes.h:29-76)
+EULER_BACKWARD = (1.0,)
+EULER_FORWARD = (0.0, 1.0)
+CRANK_NICHOLSON = (0.5, 0.5)
+ADAMS_BASHFORT = (0.0, 1.5, -0.5)
+ADAMS_MOULTON = (5.0 / 12.0, 8.0 / 12.0, -1.0 / 12.0)
 
 
 class SolverParams(ctypes.Structure):

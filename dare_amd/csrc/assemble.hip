@@ -37,6 +37,8 @@ struct Terms {
     double bc_value[6];
     long long fs[3][3];        // face array strides
     const double* phi;         // phi_old, padded
+    double div_w[3];           // the divergences' time-scheme weights (div_w[0] = 1: EULER_BACKWARD)
+    int div_nt;                // their time levels (1: EULER_BACKWARD)
 };
 
 __device__ inline double lim_psi(int id, double r) {
@@ -162,12 +164,14 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
                 o[2 * d + 1] = -G.A[d] * nbW;
                 o[0] -= G.A[d] * cW;
             }
-            for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += o[pos] * -1.;
+            // the scheme weight of the implicit level (s_c *= coef_td); the gradient's face
+            // right-hand sides are 0, so the explicit levels add nothing here
+            for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += (o[pos] * T.div_w[0]) * -1.;
             rhs += 0. * -1.;
         }
         if ((FEAT & kFeatConv) && T.conv_limiter >= 0) {   // mb += div(tvd * phi)
             double o[7] = {0., 0., 0., 0., 0., 0., 0.};
-            double orhs = 0.;
+            double orhs = 0., frd[3] = {0., 0., 0.};
             for (int d = 0; d < dim; d++) {
                 long long fi = 0;
                 for (int e = 0; e < dim; e++) fi += T.fs[d][e] * q[e];
@@ -204,10 +208,15 @@ __device__ int row_entries(const Geo& G, const Terms& T, const int q[3], int c, 
                 else o[0] += G.A[d] * cc[1];
                 o[2 * d + 1] = -G.A[d] * nb[0];
                 o[0] -= G.A[d] * cc[0];
-                if (d == 0) orhs = G.A[0] * (fr[1] - fr[0]);
-                else orhs += G.A[d] * (fr[1] - fr[0]);
+                frd[d] = fr[1] - fr[0];
+                if (d == 0) orhs = G.A[0] * frd[0];
+                else orhs += G.A[d] * frd[d];
             }
-            for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += o[pos];
+            // time scheme (Divergence_Cartesian.inl:183-204): coefficients times w0, the face
+            // right-hand sides once more per explicit level t, weighted by w_t
+            for (int t = 1; t < T.div_nt; t++)
+                for (int d = 0; d < dim; d++) orhs += T.div_w[t] * G.A[d] * frd[d];
+            for (int pos = 0; pos < 2 * dim + 1; pos++) nbv[pos] += o[pos] * T.div_w[0];
             rhs += orhs;
         }
         if ((FEAT & kFeatSource) && T.source) rhs += G.V * T.source[lp * N + c];
@@ -486,6 +495,12 @@ int make_terms(const Geo& G, const dare_terms* t, const double* phi, Terms* T) {
     T->source = t->source;
     T->react = t->react;
     T->cross = (t->react != nullptr && G.N == 2) ? 1 : 0;
+    if (t->div_levels < 0 || t->div_levels > 3) {
+        set_error("div_levels must be 0..3");
+        return DARE_ERR_INVALID;
+    }
+    T->div_nt = t->div_levels > 0 ? t->div_levels : 1;
+    for (int l = 0; l < 3; l++) T->div_w[l] = (t->div_levels > 0 && l < t->div_levels) ? t->div_weights[l] : (l == 0 ? 1. : 0.);
     for (int f = 0; f < 6; f++) {
         T->bc_type[f] = f < 2 * G.dim ? t->bc_type[f] : 0;
         T->bc_value[f] = t->bc_value[f];

@@ -338,6 +338,15 @@ struct Terms {
     // the boundary cells of this face carry v themselves (identity rows): Dirichlet on a
     // staggered axis, whose first / last cells sit on the boundary faces
     Terms& Fixed(int face, double v) { t.bc_type[face] = DARE_BC_FIXED; t.bc_value[face] = v; return *this; }
+    // the time scheme of the diffusion and convection divergences, as Divergence<Grid, Scheme>
+    // (EULER_BACKWARD, EULER_FORWARD, CRANK_NICHOLSON, ADAMS_BASHFORT, ADAMS_MOULTON)
+    template <typename Scheme>
+    Terms& DivergenceScheme() {
+        const auto w = Scheme::template GetWeights<double>();
+        t.div_levels = static_cast<int32_t>(w.size() < 3 ? w.size() : 3);
+        for (std::size_t l = 0; l < w.size() && l < 3; l++) t.div_weights[l] = w[l];
+        return *this;
+    }
 };
 
 // A built preconditioner (the RCP<OperatorType> M of the reference): which right

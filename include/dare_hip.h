@@ -95,6 +95,16 @@ typedef struct {
     const double* react;     /* ncomp == 2: per-cell 2x2 block [a b; c d], internal order */
     int32_t bc_type[6];      /* W,E,S,N,B,T: DARE_BC_* */
     double bc_value[6];
+    /* Time scheme of the diffusion and convection divergences, Divergence<Grid, TimeScheme>
+     * (Equations/TimeDiscretizationSchemes.h:29-76): div_levels = 0 (or 1) is EULER_BACKWARD
+     * (weights {1}); 2 or 3 levels take div_weights[0..levels-1] (EULER_FORWARD {0, 1},
+     * CRANK_NICHOLSON {0.5, 0.5}, ADAMS_BASHFORT {0, 1.5, -0.5}, ADAMS_MOULTON {5/12, 8/12,
+     * -1/12}). As in the reference's matrix form (Divergence_Cartesian.inl:58-66,141-207) the
+     * explicit levels reuse the implicit level's face stencil: the coefficients are scaled by
+     * w0 and the face right-hand sides (the TVD deferred correction) enter once unweighted and
+     * once per explicit level t with weight w_t. */
+    double div_weights[3];
+    int32_t div_levels;
 } dare_terms;
 
 /* Solver parameters. Replaces the Teuchos::ParameterList keys "Convergence Tolerance",

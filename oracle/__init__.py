@@ -35,7 +35,8 @@ class OTerms(ctypes.Structure):
                 ("use_diff_comp", ctypes.c_int32), ("conv_limiter", ctypes.c_int32),
                 ("vel", ctypes.c_void_p * 3), ("source", ctypes.c_void_p),
                 ("react", ctypes.c_void_p), ("bc_type", ctypes.c_int32 * 6),
-                ("bc_value", ctypes.c_double * 6)]
+                ("bc_value", ctypes.c_double * 6), ("div_weights", ctypes.c_double * 3),
+                ("div_levels", ctypes.c_int32)]
 
 
 def grid(n, size=None, ghost=2, ncomp=1, periodic=None):
@@ -137,8 +138,12 @@ def set_threads(n):
 
 
 def terms(dt=0.0, diff=0.0, diff_cell=None, diff_comp=None, limiter=-1, vel=None, source=None,
-          react=None, bc_type=None, bc_value=None):
+          react=None, bc_type=None, bc_value=None, div_scheme=None):
     t = OTerms()
+    if div_scheme is not None:          # Divergence time-scheme weights, e.g. (0.5, 0.5)
+        t.div_levels = len(div_scheme)
+        for i, w in enumerate(div_scheme):
+            t.div_weights[i] = w
     t.dt, t.diff_const = dt, diff
     t.diff_cell = _p(diff_cell)
     if diff_comp is not None:

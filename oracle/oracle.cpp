@@ -237,7 +237,12 @@ double div_value(const Geo& G, const FaceVal& s, int c) {
 
 // Divergence::ApplyDivergence(TFaceMatrixStencil) (Divergence_Cartesian.inl:141-207) for a
 // one-level scheme (EULER_BACKWARD, weight 1), the scheme the reference drivers pass.
-CenterMat div_matrix(const Geo& G, const FaceMat& s, double w0) {
+// The matrix-form divergence of a face stencil under a time scheme with nt levels and weights
+// w (Divergence_Cartesian.inl:141-207): the explicit levels carry copies of the implicit
+// level's face stencil (:58-66), so the coefficients are scaled by w[0] and the face
+// right-hand sides enter once unweighted and once per explicit level t with weight w[t].
+CenterMat div_matrix(const Geo& G, const FaceMat& s, const double* w, int nt) {
+    const double w0 = w[0];
     CenterMat o;
     zero(&o);
     for (int c = 0; c < G.N; c++) {
@@ -266,6 +271,13 @@ CenterMat div_matrix(const Geo& G, const FaceMat& s, double w0) {
         o.rhs[c] = G.A[0] * (s.rhs[c][1] - s.rhs[c][0]);
         if (G.dim > 1) o.rhs[c] += G.A[1] * (s.rhs[c][3] - s.rhs[c][2]);
         if (G.dim > 2) o.rhs[c] += G.A[2] * (s.rhs[c][5] - s.rhs[c][4]);
+    }
+    for (int t = 1; t < nt; t++) {
+        for (int c = 0; c < G.N; c++) {
+            o.rhs[c] += w[t] * G.A[0] * (s.rhs[c][1] - s.rhs[c][0]);
+            if (G.dim > 1) o.rhs[c] += w[t] * G.A[1] * (s.rhs[c][3] - s.rhs[c][2]);
+            if (G.dim > 2) o.rhs[c] += w[t] * G.A[2] * (s.rhs[c][5] - s.rhs[c][4]);
+        }
     }
     return o;
 }
@@ -489,6 +501,10 @@ void functor(const Geo& G, const orc_terms* t, const double* phi, MB* mb) {
     if (t->react && G.N == 2) {                    // explicit cross-component entries first
         for (int c = 0; c < 2; c++) mb->resize(c, 1);
     }
+    // the divergences' time scheme (0 levels: EULER_BACKWARD, weights {1})
+    const double euler[1] = {1.};
+    const int dn = t->div_levels > 0 ? t->div_levels : 1;
+    const double* dw = t->div_levels > 0 ? t->div_weights : euler;
     if (t->dt > 0.) {                              // DDT (DDT.inl:41-81), IMPLICIT weights (1,1)
         CenterMat s;
         zero(&s);
@@ -524,12 +540,12 @@ void functor(const Geo& G, const orc_terms* t, const double* phi, MB* mb) {
                 }
             }
         }
-        mb->subtract(div_matrix(G, gm, 1.));
+        mb->subtract(div_matrix(G, gm, dw, dn));
     }
     if (t->conv_limiter >= 0) {                    // implicit TVD convection, deferred correction
         double uf[6] = {0, 0, 0, 0, 0, 0};
         face_velocities(G, t->vel, p, uf);
-        mb->add(div_matrix(G, tvd_matrix(G, t->conv_limiter, phi, p, uf), 1.));
+        mb->add(div_matrix(G, tvd_matrix(G, t->conv_limiter, phi, p, uf), dw, dn));
     }
     if (t->source) {
         for (int c = 0; c < G.N; c++) mb->rhs[c] += G.V * t->source[lp * G.N + c];
@@ -850,7 +866,7 @@ int64_t orc_assemble(const orc_grid* gr, const orc_terms* t, const double* phi_o
                      int32_t* row_ptr, int32_t* col, double* val, int64_t cap,
                      double* rhs, double* x0) {
     Geo G;
-    if (!make_geo(gr, &G) || !t) return -1;
+    if (!make_geo(gr, &G) || !t || t->div_levels < 0 || t->div_levels > 3) return -1;
     const int64_t ncell = G.ncell_int;
     const int64_t nrows = ncell * G.N;
     std::vector<int32_t> cnt(static_cast<size_t>(nrows), 0);

@@ -41,6 +41,8 @@ typedef struct {
     const double* react;     /* per-cell 2x2 reaction block (internal order, 4 doubles) or NULL */
     int32_t bc_type[6];      /* W,E,S,N,B,T: 0 none/periodic, 1 Dirichlet mirror, 2 Neumann */
     double bc_value[6];
+    double div_weights[3];   /* Divergence<Grid, TimeScheme> weights of the diffusion and */
+    int32_t div_levels;      /* convection terms; 0: EULER_BACKWARD {1} */
 } orc_terms;
 
 enum { ORC_CDS = 0, ORC_UPWIND = 1, ORC_VANALBADA = 2, ORC_MINMOD = 3, ORC_VANLEER = 4 };

@@ -320,7 +320,10 @@ static void AdvectionPeriodic2D() {
 }
 
 // ---------------------------------------------------------------------------------------------
-static void General3D() {
+// DDT + harmonic-mean diffusion + VANLEER convection + source under the divergence time scheme
+// SCHEME (Divergence<Grid, SCHEME> for both divergences)
+template <typename SCHEME>
+static void General3DWith(const char* name) {
     using Grid = dare::Grid::Cartesian<3>;
     using SC = double;
     using LO = Grid::LocalOrdinalType;
@@ -351,7 +354,7 @@ static void General3D() {
     const SC volume = grep.GetCellVolume();
     auto functor = [&](auto mb) {
         using Gradient = Matrix::Gradient<Grid>;
-        using Divergence = Matrix::Divergence<Grid, Matrix::EULER_BACKWARD>;
+        using Divergence = Matrix::Divergence<Grid, SCHEME>;
         using TVD = Matrix::TVD<Grid, SC, Matrix::VANLEER>;
         using DDT = Matrix::DDT<Grid>;
         auto g_r = mb->GetRepresentation();
@@ -410,7 +413,10 @@ static void General3D() {
             t.bc_type[k] = bc[k];
             t.bc_value[k] = bv[k];
         }
-        expect_equal(host, oracle(g, t, host_values(phi.GetDataVector())), "general 3-D vs oracle");
+        const auto w = SCHEME::template GetWeights<double>();
+        t.div_levels = static_cast<int32_t>(w.size());
+        for (std::size_t l = 0; l < w.size(); l++) t.div_weights[l] = w[l];
+        expect_equal(host, oracle(g, t, host_values(phi.GetDataVector())), (std::string(name) + " vs oracle").c_str());
     } else {
         auto dev = Device::Default();
         const dare_grid g = ux.Descriptor();
@@ -421,12 +427,20 @@ static void General3D() {
         Matrix::Terms terms;
         terms.DDT(dt).DiffusionField(D.GetDeviceView()).TVD(DARE_LIMITER_VANLEER, f0.data(), f1.data(), f2.data());
         terms.Source(src.GetDeviceView());
+        terms.template DivergenceScheme<SCHEME>();
         for (int k = 0; k < 6; k++) {
             if (bc[k] == DARE_BC_DIRICHLET) terms.Dirichlet(k, bv[k]);
             else terms.Neumann(k);
         }
-        expect_equal(host, device(grep, phi.GetDataVector(), terms), "general 3-D vs device");
+        expect_equal(host, device(grep, phi.GetDataVector(), terms), (std::string(name) + " vs device").c_str());
     }
+}
+static void General3D() {
+    General3DWith<Matrix::EULER_BACKWARD>("general 3-D");
+    General3DWith<Matrix::CRANK_NICHOLSON>("general 3-D, CRANK_NICHOLSON divergences");
+    General3DWith<Matrix::EULER_FORWARD>("general 3-D, EULER_FORWARD divergences");
+    General3DWith<Matrix::ADAMS_BASHFORT>("general 3-D, ADAMS_BASHFORT divergences");
+    General3DWith<Matrix::ADAMS_MOULTON>("general 3-D, ADAMS_MOULTON divergences");
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -158,10 +158,20 @@ def test_interpolate_to_faces_and_tvd_with_cell_velocity(ctx, rng, case):
 
 
 ASSEMBLY = ["diff_dirichlet", "var_conv_src", "periodic_conv", "react2", "neumann_2d", "steady_1d", "fixed_x",
-            "fixed_react2"]
+            "fixed_react2", "cn_var_conv_src", "ab_periodic_conv", "am_vardiff", "ef_fixed_x"]
+# the divergence time schemes (Divergence<Grid, TimeScheme>): a base case under another scheme
+SCHEME_CASES = {"cn_var_conv_src": ("var_conv_src", da.CRANK_NICHOLSON),
+                "ab_periodic_conv": ("periodic_conv", da.ADAMS_BASHFORT),
+                "am_vardiff": ("vardiff_sym", da.ADAMS_MOULTON),
+                "ef_fixed_x": ("fixed_x", da.EULER_FORWARD)}
 
 
 def assembly_case(rng, name):
+    if name in SCHEME_CASES:
+        base, scheme = SCHEME_CASES[name]
+        go, kw, arrays, phi = assembly_case(rng, base)
+        kw["div_scheme"] = scheme
+        return go, kw, arrays, phi
     if name == "diff_dirichlet":
         go = orc.grid([10, 11, 12], size=[1, 1, 1], ghost=2)
         kw = dict(dt=1e-2, diff=1e-3, bc_type=[1] * 6, bc_value=[0.0] * 6)
@@ -348,7 +358,8 @@ def test_mixed_refinement_reaches_fp64_residual(ctx, rng, name):
 
 
 DIA_CASES = ["diff_dirichlet", "var_conv_src", "periodic_conv", "periodic_even", "periodic_sym", "periodic_single",
-             "wide_dirichlet", "react2", "fixed_x", "fixed_react2", "vardiff_sym", "periodic_vardiff", "fixed_diff"]
+             "wide_dirichlet", "react2", "fixed_x", "fixed_react2", "vardiff_sym", "periodic_vardiff", "fixed_diff",
+             "cn_var_conv_src", "am_vardiff"]
 
 
 def device_assembly(ctx, rng, name):
@@ -390,7 +401,7 @@ def test_diagonal_view_spmv_matches_oracle(ctx, rng, name):
     A.close()
 
 
-CONVECTION_CASES = {"var_conv_src", "periodic_conv", "periodic_even", "fixed_x"}
+CONVECTION_CASES = {"var_conv_src", "periodic_conv", "periodic_even", "fixed_x", "cn_var_conv_src"}
 
 
 def csr_bitwise_symmetric(rp, col, val):

@@ -18,6 +18,8 @@ Each check compares the oracle (oracle/oracle.cpp) with a result the mathematics
                                   is a + b rho^i, rho = 1 + Pe (u > 0) or 1 / (1 - Pe) (u < 0)
   ghost modes 2-4                 brute force over every padded cell, plus the face properties
                                   (Dirichlet: mean of ghost and cell = v; Neumann: zero jump)
+  divergence time schemes         coefficients scaled by w0 (exactly half under Crank-Nicolson),
+                                  the TVD right-hand side by 1 + sum of the explicit weights
 
 test_pins_catch_planted_mistakes compiles mutants of oracle.cpp, each with one plausible
 mistake planted (on the CPU), and requires the pin aimed at it to fail.
@@ -355,6 +357,52 @@ def pin_ghost_modes():
     return err
 
 
+def pin_time_scheme():
+    """Divergence<Grid, TimeScheme> in matrix form reuses the implicit level's face stencil for
+    the explicit levels (Divergence_Cartesian.inl:58-66,183-204). Against the same operator
+    under EULER_BACKWARD: every diffusion and convection coefficient is scaled by w0
+    (CRANK_NICHOLSON's off-diagonals are exactly half of EULER_BACKWARD's, 0.5 being a power of
+    two; EULER_FORWARD's and ADAMS_BASHFORT's vanish), and on a periodic box the convection's
+    right-hand side (the TVD deferred correction) is scaled by 1 + sum_{t>=1} w_t (x1.5 for
+    CRANK_NICHOLSON, x2 for EULER_FORWARD and ADAMS_BASHFORT, x19/12 for ADAMS_MOULTON) while
+    the DDT part is untouched. Returns the largest relative error."""
+    rng = np.random.default_rng(41)
+    schemes = {"CN": (0.5, 0.5), "EF": (0.0, 1.0), "AB": (0.0, 1.5, -0.5), "AM": (5 / 12, 8 / 12, -1 / 12)}
+    err = 0.0
+    # diffusion (per-cell diffusivity, Dirichlet / Neumann faces), no DDT
+    g = orc.grid([6, 5, 7], size=[1.0, 0.8, 1.2], ghost=2)
+    npad = int(np.prod(padded_shape(g)))
+    phi = rng.normal(size=npad)
+    D = np.exp(rng.normal(np.log(1e-3), 1.0, size=npad))
+    kw = dict(diff_cell=D, bc_type=[1, 2, 1, 1, 2, 1], bc_value=list(rng.normal(size=6)))
+    Ae = dense(*orc.assemble(g, orc.terms(**kw), phi)[:3])
+    off = ~np.eye(len(Ae), dtype=bool)
+    scale = float(np.max(np.abs(Ae[off])))
+    for name, w in schemes.items():
+        As = dense(*orc.assemble(g, orc.terms(div_scheme=w, **kw), phi)[:3])
+        if name == "CN":      # exact: a power-of-two scaling
+            err = max(err, float(np.max(np.abs(As[off] - 0.5 * Ae[off]))) / scale * 1e6)
+        err = max(err, float(np.max(np.abs(As[off] - w[0] * Ae[off]))) / scale)
+    # convection (VANLEER, random face velocities) + DDT on a periodic box
+    gp = orc.grid([6, 5, 7], size=[1.0, 0.8, 1.2], ghost=2, periodic=[1, 1, 1])
+    npad = int(np.prod(padded_shape(gp)))
+    phi = rng.normal(size=npad)
+    vel = [rng.uniform(-1, 1, size=nfaces(gp, d)) for d in range(3)]
+    ddt = orc.assemble(gp, orc.terms(dt=1e-2), phi)
+    e = orc.assemble(gp, orc.terms(dt=1e-2, limiter=4, vel=vel), phi)
+    conv_e = e[3] - ddt[3]
+    Ae = dense(*e[:3]) - dense(*ddt[:3])
+    off = ~np.eye(len(Ae), dtype=bool)
+    rscale, cscale = float(np.max(np.abs(conv_e))), float(np.max(np.abs(Ae[off])))
+    for name, w in schemes.items():
+        s_ = orc.assemble(gp, orc.terms(dt=1e-2, limiter=4, vel=vel, div_scheme=w), phi)
+        factor = 1.0 + sum(w[1:])
+        err = max(err, float(np.max(np.abs((s_[3] - ddt[3]) - factor * conv_e))) / rscale)
+        As = dense(*s_[:3]) - dense(*ddt[:3])
+        err = max(err, float(np.max(np.abs(As[off] - w[0] * Ae[off]))) / cscale)
+    return err
+
+
 PINS = {
     "harmonic_series": pin_harmonic_series,
     "reaction_block": pin_reaction_block,
@@ -365,6 +413,7 @@ PINS = {
     "tvd_matrix_identity": pin_tvd_matrix_identity,
     "convection_sign": pin_convection_sign,
     "ghost_modes": pin_ghost_modes,
+    "time_scheme": pin_time_scheme,
 }
 
 
@@ -444,8 +493,8 @@ MUTANTS = [
     ("upwind_upwind_cell", "UU->v[d] += vn - 2 * vp;", "UU->v[d] += vn - vp;", ["tvd_quadratic"]),
     ("explicit_step_sign", "phi_out[lp * G.N + c] = phi[lp * G.N + c] - fac * div_value(G, s, c);",
      "phi_out[lp * G.N + c] = phi[lp * G.N + c] + fac * div_value(G, s, c);", ["tvd_upwind_step"]),
-    ("convection_subtracted", "mb->add(div_matrix(G, tvd_matrix(G, t->conv_limiter, phi, p, uf), 1.));",
-     "mb->subtract(div_matrix(G, tvd_matrix(G, t->conv_limiter, phi, p, uf), 1.));",
+    ("convection_subtracted", "mb->add(div_matrix(G, tvd_matrix(G, t->conv_limiter, phi, p, uf), dw, dn));",
+     "mb->subtract(div_matrix(G, tvd_matrix(G, t->conv_limiter, phi, p, uf), dw, dn));",
      ["convection_sign", "tvd_matrix_identity"]),
     ("correction_sign", "s.rhs[c][f] = ((psi * -0.5) * (pD - pU)) * vel;", "s.rhs[c][f] = ((psi * 0.5) * (pD - pU)) * vel;",
      ["tvd_matrix_identity"]),
@@ -453,6 +502,10 @@ MUTANTS = [
      "else src.v[d] = G.g + (low ? l : G.n[d] - l - 1);", ["ghost_modes"]),
     ("dirichlet_ghost", "case 2: *dst = 2. * value[f] - s; break;", "case 2: *dst = value[f] - s; break;",
      ["ghost_modes"]),
+    ("scheme_level_unweighted", "o.rhs[c] += w[t] * G.A[0] * (s.rhs[c][1] - s.rhs[c][0]);",
+     "o.rhs[c] += G.A[0] * (s.rhs[c][1] - s.rhs[c][0]);", ["time_scheme"]),
+    ("scheme_coefficients_unscaled", "for (int q = 0; q < 7; q++) o.a[c][q] *= w0;",
+     "for (int q = 0; q < 7; q++) o.a[c][q] *= 1.;", ["time_scheme"]),
 ]
 
 
