@@ -7,7 +7,8 @@ BiCGStab2 to relative residual 1e-8 + copy-back. Inputs are resident in HBM befo
 
 Prints ONE JSON line. `value` is the implicit solve time (assembly + Krylov solve to 1e-8)
 per step of the fastest form; the plain FP64 form and the mixed-precision form are both
-reported. `roofline` is measured live on the dominant kernel (fused SpMV+dot, FP64 CSR) with
+reported. `roofline` is measured live on the dominant kernel (fused SpMV+dot over the FP64
+diagonal view of the CSR matrix) with
 HIP events on the stream the kernel runs on. `cpu_baseline` is the oracle (a C++ restatement
 of the reference algorithm) run on the host cores for one full 256^3 implicit step.
 
@@ -369,16 +370,20 @@ def target_leg(ctx, n, steps, inner_tol=1e-4):
     st = {f[0]: {"ms": [], "passes": [], "refine": [], "rel": [], "guess_ms": [], "rel0": []} for f in forms}
     xg = torch.empty_like(W.x)
     for k in range(steps + 1):
-        W.assemble()
+        W.assemble(base=False)
         ctx.synchronize()
         x_prev = W.x.clone()
         keep = None
         for name, mixed, pre, guess in forms:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
+            # a guess is timed with what it needs: the extrapolation, and for the network also
+            # the residual of the extrapolation that it reads
             if guess == "learned":
+                W.base_guess()
                 net.predict(W.grid, W.phi, W.phi_prev, W.xb, W.r0, W.rhs_scale, xg, extras=W.extras)
             elif guess == "extrapolated":
+                W.base_guess(with_residual=False)
                 xg.copy_(W.xb)
             else:
                 xg.copy_(x_prev)

@@ -11,6 +11,7 @@
 // every step without touching the structure.
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -432,8 +433,10 @@ __global__ __launch_bounds__(256) void k_fill_values(Geo G, Terms T, long long n
 
 // Diagonal-view fill (3-D): one lane per cell, lanes along k, no column sort. Writes the slot
 // values (coalesced, slot-major), rhs and x0. The CSR values are materialized from the slots
-// only when a CSR consumer asks for them (k_materialize_csr, same sort and merge).
-template <bool NT, bool F32, int FEAT>
+// only when a CSR consumer asks for them (k_materialize_csr, same sort and merge). HALF (an
+// operator symmetric by construction): only W, S, B, C are written; E, N, T equal the W, S, B
+// of the east, north and top neighbours, bit for bit (k_dia_mirror writes them on demand).
+template <bool NT, bool F32, int FEAT, bool HALF>
 __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __restrict__ dia, float* __restrict__ dia32,
                                                   int slots, double* __restrict__ rhs, double* __restrict__ x0) {
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -448,7 +451,13 @@ __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __rest
         double slotv[kDiaSlots];
         double r = 0., x = 0.;
         row_entries<kRowValues, false, 3, FEAT>(G, T, q, c, node, nullptr, nullptr, &r, &x, slotv);
-        if constexpr ((FEAT & kFeatCross) == 0) {      // seven slots
+        if constexpr (HALF) {                           // W, S, B, C
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
+            if constexpr (F32)
+#pragma unroll
+                for (int sl = 0; sl < 4; sl++) put<NT>(dia32 + sl * nrows + row, static_cast<float>(slotv[sl]));
+        } else if constexpr ((FEAT & kFeatCross) == 0) {      // seven slots
 #pragma unroll
             for (int sl = 0; sl < 7; sl++) put<NT>(dia + sl * nrows + row, slotv[sl]);
             if constexpr (F32)
@@ -462,6 +471,29 @@ __global__ __launch_bounds__(256) void k_fill_dia(Geo G, Terms T, double* __rest
         put<NT>(rhs + row, r);
         if (x0) put<NT>(x0 + row, x);
     }
+}
+
+// E, N, T of every row of a one-component view from the W, S, B slots of its east, north and
+// top neighbours (periodic wraps as in the traversals); entries a boundary edit removed get 0
+// (never read). Exact for a view whose fill left them out (k_fill_dia<..., HALF>).
+template <typename VT>
+__global__ __launch_bounds__(256) void k_dia_mirror(DiaView V, VT* __restrict__ D) {
+    const long long r = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= V.nrows) return;
+    const int n0 = V.n[0], n1 = V.n[1], n2 = V.n[2];
+    const long long sI = static_cast<long long>(n1) * n2, sJ = n2, nr = V.nrows;
+    const int k = static_cast<int>(r % n2);
+    const long long t = r / n2;
+    const int j = static_cast<int>(t % n1), i = static_cast<int>(t / n1);
+    const bool hasE = !(i == n0 - 1 && V.rm_hi[0]);
+    const bool hasN = !(j == n1 - 1 && V.rm_hi[1]);
+    const bool hasT = !(k == n2 - 1 && V.rm_hi[2]);
+    const long long east = (i == n0 - 1) ? r - static_cast<long long>(n0 - 1) * sI : r + sI;
+    const long long north = (j == n1 - 1) ? r - static_cast<long long>(n1 - 1) * sJ : r + sJ;
+    const long long top = (k == n2 - 1) ? r - (n2 - 1) : r + 1;
+    D[4 * nr + r] = hasT ? D[2 * nr + top] : VT(0);
+    D[5 * nr + r] = hasN ? D[1 * nr + north] : VT(0);
+    D[6 * nr + r] = hasE ? D[0 * nr + east] : VT(0);
 }
 
 __global__ __launch_bounds__(256) void k_materialize_csr(Geo G, Terms T, long long nrows,
@@ -543,10 +575,24 @@ int make_terms(const Geo& G, const dare_terms* t, const double* phi, Terms* T) {
 
 }  // namespace
 
+int dia_ensure_full(dare_csr* A, hipStream_t stream) {
+    if (!A->ent_lazy) return DARE_OK;
+    const unsigned grid = static_cast<unsigned>(ceil_div(A->nrows, 256));
+    hipLaunchKernelGGL(k_dia_mirror<double>, dim3(grid), dim3(256), 0, stream, A->dia_view, A->dia);
+    if (A->dia32 && A->dia32_fresh)
+        hipLaunchKernelGGL(k_dia_mirror<float>, dim3(grid), dim3(256), 0, stream, A->dia_view, A->dia32);
+    DARE_LAUNCH_CHECK();
+    A->ent_lazy = 0;
+    A->ent_full = 1;
+    return DARE_OK;
+}
+
 int csr_ensure_values(dare_csr* A, hipStream_t stream) {
     if (!A->vals_stale) return DARE_OK;
+    int rc = dia_ensure_full(A, stream);
+    if (rc) return rc;
     Geo G;
-    int rc = make_geo(&A->grid_desc, &G);
+    rc = make_geo(&A->grid_desc, &G);
     if (rc) return rc;
     Terms T;
     std::memset(&T, 0, sizeof(T));
@@ -714,6 +760,7 @@ int dare_csr_build_structure(dare_ctx* ctx, dare_csr* A, const dare_grid* g, con
             return DARE_ERR_OOM;
         }
         A->sym64 = -1;
+        A->ent_lazy = 0;
         A->has_dia = 1;
     }
     return DARE_OK;
@@ -751,23 +798,6 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         // edits take the instantiation without the other terms' code
         bool plain = T.conv_limiter < 0 && !T.diff_cell && !T.source && !T.cross;
         for (int f = 0; f < 6; f++) plain = plain && T.bc_type[f] != DARE_BC_FIXED;
-#define DARE_FILL(NTV, F32V)                                                                                   \
-    do {                                                                                                       \
-        if (plain)                                                                                             \
-            hipLaunchKernelGGL((k_fill_dia<NTV, F32V, 0>), grid, dim3(256), 0, ctx->stream, G, T, A->dia,      \
-                               F32V ? A->dia32 : nullptr, slots, rhs, x0);                                     \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_fill_dia<NTV, F32V, kFeatAll>), grid, dim3(256), 0, ctx->stream, G, T,       \
-                               A->dia, F32V ? A->dia32 : nullptr, slots, rhs, x0);                             \
-    } while (0)
-        if (nt && f32) DARE_FILL(true, true);
-        else if (nt) DARE_FILL(true, false);
-        else if (f32) DARE_FILL(false, true);
-        else DARE_FILL(false, false);
-#undef DARE_FILL
-        DARE_LAUNCH_CHECK();
-        A->dia32_fresh = f32 ? 1 : 0;
-        A->vals_stale = 1;
         // Symmetric by construction: a face coefficient is A_d * (dnr_d * D_f) in both rows it
         // couples (with the signs exchanged twice), and the per-cell harmonic mean takes its two
         // diffusivities in face order (low cell first) from either side. Not claimed for fixed
@@ -778,6 +808,37 @@ int dare_csr_fill_values(dare_ctx* ctx, dare_csr* A, const dare_grid* g, const d
         bool sym = G.N == 1 && t->conv_limiter < 0 && t->react == nullptr;
         for (int f = 0; f < 6; f++) sym = sym && t->bc_type[f] != DARE_BC_FIXED;
         for (int d = 0; d < G.dim; d++) sym = sym && G.n[d] > 1 && !(t->diff_cell && G.per[d]);
+        // E, N, T are left out when the solvers will take the symmetric traversals (an even k
+        // extent for the FP64 pairs, a multiple of four for the FP32 quads) and no consumer of
+        // this matrix has needed the seven slots yet
+        const bool half = sym && !A->cross && A->sym_traversal && !A->ent_full && G.n[2] % 2 == 0 &&
+                          (!f32 || G.n[2] % 4 == 0);
+        auto launch = [&](auto ntc, auto f32c, auto halfc) {
+            constexpr bool NTV = decltype(ntc)::value, F32V = decltype(f32c)::value, HV = decltype(halfc)::value;
+            if (plain)
+                hipLaunchKernelGGL((k_fill_dia<NTV, F32V, 0, HV>), grid, dim3(256), 0, ctx->stream, G, T, A->dia,
+                                   F32V ? A->dia32 : nullptr, slots, rhs, x0);
+            else
+                hipLaunchKernelGGL((k_fill_dia<NTV, F32V, kFeatAll, HV>), grid, dim3(256), 0, ctx->stream, G, T,
+                                   A->dia, F32V ? A->dia32 : nullptr, slots, rhs, x0);
+        };
+        using Yes = std::true_type;
+        using No = std::false_type;
+        if (half) {
+            if (nt && f32) launch(Yes{}, Yes{}, Yes{});
+            else if (nt) launch(Yes{}, No{}, Yes{});
+            else if (f32) launch(No{}, Yes{}, Yes{});
+            else launch(No{}, No{}, Yes{});
+        } else {
+            if (nt && f32) launch(Yes{}, Yes{}, No{});
+            else if (nt) launch(Yes{}, No{}, No{});
+            else if (f32) launch(No{}, Yes{}, No{});
+            else launch(No{}, No{}, No{});
+        }
+        DARE_LAUNCH_CHECK();
+        A->dia32_fresh = f32 ? 1 : 0;
+        A->vals_stale = 1;
+        A->ent_lazy = half ? 1 : 0;
         A->sym_fill = sym ? 1 : 0;
         // the cases just excluded may still give an exactly symmetric view: the solvers check
         // those on the device
@@ -818,6 +879,7 @@ int dare_csr_upload_host(dare_ctx* ctx, dare_csr* A, int64_t nrows, int64_t nnz,
     A->sym_fill = 0;
     A->sym_cand = 0;
     A->sym64 = -1;
+    A->ent_lazy = 0;
     return DARE_OK;
 }
 

@@ -610,6 +610,8 @@ int refresh_tables(dare_batch* B) {
     for (int s = 0; s < B->nsys; s++) {
         const dare_csr* A = B->sys[s];
         DARE_CHECK_ARG(A->has_dia && A->dia, "a batch system has no assembled values (call dare_csr_fill_values)");
+        DARE_CHECK_ARG(!A->ent_lazy, "a batch system's view lacks its E, N, T slots (its structure was rebuilt "
+                                     "after the batch was created)");
         float* d32 = B->dia32 ? B->dia32 + 8 * B->row0[s] : nullptr;
         if (B->h_sys[s].dia != A->dia || B->h_sys[s].dia32 != d32) {
             B->h_sys[s].dia = A->dia;
@@ -838,6 +840,13 @@ int dare_batch_create(dare_ctx* ctx, int32_t nsys, dare_csr* const* systems, dar
         DARE_CHECK_ARG(systems[s]->dia_view.N == N, "all systems of a batch need the same number of components");
     }
     DARE_HIP_TRY(hipSetDevice(ctx->device));
+    // the batched traversals read every slot: views a fill left without E, N, T get them now,
+    // and later fills of these matrices write them
+    for (int s = 0; s < nsys; s++) {
+        const int erc = dare_hip::dia_ensure_full(systems[s], ctx->stream);
+        if (erc) return erc;
+        systems[s]->ent_full = 1;
+    }
     dare_batch* B = new dare_batch();
     B->device = ctx->device;
     B->nsys = nsys;

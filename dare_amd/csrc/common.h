@@ -125,6 +125,13 @@ struct dare_csr {
     int* asym64 = nullptr;
     // 0: every traversal reads the seven slots (dare_csr_set_symmetric_traversal)
     int sym_traversal = 1;
+    // ent_lazy 1: the last fill left the E, N, T slots (4-6) of the view -- and of the FP32 copy
+    // while dia32_fresh -- unwritten: the operator is symmetric by construction and the symmetric
+    // traversals read those coefficients as the W, S, B slots of the east, north and top
+    // neighbours. dia_ensure_full writes them for any other consumer (seven-slot traversals,
+    // Jacobi scaling, multigrid, CSR values, batches) and sets ent_full, after which the fills
+    // of this matrix write every slot.
+    int ent_lazy = 0, ent_full = 0;
     // Jacobi right preconditioning: 1 / diag(A) and the column-scaled FP64 operator A diag(A)^-1
     // (diagonal-view slots, or CSR values for a matrix without a diagonal view)
     double* dinv = nullptr;
@@ -151,6 +158,9 @@ void mg_free(void* mg);
 
 // Writes the CSR values of a matrix whose last fill only wrote the diagonal view.
 int csr_ensure_values(dare_csr* A, hipStream_t stream);
+// the E, N, T slots of a view the last fill left unwritten (ent_lazy), from the neighbours'
+// W, S, B slots, on `stream`; no work when they are present
+int dia_ensure_full(dare_csr* A, hipStream_t stream);
 
 inline unsigned int ceil_div(long long a, long long b) { return static_cast<unsigned int>((a + b - 1) / b); }
 

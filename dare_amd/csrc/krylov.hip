@@ -406,6 +406,8 @@ int mg_prepare(dare_ctx* ctx, dare_csr* A, const SolveOp<VT>& op, const VT* D0, 
         set_error("the multigrid preconditioner needs a device-assembled 3-D operator (diagonal view)");
         return DARE_ERR_UNSUPPORTED;
     }
+    const int erc = dia_ensure_full(A, ctx->stream);     // the levels read all seven slots
+    if (erc) return erc;
     if (!A->mg) A->mg = new MgHierarchy();
     MgHierarchy* H = static_cast<MgHierarchy*>(A->mg);
     const int rc = mg_setup<VT, XT>(ctx, H, A->dia_view, D0);
@@ -453,6 +455,8 @@ static int op64(dare_ctx* ctx, const dare_csr* A, bool allow_dia, SolveOp<double
         if (rc) return rc;
         op.d.sym = A->sym64 == 1;
     }
+    // any traversal but the symmetric one reads E, N, T (or the CSR values made from them)
+    if (!(op.dia && op.d.sym)) return dia_ensure_full(const_cast<dare_csr*>(A), ctx->stream);
     return DARE_OK;
 }
 
@@ -651,6 +655,10 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     rc = op64(ctx, A, true, &op);
     if (rc) return rc;
     const bool pre = p->precond == 1;
+    if (pre || left) {          // the scaled views are made from all seven slots
+        rc = dia_ensure_full(const_cast<dare_csr*>(A), ctx->stream);
+        if (rc) return rc;
+    }
     MgHierarchy* mg = nullptr;
     if (use_mg) {
         rc = mg_prepare<double, double>(ctx, const_cast<dare_csr*>(A), op, A->dia, &mg);
@@ -795,6 +803,10 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     const bool pre = p->precond == 1;
     const bool use_mg = p->precond == DARE_PRECOND_MG;
     if (A->has_dia && kXcd * make_dia<double>(A->dia_view, A->dia).per_xcd <= kReduceBlocks) {
+        if (pre || use_mg) {    // the scaled FP32 view and the multigrid levels read all seven slots
+            const int erc = dia_ensure_full(A, ctx->stream);
+            if (erc) return erc;
+        }
         const int slots = A->dia_view.cross ? 8 : 7;
         const int64_t need = static_cast<int64_t>(slots) * n;
         if (A->cap_dia32 < need) {
@@ -833,6 +845,10 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
             const int src = sym64_prepare(ctx, A);
             if (src) return src;
             op32.d.sym = A->sym64 == 1;
+        }
+        if (!op32.d.sym) {      // the seven-slot FP32 traversal
+            const int erc = dia_ensure_full(A, ctx->stream);
+            if (erc) return erc;
         }
     } else {
         int erc = csr_ensure_values(A, ctx->stream);

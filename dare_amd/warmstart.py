@@ -105,19 +105,23 @@ def evaluate(ctx, net, n=128, eval_seeds=range(5000, 5020), steps=200, tol=TOL, 
             xg = torch.empty_like(W.x)
             rbuf = torch.empty_like(W.x)
         for k in range(steps):
-            W.assemble()
+            W.assemble(base=False)
             x_prev = W.x.clone()                    # phi^n in solver order (fill wrote x0 = phi^n)
             keep = None
             for m in modes:
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
+                # each guess is timed with everything it needs: the extrapolation xb, and for the
+                # network also the residual r0 = b - A xb it reads
                 if m == "zero":
                     xg.zero_()
                 elif m == "previous":
                     xg.copy_(x_prev)
                 elif m == "extrapolated":
-                    xg.copy_(W.xb)                  # 2 phi^n - phi^{n-1} (assemble computed it)
+                    W.base_guess(with_residual=False)
+                    xg.copy_(W.xb)                  # 2 phi^n - phi^{n-1}
                 else:
+                    W.base_guess()
                     net.predict(W.grid, W.phi, W.phi_prev, W.xb, W.r0, W.rhs_scale, xg, extras=W.extras)
                 torch.cuda.synchronize()
                 guess_s = time.perf_counter() - t0

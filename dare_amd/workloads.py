@@ -146,14 +146,23 @@ class C4Batch:
             A.close()
 
 
-def _base_guess(W):
-    """xb = 2 phi^n - phi^{n-1} in solver rows (x holds phi^n after the fill) and r0 = rhs - A xb."""
+def _base_guess(W, with_residual=True):
+    """xb = 2 phi^n - phi^{n-1} in solver rows (x holds phi^n after the fill) and, with
+    with_residual, r0 = rhs - A xb."""
     from . import residual
     g = W.grid.ghost
     n = W.n
     prev_int = W.phi_prev.view(n + 2 * g, n + 2 * g, n + 2 * g)[g:-g, g:-g, g:-g].reshape(-1)
     _torch().sub(W.x * 2.0, prev_int, out=W.xb)
-    residual(W.ctx, W.A, W.rhs, W.xb, W.r0)
+    if with_residual:
+        residual(W.ctx, W.A, W.rhs, W.xb, W.r0)
+
+
+class _BaseGuess:
+    def base_guess(self, with_residual=True):
+        """the extrapolated guess xb (and its residual r0) of the assembled step; callers that
+        time a guess call this inside the timed region (assemble(base=False) skips it)"""
+        _base_guess(self, with_residual)
 
 
 def gaussian_sum(n, rng, count=8, ghost=2, device="cuda"):
@@ -172,7 +181,7 @@ def gaussian_sum(n, rng, count=8, ghost=2, device="cuda"):
     return phi.contiguous().view(-1)
 
 
-class C5AdvDiff:
+class C5AdvDiff(_BaseGuess):
     """Transient advection-diffusion trajectory on an n^3 unit cube (learned-form workload).
 
     Per seed: D ~ U[1e-4, 1e-2], constant velocity u ~ U[-1, 1]^3 on every face, initial field
@@ -219,14 +228,15 @@ class C5AdvDiff:
         self.extras = []
         self.in_channels = 3
 
-    def assemble(self):
-        """ghost fill, then values / rhs / x0 = phi^n of the next implicit step; also the
-        extrapolated guess xb = 2 phi^n - phi^{n-1} and its residual r0 = rhs - A xb (the
+    def assemble(self, base=True):
+        """ghost fill, then values / rhs / x0 = phi^n of the next implicit step; with base, also
+        the extrapolated guess xb = 2 phi^n - phi^{n-1} and its residual r0 = rhs - A xb (the
         warm-start network's base guess and input)"""
         from . import fill_values, ghost_fill
         ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
         fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
-        _base_guess(self)
+        if base:
+            _base_guess(self)
 
     def advance(self, x_solution):
         """phi^{n-1} <- phi^n, phi^n <- solution of the step"""
@@ -236,7 +246,7 @@ class C5AdvDiff:
         self.step_index += 1
 
 
-class VarDiffusion:
+class VarDiffusion(_BaseGuess):
     """Transient variable-diffusivity diffusion on an n^3 unit cube (the precision/learned target).
 
     Per seed: cell diffusivity ln D ~ N(ln 1e-3, 1) (face value: harmonic mean), source
@@ -276,11 +286,12 @@ class VarDiffusion:
         self.extras = [(self.lnD, 1.0, math.log(1e-3))]
         self.in_channels = 4
 
-    def assemble(self):
+    def assemble(self, base=True):
         from . import fill_values, ghost_fill
         ghost_fill(self.ctx, self.grid, self.phi, self.ghost_modes, self.ghost_values)
         fill_values(self.ctx, self.A, self.grid, self.terms, self.phi, self.rhs, self.x)
-        _base_guess(self)
+        if base:
+            _base_guess(self)
 
     def advance(self, x_solution):
         from . import copy_to_field

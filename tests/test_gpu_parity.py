@@ -462,6 +462,97 @@ def test_symmetric_traversal_is_bit_identical(ctx, rng, name):
     A.close()
 
 
+def _fill_sym_case(ctx, rng_seed, name, stale):
+    """A symmetric-by-construction case assembled on the device. stale: the matrix is first
+    filled with all slots from a different operator (diffusivity x3), then refilled with the
+    case's operator with the symmetric traversal on, which leaves the E, N, T slots out (they
+    keep the other operator's values). Otherwise the case is filled with every slot."""
+    import torch
+    go, kw, arrays, phi = assembly_case(np.random.default_rng(rng_seed), name)
+    g = grids(go)
+    darr = {k: dev(a) for k, a in arrays.items()}
+    td = da.Terms.make(**kw, **darr)
+    A = da.Csr(ctx)
+    da.build_structure(ctx, A, g, td)
+    n = orc.lib().orc_num_cells_internal(ctypes.byref(go)) * go.ncomp
+    rhs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    x0 = torch.zeros_like(rhs)
+    da.set_symmetric_traversal(A, False)            # every slot written
+    if stale:
+        kw2 = dict(kw)
+        arr2 = dict(darr)
+        if "diff_cell" in arr2:
+            arr2["diff_cell"] = arr2["diff_cell"] * 3.0
+        else:
+            kw2["diff"] = kw["diff"] * 3.0
+        da.fill_values(ctx, A, g, da.Terms.make(**kw2, **arr2), dev(phi), rhs, x0)
+        da.set_symmetric_traversal(A, True)         # the refill leaves E, N, T out
+        da.fill_values(ctx, A, g, td, dev(phi), rhs, x0)
+    else:
+        da.fill_values(ctx, A, g, td, dev(phi), rhs, x0)
+        da.set_symmetric_traversal(A, True)
+    ctx.synchronize()
+    return A, rhs, x0, (g, td, darr)
+
+
+LAZY_CONSUMERS = ["csr", "spmv_seven_slot", "jacobi", "jacobi_left", "mg", "mixed_jacobi", "mixed_mg", "batch"]
+
+
+@pytest.mark.parametrize("consumer", LAZY_CONSUMERS)
+@pytest.mark.parametrize("name", ["diff_dirichlet", "vardiff_sym", "periodic_sym"])
+def test_consumers_of_a_view_filled_without_east_north_top(ctx, name, consumer):
+    """An operator that is symmetric by construction is filled without its E, N, T slots (the
+    symmetric traversals read them as the neighbours' W, S, B slots). Every consumer that reads
+    the seven slots -- CSR values, the seven-slot traversal, Jacobi scaling on either side,
+    multigrid, the FP32 copy's scaled view, batches -- must see them written: the slots first
+    hold another operator's values, and each result is compared bit for bit with the same call
+    on the same operator filled with every slot."""
+    import torch
+    out = []
+    for stale in (True, False):
+        A, rhs, x0, keep = _fill_sym_case(ctx, 7, name, stale)
+        assert da.csr_symmetric(ctx, A)
+        n = rhs.numel()
+        if consumer == "csr":
+            got = A.download()[2]
+        elif consumer == "spmv_seven_slot":
+            da.set_symmetric_traversal(A, False)
+            x = dev(np.random.default_rng(3).normal(size=n))
+            y = torch.zeros_like(x)
+            da.spmv_dot(ctx, A, x, y, x)
+            got = host(y)
+        elif consumer in ("jacobi", "jacobi_left", "mg"):
+            pre = {"jacobi": 1, "jacobi_left": 3, "mg": 2}[consumer]
+            x = x0.clone()
+            r = da.bicgstab2(ctx, A, rhs, x, tol=1e-10, tol_kind=1, max_it=300, precond=pre)
+            assert r["converged"] == 1
+            got = (host(x), r["iterations"])
+        elif consumer in ("mixed_jacobi", "mixed_mg"):
+            x = x0.clone()
+            r = da.refine_mixed(ctx, A, rhs, x, tol=1e-10, tol_kind=1, inner_tol=1e-4,
+                                precond=1 if consumer == "mixed_jacobi" else 2)
+            assert r["converged"] == 1
+            got = (host(x), r["iterations"], r["refinements"])
+        else:
+            B = da.Batch(ctx, [A])
+            for src, addr in ((host(rhs), B.rhs_ptr(0)), (host(x0), B.x_ptr(0))):
+                da.check(da.lib().dare_memcpy_h2d(ctx.h, addr, src.ctypes.data, 8 * n), "h2d")
+            r = B.solve(tol=1e-10, tol_kind=1, max_it=300)[0]
+            assert r["converged"] == 1
+            x = np.empty(n)
+            da.check(da.lib().dare_memcpy_d2h(ctx.h, x.ctypes.data, B.x_ptr(0), 8 * n), "d2h")
+            ctx.synchronize()
+            got = (x, r["iterations"])
+            B.close()
+        out.append(got)
+        A.close()
+    a, b = out
+    if isinstance(a, tuple):
+        assert bits_equal(a[0], b[0]) and a[1:] == b[1:]
+    else:
+        assert bits_equal(a, b)
+
+
 @pytest.mark.parametrize("name", DIA_CASES)
 def test_solver_on_device_assembly_matches_oracle(ctx, rng, name):
     go, A, rhs, x0, (rp, col, val, orhs, ox0), keep = device_assembly(ctx, rng, name)
