@@ -140,8 +140,11 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
     XT* y = (mg && !x_zero) ? y_buf : x;       // the iterate the passes update
     const XT* rhv = x_zero ? b : rh;           // the shadow residual r0: b itself when x0 = 0
     auto pre = [&](const XT* in) -> const XT* { return mg ? mg_apply<VT, XT>(sm, *mg, in) : in; };
-    if (x_zero) {   // x = 0 written here: no operator sweep needed for r = b - A x
-        hipLaunchKernelGGL((k_init_zero<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, b, x, r, u, ws, st);
+    // x = 0: no operator sweep for r = b - A x, and no vector written before the first pass
+    // (which writes x and r); a solve that ends before it zeroes x itself (x_written)
+    bool x_written = !x_zero;
+    if (x_zero) {
+        hipLaunchKernelGGL((k_init_zero<XT>), dim3(gr), dim3(kThreads), 0, sm, n, b, ws, st);
         launch_finish<2>(sm, ws, gr, st, TAIL_INIT, 0);
     } else {
         A.launch(sm, x, EpiInit<XT, NT>{b, r, rh, u, v, w}, ws, st, TAIL_INIT, 0);     // (0)-(2)
@@ -153,6 +156,8 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
     if (rc) return rc;
     double conv = h.rnorm;
     if (conv < tol_abs) {
+        if (!x_written) hipLaunchKernelGGL((k_zero<XT>), dim3(gr), dim3(kThreads), 0, sm, n, x);
+        DARE_LAUNCH_CHECK();
         *iters = 0;
         *rnorm = conv;
         *converged = true;
@@ -166,9 +171,14 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
             *status = h.next_status;
             break;
         }
-        hipLaunchKernelGGL((k_upd_u<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, u, st);
+        // the first pass from a zero guess reads b for r and 0 for x, u (k_init_zero wrote none)
+        const bool first = x_zero && k == 1;
+        if (first) hipLaunchKernelGGL((k_upd_u<XT, NT, true>), dim3(gr), dim3(kThreads), 0, sm, n, b, u, st);
+        else hipLaunchKernelGGL((k_upd_u<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, u, st);
         A.launch(sm, pre(u), EpiDot<XT, NT>{v, rhv}, ws, st, TAIL_GAMMA, 1);
-        hipLaunchKernelGGL((k_upd_rx<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, y, u, st);
+        if (first) hipLaunchKernelGGL((k_upd_rx<XT, NT, true>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, y, u, st, b);
+        else hipLaunchKernelGGL((k_upd_rx<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, r, v, y, u, st, nullptr);
+        x_written = true;
         A.launch(sm, pre(r), EpiDot<XT, NT>{s, rhv}, ws, st, TAIL_RHO, 1);
         hipLaunchKernelGGL((k_upd_v<XT, NT>), dim3(gr), dim3(kThreads), 0, sm, n, s, v, st);
         A.launch(sm, pre(v), EpiDot<XT, NT>{w, rhv}, ws, st, TAIL_GAMMA, 1);
@@ -213,6 +223,7 @@ int bicgstab2_run_nt(dare_ctx* ctx, const SolveOp<VT>& A, const XT* b, XT* x, do
         if (conv <= tol_abs && k >= min_passes) break;   // reference: k > 1 (min_passes 2)
         ++k;
     }
+    if (!x_written) hipLaunchKernelGGL((k_zero<XT>), dim3(gr), dim3(kThreads), 0, sm, n, x);
     if (mg) {   // x = x0 + M^-1 y (x0 = 0: x holds y)
         const XT* z = pre(y);
         hipLaunchKernelGGL(k_mg_combine<XT>, dim3(gr), dim3(kThreads), 0, sm, n,

@@ -1054,7 +1054,9 @@ constexpr int kU = 4;
          base_ += (long long)gridDim.x * (kThreads * kU))
 #define DARE_IDX(u) (base_ + (long long)(u) * kThreads + threadIdx.x)
 
-template <typename XT, bool NT>
+// FIRST: the first pass from a zero guess (k_init_zero), where r = b and u = 0 were never
+// written: r is b itself, u reads as 0 (the same arithmetic, r - beta * 0).
+template <typename XT, bool NT, bool FIRST = false>
 __global__ __launch_bounds__(kThreads) void k_upd_u(long long n, const XT* __restrict__ r, XT* __restrict__ u,
                                                     const KState* __restrict__ st) {
     if (st->status) return;
@@ -1065,7 +1067,7 @@ __global__ __launch_bounds__(kThreads) void k_upd_u(long long n, const XT* __res
         for (int q = 0; q < kU; q++) {
             const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
             rv[q] = static_cast<double>(r[i]);
-            uv[q] = static_cast<double>(u[i]);
+            uv[q] = FIRST ? 0. : static_cast<double>(u[i]);
         }
 #pragma unroll
         for (int q = 0; q < kU; q++)
@@ -1073,20 +1075,31 @@ __global__ __launch_bounds__(kThreads) void k_upd_u(long long n, const XT* __res
     }
 }
 
-template <typename XT, bool NT>
+// FIRST: the first pass from a zero guess: the old r is b (read from b0) and the old x is 0,
+// both written here for the first time -- also when the pass has already broken down, so the
+// solve then returns x = 0, r = b as the zero-guess initialisation leaves them.
+template <typename XT, bool NT, bool FIRST = false>
 __global__ __launch_bounds__(kThreads) void k_upd_rx(long long n, XT* __restrict__ r, const XT* __restrict__ v,
                                                      XT* __restrict__ x, const XT* __restrict__ u,
-                                                     const KState* __restrict__ st) {
-    if (st->status) return;
+                                                     const KState* __restrict__ st, const XT* __restrict__ b0 = nullptr) {
+    if (st->status) {
+        if constexpr (FIRST) {
+            for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads) {
+                put<NT>(r + i, b0[i]);
+                put<NT>(x + i, XT(0));
+            }
+        }
+        return;
+    }
     const double a = st->alpha;
     DARE_CHUNK_LOOP(n) {
         double rv[kU], vv[kU], xv[kU], uv[kU];
 #pragma unroll
         for (int q = 0; q < kU; q++) {
             const long long i = DARE_IDX(q) < n ? DARE_IDX(q) : 0;
-            rv[q] = static_cast<double>(r[i]);
+            rv[q] = static_cast<double>(FIRST ? b0[i] : r[i]);
             vv[q] = static_cast<double>(v[i]);
-            xv[q] = static_cast<double>(x[i]);
+            xv[q] = FIRST ? 0. : static_cast<double>(x[i]);
             uv[q] = static_cast<double>(u[i]);
         }
 #pragma unroll
@@ -1213,13 +1226,13 @@ __global__ __launch_bounds__(kThreads) void k_dot(long long n, const XT* __restr
 }
 
 // Initialisation (0)-(2) for a zero initial guess: A x0 is +0 in every row, so r = b - A x0 = b
-// bit for bit; x = u = 0, r = b, and <r,r>, <rh,r> go to the reduction as EpiInit's do. rh = r0
-// is never written again, so the caller reads b itself as rh (b stays untouched during the
-// solve); v and w are written before they are read (EpiInit).
-template <typename XT, bool NT>
-__global__ __launch_bounds__(kThreads) void k_init_zero(long long n, const XT* __restrict__ b, XT* __restrict__ x,
-                                                        XT* __restrict__ r, XT* __restrict__ u, ReduceWs ws,
-                                                        KState* st) {
+// bit for bit, and <r,r>, <rh,r> go to the reduction as EpiInit's do. Nothing is written: the
+// first pass reads b where it would read r and 0 where it would read x or u
+// (k_upd_u / k_upd_rx <..., FIRST>), and writes them. rh = r0 is never written again, so the
+// caller reads b itself as rh (b stays untouched during the solve); v and w are written before
+// they are read (EpiInit).
+template <typename XT>
+__global__ __launch_bounds__(kThreads) void k_init_zero(long long n, const XT* __restrict__ b, ReduceWs ws, KState* st) {
     double acc[2] = {0., 0.};
     DARE_CHUNK_LOOP(n) {
         XT bv[kU];
@@ -1227,11 +1240,7 @@ __global__ __launch_bounds__(kThreads) void k_init_zero(long long n, const XT* _
         for (int q = 0; q < kU; q++) bv[q] = b[DARE_IDX(q) < n ? DARE_IDX(q) : 0];
 #pragma unroll
         for (int q = 0; q < kU; q++) {
-            const long long i = DARE_IDX(q);
-            if (i < n) {
-                put<NT>(x + i, XT(0));
-                put<NT>(r + i, bv[q]);
-                put<NT>(u + i, XT(0));
+            if (DARE_IDX(q) < n) {
                 const double rd = static_cast<double>(bv[q]);
                 acc[0] += rd * rd;
                 acc[1] += rd * rd;
