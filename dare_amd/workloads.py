@@ -152,8 +152,9 @@ def _base_guess(W, with_residual=True):
     from . import residual
     g = W.grid.ghost
     n = W.n
-    prev_int = W.phi_prev.view(n + 2 * g, n + 2 * g, n + 2 * g)[g:-g, g:-g, g:-g].reshape(-1)
-    _torch().sub(W.x * 2.0, prev_int, out=W.xb)
+    prev_int = W.phi_prev.view(n + 2 * g, n + 2 * g, n + 2 * g)[g:-g, g:-g, g:-g]
+    # one elementwise pass over x and the interior view of phi^{n-1}: lerp(p, x, 2) = x + (x - p)
+    _torch().lerp(prev_int, W.x.view(n, n, n), 2.0, out=W.xb.view(n, n, n))
     if with_residual:
         residual(W.ctx, W.A, W.rhs, W.xb, W.r0)
 

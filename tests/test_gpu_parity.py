@@ -576,6 +576,23 @@ def test_mixed_on_device_assembly(ctx, rng, name):
     A.close()
 
 
+def test_zero_guess_inner_solve_that_runs_no_pass_returns_zero(ctx, rng):
+    """The inner FP32 solves start from a zero guess whose initialisation writes no vector (the
+    first pass writes x and r). An inner solve that stops before its first pass (here: an inner
+    tolerance above 1, so the zero guess already meets it) must still hand back x = 0: after a
+    solve that left other values in the work vectors, refinements with such inner solves leave
+    the iterate bit for bit unchanged."""
+    go, A, rhs, x0, ref, keep = device_assembly(ctx, rng, "diff_dirichlet")
+    warm = x0.clone()
+    assert da.refine_mixed(ctx, A, rhs, warm, tol=1e-10, tol_kind=1, inner_tol=1e-4)["converged"] == 1
+    x = dev(rng.normal(size=x0.numel()))
+    before = host(x).copy()
+    r = da.refine_mixed(ctx, A, rhs, x, tol=1e-10, tol_kind=1, inner_tol=2.0, max_refine=3)
+    assert r["converged"] == 0 and r["refinements"] == 3 and r["iterations"] == 0
+    assert bits_equal(host(x), before)
+    A.close()
+
+
 def test_full_size_structure_and_solve_properties(ctx):
     """256^3 (the benchmark size), properties that do not need the oracle: nnz = 7n^3 - 6n^2,
     row lengths 4..7, sorted columns, SpMV of ones = row sums, and the FP64 solve's true
