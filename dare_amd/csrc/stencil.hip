@@ -587,10 +587,10 @@ __global__ __launch_bounds__(256) void k_grad3(Geo G, const double* __restrict__
     });
 }
 
-// k_grad3 with two k-adjacent cells per lane (n2 even, 16-byte aligned arrays): phi and the x
-// and y faces move as 16-byte vectors, the z faces too where the row starts on an even face
-// (z-face rows hold n2 + 1 faces). Each face is the same difference times the same factor as
-// in k_grad3, so the results are bit-identical.
+// k_grad3 with two k-adjacent cells per lane (n2 even, 16-byte aligned arrays): phi and the x,
+// y and z faces move as 16-byte vectors (z-face rows hold n2 + 1 faces, so a row starting on an
+// odd face pairs each lane's second face with the next lane's first). Each face is the same
+// difference times the same factor as in k_grad3, so the results are bit-identical.
 template <bool NT>
 __global__ __launch_bounds__(256) void k_grad3_pair(Geo G, const double* __restrict__ phi, double* __restrict__ fx,
                                                     double* __restrict__ fy, double* __restrict__ fz, int tiles_k,
@@ -623,13 +623,21 @@ __global__ __launch_bounds__(256) void k_grad3_pair(Geo G, const double* __restr
             st2(fy + oy, (p0 - pS) * d1);
             if (lastj) st2(fy + oy + n2, (pN - p0) * d1);
             const double z0 = (p0.x - pB) * dz2, z1 = (p0.y - p0.x) * dz2;
-            if ((oz & 1) == 0) {
+            if ((oz & 1) == 0) {              // wave-uniform: k is even in every lane
                 st2(fz + oz, d2{z0, z1});
+                if (lastk) put<NT>(fz + oz + 2, (pT - p0.y) * dz2);
             } else {
-                put<NT>(fz + oz, z0);
-                put<NT>(fz + oz + 1, z1);
+                // a row starting on an odd face: the aligned pairs are (k + 1, k + 2), face k + 2
+                // coming from the next lane (active whenever this one is not the row's last
+                // pair); lane 0 writes its face k alone, lane 63 its face k + 1 alone (the next
+                // tile's lane 0 writes k + 2)
+                const int lane = threadIdx.x & 63;
+                const double zn = __shfl_down(z0, 1);
+                if (lane == 0) put<NT>(fz + oz, z0);
+                if (lastk) st2(fz + oz + 1, d2{z1, (pT - p0.y) * dz2});
+                else if (lane < 63) st2(fz + oz + 1, d2{z1, zn});
+                else put<NT>(fz + oz + 1, z1);
             }
-            if (lastk) put<NT>(fz + oz + 2, (pT - p0.y) * dz2);
             pm = p0;
             lp += sI;
             ox += fxs;
