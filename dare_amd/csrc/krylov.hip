@@ -12,7 +12,9 @@
 // registers, tiles ordered so that each XCD (blocks b = x mod 8) owns one contiguous slab.
 // Matrix values then stream fully coalesced and x is read from HBM about once per SpMV.
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -451,7 +453,11 @@ static int sym64_prepare(dare_ctx* ctx, dare_csr* A) {
 
 // The FP64 operator of A: the diagonal view (its symmetric pair traversal when the view is
 // exactly symmetric) or the CSR values.
-static int op64(dare_ctx* ctx, const dare_csr* A, bool allow_dia, SolveOp<double>* out) {
+// vecs: the caller's vectors the traversal reads or writes. The paired, quad and two-component
+// traversals move 16-byte vectors, so those must start on 16-byte boundaries (any allocation
+// does; a slice at an odd row offset does not).
+static int op64(dare_ctx* ctx, const dare_csr* A, bool allow_dia, SolveOp<double>* out,
+                std::initializer_list<const void*> vecs = {}) {
     SolveOp<double>& op = *out;
     op = SolveOp<double>{};
     op.n = A->nrows;
@@ -466,6 +472,11 @@ static int op64(dare_ctx* ctx, const dare_csr* A, bool allow_dia, SolveOp<double
         if (rc) return rc;
         op.d.sym = A->sym64 == 1;
     }
+    if (op.dia && (op.d.pair > 1 || op.d.V.N == 2))
+        for (const void* v : vecs)
+            DARE_CHECK_ARG((reinterpret_cast<uintptr_t>(v) & 15) == 0,
+                           "vectors must start on a 16-byte boundary (the diagonal-view traversals move 16-byte "
+                           "vectors)");
     // any traversal but the symmetric one reads E, N, T (or the CSR values made from them)
     if (!(op.dia && op.d.sym)) return dia_ensure_full(const_cast<dare_csr*>(A), ctx->stream);
     return DARE_OK;
@@ -547,7 +558,7 @@ int dare_spmv(dare_ctx* ctx, const dare_csr* A, const double* x, double* y) {
     // (80 B/row, no CSR values to materialize) with bit-identical results; otherwise the CSR.
     const bool wrap = A->has_dia && (A->dia_view.per[0] || A->dia_view.per[1] || A->dia_view.per[2]);
     SolveOp<double> op;
-    int rc = op64(ctx, A, !wrap, &op);
+    int rc = op64(ctx, A, !wrap, &op, {x, y});
     if (rc) return rc;
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
@@ -569,7 +580,7 @@ int dare_residual(dare_ctx* ctx, const dare_csr* A, const double* b, const doubl
     int rc = ws_reserve(ctx, 0, &W);
     if (rc) return rc;
     SolveOp<double> op;
-    rc = op64(ctx, A, true, &op);
+    rc = op64(ctx, A, true, &op, {b, x, r});
     if (rc) return rc;
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
@@ -601,7 +612,7 @@ int dare_spmv_dot(dare_ctx* ctx, const dare_csr* A, const double* x, double* y, 
     hipLaunchKernelGGL(k_clear_state, dim3(1), dim3(64), 0, ctx->stream, W.st);
     // the operator the solver uses (diagonal view when the matrix has one)
     SolveOp<double> op;
-    rc = op64(ctx, A, true, &op);
+    rc = op64(ctx, A, true, &op, {x, y, w});
     if (rc) return rc;
     if (A->nrows >= kStreamRows) op.launch(ctx->stream, x, EpiDot<double, true>{y, w}, ctx->ws, W.st, TAIL_STORE, 1);
     else op.launch(ctx->stream, x, EpiDot<double, false>{y, w}, ctx->ws, W.st, TAIL_STORE, 1);
@@ -663,7 +674,7 @@ int dare_bicgstab2_solve(dare_ctx* ctx, const dare_csr* A, const double* b, doub
     const double bnorm = std::sqrt(bnorm2);
     const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
     SolveOp<double> op;
-    rc = op64(ctx, A, true, &op);
+    rc = op64(ctx, A, true, &op, {b, x});
     if (rc) return rc;
     const bool pre = p->precond == 1;
     if (pre || left) {          // the scaled views are made from all seven slots
@@ -779,7 +790,7 @@ int dare_bicgstab2_solve_noalloc(dare_ctx* ctx, const dare_csr* A, const double*
     const double bnorm = std::sqrt(bnorm2);
     const double tol_abs = p->tol_kind == 1 ? p->tol * bnorm : p->tol;
     SolveOp<double> op;
-    rc = op64(ctx, A, true, &op);
+    rc = op64(ctx, A, true, &op, {b, x});
     if (rc) return rc;
     if (!op.dia) {
         rc = csr_ensure_values(const_cast<dare_csr*>(A), ctx->stream);
@@ -902,7 +913,7 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
     // ||b|| comes out of the first residual sweep (EpiResidB), which reads b anyway
     double bnorm = 0., tol_abs = 0.;
     SolveOp<double> op;
-    rc = op64(ctx, A, true, &op);
+    rc = op64(ctx, A, true, &op, {b, x});
     if (rc) return rc;
     if (!op.dia) {
         rc = csr_ensure_values(A, ctx->stream);

@@ -593,22 +593,52 @@ __device__ inline void dia_column(const DiaView& V, const VT* __restrict__ D, co
         const bool hasE = !(i == n0 - 1 && V.rm_hi[0] && !(i == 0 && V.rm_lo[0]));
         const long long nnext = !hasE ? node : (i == n0 - 1) ? node - (long long)(n0 - 1) * sI : node + sI;
         double xn[NC], xs[NC], xnn[NC], xb[NC], xt[NC], dv[NC][8], pre[NC];
+        if constexpr (NC == 2) {
+            // the two component rows of a cell are adjacent (row = 2 node + c, nrows even, the
+            // arrays start on 2-element boundaries): one 2-wide load per slot and per x entry
+            typedef VT v2 __attribute__((ext_vector_type(2)));
+            typedef XT x2 __attribute__((ext_vector_type(2)));
+            auto ldx = [&](long long nd, double* o) {
+                const x2 v = *reinterpret_cast<const x2*>(x + nd * 2);
+                o[0] = static_cast<double>(v.x);
+                o[1] = static_cast<double>(v.y);
+            };
+            ldx(nnext, xn);
+            ldx(node + dS, xs);
+            ldx(node + dN, xnn);
+            ldx(node + dB, xb);
+            ldx(node + dT, xt);
+            const int nsl = V.cross ? 8 : 7;
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-            const long long row = node * NC + c;
-            xn[c] = static_cast<double>(x[nnext * NC + c]);
-            xs[c] = static_cast<double>(x[(node + dS) * NC + c]);
-            xnn[c] = static_cast<double>(x[(node + dN) * NC + c]);
-            xb[c] = static_cast<double>(x[(node + dB) * NC + c]);
-            xt[c] = static_cast<double>(x[(node + dT) * NC + c]);
-#pragma unroll
-            for (int sl = 0; sl < 7; sl++) {
-                if constexpr (Epi::kNT) dv[c][sl] = static_cast<double>(ld_nt(D + sl * nr + row));
-                else dv[c][sl] = static_cast<double>(D[sl * nr + row]);
+            for (int sl = 0; sl < 8; sl++) {
+                v2 d = v2{VT(0), VT(0)};
+                if (sl < nsl) {
+                    const v2* p = reinterpret_cast<const v2*>(D + sl * nr + node * 2);
+                    if constexpr (Epi::kNT) d = __builtin_nontemporal_load(p);
+                    else d = *p;
+                }
+                dv[0][sl] = static_cast<double>(d.x);
+                dv[1][sl] = static_cast<double>(d.y);
             }
-            if constexpr (Epi::kNT) dv[c][7] = (NC == 2 && V.cross) ? static_cast<double>(ld_nt(D + 7 * nr + row)) : 0.;
-            else dv[c][7] = (NC == 2 && V.cross) ? static_cast<double>(D[7 * nr + row]) : 0.;
-            pre[c] = epi.load(row0 + row);
+#pragma unroll
+            for (int c = 0; c < NC; c++) pre[c] = epi.load(row0 + node * NC + c);
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const long long row = node * NC + c;
+                xn[c] = static_cast<double>(x[nnext * NC + c]);
+                xs[c] = static_cast<double>(x[(node + dS) * NC + c]);
+                xnn[c] = static_cast<double>(x[(node + dN) * NC + c]);
+                xb[c] = static_cast<double>(x[(node + dB) * NC + c]);
+                xt[c] = static_cast<double>(x[(node + dT) * NC + c]);
+#pragma unroll
+                for (int sl = 0; sl < 7; sl++) {
+                    if constexpr (Epi::kNT) dv[c][sl] = static_cast<double>(ld_nt(D + sl * nr + row));
+                    else dv[c][sl] = static_cast<double>(D[sl * nr + row]);
+                }
+                dv[c][7] = 0.;
+                pre[c] = epi.load(row0 + row);
+            }
         }
         // straight-line per component (no loop around the epilogue's stores: LLVM's loop
         // invariant code motion promotes a store out of a one-trip loop and drops its

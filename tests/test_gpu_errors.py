@@ -79,6 +79,13 @@ def test_invalid_arguments_are_refused_and_the_context_survives(ctx):
     da.fill_values(ctx, A, g, t, phi, rhs, x)
     _raises(lambda: da.bicgstab2(ctx, A, rhs, x, precond=4), "precond must be 0 (none), 1 (Jacobi), 2 (multigrid) or 3")
     _raises(lambda: da.refine_mixed(ctx, A, rhs, x, precond=-1), "precond must be 0 (none), 1 (Jacobi) or 2")
+    # a vector 8 bytes off a 16-byte boundary (a slice at an odd row): the paired traversal of
+    # this even-k grid moves 16-byte vectors, so it is refused before any kernel reads it
+    x_odd = torch.zeros(121, dtype=torch.float64, device="cuda")[1:]
+    _raises(lambda: da.bicgstab2(ctx, A, rhs, x_odd), "16-byte boundary")
+    _raises(lambda: da.refine_mixed(ctx, A, rhs, x_odd), "16-byte boundary")
+    _raises(lambda: da.spmv(ctx, A, x, x_odd), "16-byte boundary")
+    _raises(lambda: da.residual(ctx, A, rhs, x, x_odd), "16-byte boundary")
 
     # batch configuration and systems
     Bt = da.Batch(ctx, [A])
