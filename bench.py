@@ -383,8 +383,7 @@ def target_leg(ctx, n, steps, inner_tol=1e-4):
                 W.base_guess()
                 net.predict(W.grid, W.phi, W.phi_prev, W.xb, W.r0, W.rhs_scale, xg, extras=W.extras)
             elif guess == "extrapolated":
-                W.base_guess(with_residual=False)
-                xg.copy_(W.xb)
+                W.base_guess(with_residual=False, out=xg)       # formed in the solver's vector
             else:
                 xg.copy_(x_prev)
             torch.cuda.synchronize()

@@ -118,8 +118,7 @@ def evaluate(ctx, net, n=128, eval_seeds=range(5000, 5020), steps=200, tol=TOL, 
                 elif m == "previous":
                     xg.copy_(x_prev)
                 elif m == "extrapolated":
-                    W.base_guess(with_residual=False)
-                    xg.copy_(W.xb)                  # 2 phi^n - phi^{n-1}
+                    W.base_guess(with_residual=False, out=xg)   # 2 phi^n - phi^{n-1}
                 else:
                     W.base_guess()
                     net.predict(W.grid, W.phi, W.phi_prev, W.xb, W.r0, W.rhs_scale, xg, extras=W.extras)

@@ -146,24 +146,26 @@ class C4Batch:
             A.close()
 
 
-def _base_guess(W, with_residual=True):
-    """xb = 2 phi^n - phi^{n-1} in solver rows (x holds phi^n after the fill) and, with
-    with_residual, r0 = rhs - A xb."""
+def _base_guess(W, with_residual=True, out=None):
+    """xb = 2 phi^n - phi^{n-1} in solver rows (x holds phi^n after the fill) into out (default
+    W.xb) and, with with_residual, r0 = rhs - A xb."""
     from . import residual
     g = W.grid.ghost
     n = W.n
+    xb = W.xb if out is None else out
     prev_int = W.phi_prev.view(n + 2 * g, n + 2 * g, n + 2 * g)[g:-g, g:-g, g:-g]
     # one elementwise pass over x and the interior view of phi^{n-1}: lerp(p, x, 2) = x + (x - p)
-    _torch().lerp(prev_int, W.x.view(n, n, n), 2.0, out=W.xb.view(n, n, n))
+    _torch().lerp(prev_int, W.x.view(n, n, n), 2.0, out=xb.view(n, n, n))
     if with_residual:
-        residual(W.ctx, W.A, W.rhs, W.xb, W.r0)
+        residual(W.ctx, W.A, W.rhs, xb, W.r0)
 
 
 class _BaseGuess:
-    def base_guess(self, with_residual=True):
-        """the extrapolated guess xb (and its residual r0) of the assembled step; callers that
-        time a guess call this inside the timed region (assemble(base=False) skips it)"""
-        _base_guess(self, with_residual)
+    def base_guess(self, with_residual=True, out=None):
+        """the extrapolated guess xb (and its residual r0) of the assembled step, into out
+        (default self.xb); callers that time a guess call this inside the timed region
+        (assemble(base=False) skips it)"""
+        _base_guess(self, with_residual, out)
 
 
 def gaussian_sum(n, rng, count=8, ghost=2, device="cuda"):
