@@ -955,8 +955,11 @@ int dare_refine_solve_mixed(dare_ctx* ctx, const dare_csr* Ac, const double* b, 
         int it = 0, ist = 0, dummy_len = 0;
         double irn = 0.;
         bool iconv = false;
+        // an inner solve may stop after one pass once it meets its threshold when a pass costs
+        // four V-cycles (multigrid); otherwise the second pass is cheaper than the extra
+        // refinement (FP64 residual, correction, new inner start) that stopping early brings
         rc = bicgstab2_run<float, float>(ctx, op32, r32, d32, inner_threshold(inner_tol, rn, tol_abs), inner_max,
-                                         nullptr, 0, &dummy_len, &it, &irn, &ist, &iconv, work, 2,
+                                         nullptr, 0, &dummy_len, &it, &irn, &ist, &iconv, work, use_mg ? 1 : 2,
                                          true, W.st, ctx->ws, mg);   // d32 = 0 to start
         if (rc) return rc;
         total_it += it;
