@@ -2,7 +2,8 @@
 
 ctypes wrapper over oracle/liboracle.so (oracle.cpp, a C++ restatement of the reference) and,
 where it was built, oracle/_ref/libdare_ref.so (the reference's own standalone headers). Only
-tests/, __graft_entry__.smoke() and bench.py's cpu_baseline use this package, as the checker.
+tests/ and __graft_entry__.smoke() (to check the HIP results against it) and bench.py's
+cpu_baseline (to time it on the host) use this package.
 """
 import contextlib
 import ctypes

@@ -3,8 +3,9 @@
  *
  * A plain C++ restatement of hpc-dave/dare v0.3.1 (the reference), written so that the
  * floating-point operation order of every stencil and assembly term is the one the
- * reference executes; citations are reference file:line. Only tests/, smoke() and the
- * CPU-baseline leg of bench.py use it, as the checker. The shipped library never does.
+ * reference executes; citations are reference file:line. Only tests/ and smoke() (to check
+ * the HIP results against it) and the CPU-baseline leg of bench.py (to time it) use it. The
+ * shipped library never does.
  *
  * Parity pins (which check fixes each part; see tests/test_oracle.py):
  *   limiters, Diffusion_N0D1 ........ reference headers compiled into oracle/_ref (exact)

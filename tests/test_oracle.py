@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.cpp) before the GPU results are checked against it.
 
 Each check is one the reference itself holds or one the mathematics fixes:
   * the reference's own headers compiled into oracle/_ref (limiters, Diffusion_N0D1, the
