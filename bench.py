@@ -54,7 +54,7 @@ def log(msg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
+        print(f"bench.py runs on one GPU only: this implementation has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -523,7 +523,7 @@ public:
                      const std::shared_ptr<DeviceCsr>& A, const std::shared_ptr<DeviceVector>& x,
                      const std::shared_ptr<DeviceVector>& B, const std::shared_ptr<ParameterList>& params) {
         if (pack != SolverPackage::BumbleBee || type != "BICGSTAB2")
-            throw std::runtime_error("this build provides BumbleBee/BICGSTAB2 only (got '" + type + "')");
+            throw std::runtime_error("this library provides BumbleBee/BICGSTAB2 only (got '" + type + "')");
         dare_solver_params p{};
         p.precond = M ? M->precond : 0;      // right preconditioner (BiCGStab2.inl:353-365)
         p.tol = params ? params->get<double>("Convergence Tolerance", 1e-8) : 1e-8;   // BiCGStab2.inl:29-30
@@ -542,7 +542,7 @@ public:
         return Solve(pack, type, nullptr, A, x, B, params);
     }
     // TrilinosSolver.inl:97-127 + CreatePreconditionerIfPack2 (:164-175) and
-    // CreatePreconditionerMueLu (:180-246). This build provides
+    // CreatePreconditionerMueLu (:180-246). This library provides
     //  - Ifpack2 "RELAXATION" of type "Jacobi" with one sweep and damping 1, i.e.
     //    M^-1 = diag(A)^-1 (any operator);
     //  - MueLu "AMG": the aggregation multigrid V-cycle of the device solver (2x2x2 aggregates,
@@ -564,7 +564,7 @@ public:
         if (pack == PreCondPackage::Ifpack2 && type == "RELAXATION" && relax == "Jacobi" && sweeps == 1 && omega == 1.)
             return std::make_shared<Preconditioner>(Preconditioner{DARE_PRECOND_JACOBI});
         throw std::runtime_error("Preconditioner of type " + type +
-                                 " is unknown to this build (Ifpack2 RELAXATION / Jacobi, one sweep, damping 1;"
+                                 " is unknown to this library (Ifpack2 RELAXATION / Jacobi, one sweep, damping 1;"
                                  " MueLu AMG)");
     }
     int GetNumIterations() const { return num_iter_; }      // TrilinosSolver.inl:263-265

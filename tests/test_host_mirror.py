@@ -1,6 +1,6 @@
 """C++ host mirror of the reference's call surface (dare_amd/include/dare).
 
-The driver tests/cpp/test_host_mirror.cpp restates reference tests on the mirror:
+The C++ program tests/cpp/test_host_mirror.cpp restates reference tests on the mirror:
 MatrixSystem/test/test_Trilinos.cpp:60-191, MatrixSystem/test/test_TrilinosSolver.cpp:98-205,
 test/test_integration_Cartesian_Trilinos.cpp:82-160 and MPI/test/test_HaloBuffer.cpp:33-115,
 plus a 3-D diffusion run on the device assembly checked against the heat kernel.
